@@ -1,0 +1,890 @@
+// ib3d_ib.hip -- ghost-cell search, list compaction, IB forcing and surface
+// forces (subSearch.f90, subForcing.f90, subStressCal1.f90/subStressCal2.f90).
+//
+// Tags, lists and stencil indices are integer functions of FP64 comparisons;
+// they are evaluated with the reference's expressions in the reference's
+// order (correctly rounded sqrt/division, no contraction) and are therefore
+// identical to the reference's.
+//
+// The reference's forcing loops run serially: a face shared by two
+// intercepted cells takes the value of the later list entry, and an image
+// stencil may read a face written earlier in the same loop.  Here the updates
+// are regrouped into dependency levels once per geometry: every update of a
+// level reads either the field as it was when the loop started or the
+// result of an update of an earlier level, and the last writer of each face
+// stores its value at the end.  That reproduces the serial result exactly.
+#include "ib3d_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <unordered_map>
+
+namespace ib3d {
+
+__device__ __forceinline__ int64_t lin(int i, int j, int k, int sx, int sy) {
+    return (int64_t)(i - 1) + (int64_t)sx * ((int64_t)(j - 1) + (int64_t)sy * (k - 1));
+}
+
+// subSearch.f90:28-35
+__global__ void k_shift_nodes(const double *x, const double *y, const double *z, double *x1, double *y1,
+                              double *z1, int64_t n, double xs, double ys, double zs) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    x1[q] = x[q] + xs;
+    y1[q] = y[q] + ys;
+    z1[q] = z[q] + zs;
+}
+
+// subSearch.f90:95-125 (node ids are 1-based)
+__global__ void k_surface_norm(const double *xn, const double *yn, const double *zn, const int *elp, int64_t ne,
+                               double inor, double *xc, double *yc, double *zc, double *ca, double *cb, double *cg,
+                               double *a3, double *b3, double *g3) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= ne) return;
+    const int e1 = elp[n] - 1, e2 = elp[ne + n] - 1, e3 = elp[2 * ne + n] - 1;
+    const double p1x = xn[e1], p1y = yn[e1], p1z = zn[e1];
+    const double p2x = xn[e2], p2y = yn[e2], p2z = zn[e2];
+    const double p3x = xn[e3], p3y = yn[e3], p3z = zn[e3];
+    xc[n] = ((p2x + p1x) + p3x) / 3.0;
+    yc[n] = ((p2y + p1y) + p3y) / 3.0;
+    zc[n] = ((p2z + p1z) + p3z) / 3.0;
+    const double a = (p2y - p1y) * (p3z - p1z) - (p3y - p1y) * (p2z - p1z);
+    const double b = (p2z - p1z) * (p3x - p1x) - (p3z - p1z) * (p2x - p1x);
+    const double g = (p2x - p1x) * (p3y - p1y) - (p3x - p1x) * (p2y - p1y);
+    a3[n] = a;
+    b3[n] = b;
+    g3[n] = g;
+    const double len = sqrt(((a * a) + (b * b)) + (g * g));
+    ca[n] = (a / len) * inor;
+    cb[n] = (b / len) * inor;
+    cg[n] = (g / len) * inor;
+}
+
+// subSearch.f90:166-247: nearest centroid (first index wins ties of the
+// sqrt distance) and the 8-corner plane test.  sqrt is only evaluated when
+// the squared distance is below the current minimum's: sqrt is monotone, so
+// a larger square can never give a strictly smaller distance.
+static constexpr int kTagTile = 256;
+__global__ void __launch_bounds__(kTagTile)
+k_tagging(int i0, int i1, int j0, int j1, int k0, int k1, int sx, int sy, const double *__restrict__ xp,
+          const double *__restrict__ yp, const double *__restrict__ zp, const double *__restrict__ x1,
+          const double *__restrict__ y1, const double *__restrict__ z1, const double *__restrict__ xc,
+          const double *__restrict__ yc, const double *__restrict__ zc, const double *__restrict__ ca,
+          const double *__restrict__ cb, const double *__restrict__ cg, int ne, int8_t *tag, int *minElem,
+          int *err) {
+    __shared__ double sx_[kTagTile], sy_[kTagTile], sz_[kTagTile];
+    const int wi = i1 - i0 + 1, wj = j1 - j0 + 1, wk = k1 - k0 + 1;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = q < (int64_t)wi * wj * wk;
+    int i = 0, j = 0, k = 0;
+    if (active) {
+        i = i0 + (int)(q % wi);
+        const int64_t r = q / wi;
+        j = j0 + (int)(r % wj);
+        k = k0 + (int)(r / wj);
+    }
+    const double n1x = active ? xp[i] : 0.0, n1y = active ? yp[j] : 0.0, n1z = active ? zp[k] : 0.0;
+    double minDis = 1e14, smin = __builtin_huge_val();
+    int nel = 0;
+    for (int base = 0; base < ne; base += kTagTile) {
+        const int m = base + threadIdx.x;
+        __syncthreads();
+        if (m < ne) {
+            sx_[threadIdx.x] = xc[m];
+            sy_[threadIdx.x] = yc[m];
+            sz_[threadIdx.x] = zc[m];
+        }
+        __syncthreads();
+        const int cnt = min(kTagTile, ne - base);
+        for (int t = 0; t < cnt; t++) {
+            const double dy = n1y - sy_[t], dx = n1x - sx_[t], dz = n1z - sz_[t];
+            const double s = ((dy * dy) + (dx * dx)) + (dz * dz);
+            if (s < smin) {
+                const double dis = sqrt(s);
+                if (dis < minDis) {
+                    minDis = dis;
+                    smin = s;
+                    nel = base + t + 1;
+                }
+            }
+        }
+    }
+    if (!active) return;
+    if (nel < 1) {
+        atomicOr(err, 1);
+        return;
+    }
+    const int e = nel - 1;
+    const double cx = xc[e], cy = yc[e], cz = zc[e], a = ca[e], b = cb[e], g = cg[e];
+    const double n2x = x1[i], n3x = x1[i + 1], n2y = y1[j], n3y = y1[j + 1], n2z = z1[k], n3z = z1[k + 1];
+    double d[8];
+    d[0] = ((n2x - cx) * a + (n2y - cy) * b) + (n2z - cz) * g;
+    d[1] = ((n2x - cx) * a + (n3y - cy) * b) + (n2z - cz) * g;
+    d[2] = ((n3x - cx) * a + (n2y - cy) * b) + (n2z - cz) * g;
+    d[3] = ((n3x - cx) * a + (n3y - cy) * b) + (n2z - cz) * g;
+    d[4] = ((n2x - cx) * a + (n2y - cy) * b) + (n3z - cz) * g;
+    d[5] = ((n2x - cx) * a + (n3y - cy) * b) + (n3z - cz) * g;
+    d[6] = ((n3x - cx) * a + (n2y - cy) * b) + (n3z - cz) * g;
+    d[7] = ((n3x - cx) * a + (n3y - cy) * b) + (n3z - cz) * g;
+    bool all_in = true, all_out = true;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        all_in = all_in && (d[t] <= -1e-16);
+        all_out = all_out && (d[t] > -1e-16);
+    }
+    const int64_t c = lin(i, j, k, sx, sy);
+    tag[c] = all_in ? 1 : (all_out ? 0 : 2);
+    minElem[c] = nel;
+}
+
+// per interior row (j,k): counts of fluid, solid, intercepted, red-fluid
+__global__ void k_row_counts(const int8_t *tag, int nx, int ny, int nz, int *counts) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nrows = ny * nz;
+    if (r >= nrows) return;
+    const int j = 2 + r % ny, k = 2 + r / ny;
+    const int sx = nx + 2, sy = ny + 2;
+    const int8_t *row = tag + lin(1, j, k, sx, sy);
+    int f = 0, s = 0, b = 0, red = 0;
+    for (int i = 2; i <= nx + 1; i++) {
+        const int8_t t = row[i - 1];
+        if (t == 0) {
+            f++;
+            red += (i + j + k) & 1;
+        } else if (t == 1) {
+            s++;
+        } else if (t == 2) {
+            b++;
+        }
+    }
+    counts[r] = f;
+    counts[nrows + r] = s;
+    counts[2 * nrows + r] = b;
+    counts[3 * nrows + r] = red;
+}
+
+// exclusive scan of the 4 count arrays (one block, in place), totals at the end
+__global__ void k_scan_counts(int *counts, int nrows, int *totals) {
+    __shared__ int sm[1024];
+    for (int a = 0; a < 4; a++) {
+        int *cnt = counts + (int64_t)a * nrows;
+        const int per = (nrows + blockDim.x - 1) / blockDim.x;
+        const int lo = threadIdx.x * per, hi = min(nrows, lo + per);
+        int s = 0;
+        for (int r = lo; r < hi; r++) s += cnt[r];
+        sm[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int run = 0;
+            for (int t = 0; t < (int)blockDim.x; t++) {
+                const int v = sm[t];
+                sm[t] = run;
+                run += v;
+            }
+            totals[a] = run;
+        }
+        __syncthreads();
+        int run = sm[threadIdx.x];
+        for (int r = lo; r < hi; r++) {
+            const int v = cnt[r];
+            cnt[r] = run;
+            run += v;
+        }
+        __syncthreads();
+    }
+}
+
+// order-preserving scatter into (count,3) lists, (k,j,i) order
+__global__ void k_row_scatter(const int8_t *tag, int nx, int ny, int nz, const int *offs, int *fl, int nf, int *sl,
+                              int ns, int *bl, int nb) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nrows = ny * nz;
+    if (r >= nrows) return;
+    const int j = 2 + r % ny, k = 2 + r / ny;
+    const int sx = nx + 2, sy = ny + 2;
+    const int8_t *row = tag + lin(1, j, k, sx, sy);
+    int f = offs[r], s = offs[nrows + r], b = offs[2 * nrows + r];
+    for (int i = 2; i <= nx + 1; i++) {
+        const int8_t t = row[i - 1];
+        if (t == 0) {
+            fl[f] = i; fl[nf + f] = j; fl[2 * nf + f] = k; f++;
+        } else if (t == 1) {
+            sl[s] = i; sl[ns + s] = j; sl[2 * ns + s] = k; s++;
+        } else if (t == 2) {
+            bl[b] = i; bl[nb + b] = j; bl[2 * nb + b] = k; b++;
+        }
+    }
+}
+
+// subSearch.f90:621-672
+__global__ void k_norm_distance(const int *ib, int nb, const int *minElem, int sx, int sy, const double *xp,
+                                const double *yp, const double *zp, const double *x1, const double *y1,
+                                const double *z1, const double *xc, const double *yc, const double *zc,
+                                const double *ca, const double *cb, const double *cg, int *nel, double *nd, int *err) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nb) return;
+    const int i = ib[n], j = ib[nb + n], k = ib[2 * nb + n];
+    const int e1 = minElem[lin(i, j, k, sx, sy)];
+    if (e1 < 1) {
+        atomicOr(err, 2);
+        return;
+    }
+    nel[n] = e1;
+    const int e = e1 - 1;
+    const double n1x = xp[i], n2x = x1[i], n3x = x1[i + 1];
+    const double n1y = yp[j], n2y = y1[j], n3y = y1[j + 1];
+    const double n1z = zp[k], n2z = z1[k], n3z = z1[k + 1];
+    const double cx = xc[e], cy = yc[e], cz = zc[e], a = ca[e], b = cb[e], g = cg[e];
+    nd[n] = ((n1x - cx) * a + (n1y - cy) * b) + (n1z - cz) * g;            // p
+    nd[nb + n] = ((n2x - cx) * a + (n1y - cy) * b) + (n1z - cz) * g;       // u1
+    nd[2 * nb + n] = ((n3x - cx) * a + (n1y - cy) * b) + (n1z - cz) * g;   // u2
+    nd[3 * nb + n] = ((n1x - cx) * a + (n2y - cy) * b) + (n1z - cz) * g;   // v1
+    nd[4 * nb + n] = ((n1x - cx) * a + (n3y - cy) * b) + (n1z - cz) * g;   // v2
+    nd[5 * nb + n] = ((n1x - cx) * a + (n1y - cy) * b) + (n2z - cz) * g;   // w1
+    nd[6 * nb + n] = ((n1x - cx) * a + (n1y - cy) * b) + (n3z - cz) * g;   // w2
+}
+
+// ------------------------------------------------------------------------
+// image point stencils
+// ------------------------------------------------------------------------
+// largest l in [lo,hi] with c[l] <= pos and pos < c[l+1]; -1 if none.  For
+// strictly increasing c this equals the reference's scan that keeps the
+// last matching line (subForcing.f90:156-166).
+__device__ __forceinline__ int bracket(const double *c, int lo, int hi, double pos) {
+    if (!(pos >= c[lo])) return -1;
+    int a = lo, b = hi;
+    while (a < b) {
+        const int m = (a + b + 1) >> 1;
+        if (c[m] <= pos) a = m;
+        else b = m - 1;
+    }
+    return (pos < c[a + 1]) ? a : -1;
+}
+
+struct Coords {
+    const double *x[4], *y[4], *z[4];  // per component u, v, w, p
+};
+
+__device__ __forceinline__ double lerp_ref(double a, double b, double r, double d) { return a + ((b - a) * r) / d; }
+__device__ __forceinline__ double d3(double h1, double h2, double f1, double f2, double fp) {
+    return (((h1 * h1) * f2 - (h2 * h2) * f1) + ((h2 * h2) - (h1 * h1)) * fp) / (((h1 * h2) * (h1 + h2)) + 1e-16);
+}
+
+// trilinear value and normal derivative at the image point from the 8
+// stencil values f[a + 2b + 4c] (subForcing.f90:175-217)
+__device__ void image_values(const double f[8], const double *cx, const double *cy, const double *cz, int ix,
+                             int iy, int iz, double px, double py, double pz, double na, double nb, double ng,
+                             double *fpos, double *dfdn) {
+    const double f000 = f[0], f100 = f[1], f010 = f[2], f110 = f[3];
+    const double f001 = f[4], f101 = f[5], f011 = f[6], f111 = f[7];
+    const double rx = px - cx[ix], gx = cx[ix + 1] - cx[ix];
+    const double ry = py - cy[iy], gy = cy[iy + 1] - cy[iy];
+    const double rz = pz - cz[iz], gz = cz[iz + 1] - cz[iz];
+    const double x1z1 = lerp_ref(f000, f100, rx, gx), x2z1 = lerp_ref(f010, f110, rx, gx);
+    const double x1z2 = lerp_ref(f001, f101, rx, gx), x2z2 = lerp_ref(f011, f111, rx, gx);
+    const double vz1 = lerp_ref(x1z1, x2z1, ry, gy), vz2 = lerp_ref(x1z2, x2z2, ry, gy);
+    const double vy1 = lerp_ref(x1z1, x1z2, rz, gz), vy2 = lerp_ref(x2z1, x2z2, rz, gz);
+    const double z1x1 = lerp_ref(f000, f001, rz, gz), z2x1 = lerp_ref(f010, f011, rz, gz);
+    const double z1x2 = lerp_ref(f100, f101, rz, gz), z2x2 = lerp_ref(f110, f111, rz, gz);
+    const double vx1 = lerp_ref(z1x1, z2x1, ry, gy), vx2 = lerp_ref(z1x2, z2x2, ry, gy);
+    const double vp = lerp_ref(vx1, vx2, rx, gx);
+    double h2 = fabs(cx[ix + 1] - px), h1 = fabs(cx[ix] - px);
+    const double ddx = d3(h1, h2, vx1, vx2, vp);
+    h2 = fabs(cy[iy + 1] - py);
+    h1 = fabs(cy[iy] - py);
+    const double ddy = d3(h1, h2, vy1, vy2, vp);
+    h2 = fabs(cz[iz + 1] - pz);
+    h1 = fabs(cz[iz] - pz);
+    const double ddz = d3(h1, h2, vz1, vz2, vp);
+    *fpos = vp;
+    *dfdn = (ddx * na + ddy * nb) + ddz * ng;
+}
+
+// update kinds, in the reference's per-cell order: u2 u1 v2 v1 w2 w1, then p
+__constant__ int c_comp[7] = {0, 0, 1, 1, 2, 2, 3};
+
+struct StencilArgs {
+    const int *ib;
+    int nb;
+    const int *nel;
+    const double *nd;  // [7][nb]: p u1 u2 v1 v2 w1 w2
+    const double *dx, *dy, *dz;
+    const double *ca, *cb, *cg;
+    Coords C;
+    int nx, ny, nz;
+};
+
+// One thread per (cell, kind): image point, bracketing grid lines with the
+// reference's search ranges and clamps (subForcing.f90:148-173 and the five
+// sibling blocks; pressure: 28-46).  Output: idx[3], pos[3], pt1, sur, nel,
+// write location.
+__global__ void k_force_stencil(StencilArgs A, int *oi, double *od, int *err) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 7 * A.nb) return;
+    const int n = q / 7, kind = q % 7;
+    const int i = A.ib[n], j = A.ib[A.nb + n], k = A.ib[2 * A.nb + n];
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    // normal-distance slot of each kind and the face node
+    const int ndslot[7] = {2, 1, 4, 3, 6, 5, 0};
+    const double sur = A.nd[ndslot[kind] * A.nb + n];
+    const int e = A.nel[n] - 1;
+    const double na = A.ca[e], nb_ = A.cb[e], ng = A.cg[e];
+    const int comp = c_comp[kind];
+    const double *cx = A.C.x[comp], *cy = A.C.y[comp], *cz = A.C.z[comp];
+    const double pt1 = 1.5 * sqrt(((A.dx[i] * A.dx[i]) + (A.dy[j] * A.dy[j])) + (A.dz[k] * A.dz[k])) +
+                       (fabs(sur) - sur) * 0.5;
+    int fi = i, fj = j, fk = k;           // face node index on its own grid
+    int hx = nx + 1, hy = ny + 1, hz = nz + 1;
+    int clamp_axis = -1, from = 0, to = 0;
+    int wi = i, wj = j, wk = k;           // written location
+    switch (kind) {
+        case 0: fi = i + 1; hx = nx + 2; clamp_axis = 0; from = nx + 2; to = nx + 1; break;          // u2
+        case 1: fi = i; clamp_axis = 0; from = 1; to = 2; wi = i - 1; break;                          // u1
+        case 2: fj = j + 1; hy = ny + 2; clamp_axis = 1; from = ny + 2; to = ny + 1; break;          // v2
+        case 3: fj = j; clamp_axis = 1; from = 1; to = 2; wj = j - 1; break;                          // v1
+        case 4: fk = k + 1; hz = nz + 2; clamp_axis = 2; from = nz + 2; to = nz + 1; break;          // w2
+        case 5: fk = k; clamp_axis = 2; from = 1; to = 2; wk = k - 1; break;                          // w1
+        default: break;                                                                               // p
+    }
+    const double px = cx[fi] + pt1 * na, py = cy[fj] + pt1 * nb_, pz = cz[fk] + pt1 * ng;
+    int idx[3] = {bracket(cx, 1, hx, px), bracket(cy, 1, hy, py), bracket(cz, 1, hz, pz)};
+    if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0) atomicOr(err, 4);
+    if (clamp_axis >= 0 && idx[clamp_axis] == from) idx[clamp_axis] = to;
+    // every stencil corner must lie inside the padded grid (the reference
+    // would read outside its arrays otherwise); such a geometry is rejected
+    const int ax = idx[0] - (comp == 0), ay = idx[1] - (comp == 1), az = idx[2] - (comp == 2);
+    if (ax < 1 || ax + 1 > nx + 2 || ay < 1 || ay + 1 > ny + 2 || az < 1 || az + 1 > nz + 2) {
+        atomicOr(err, 16);
+        idx[0] = idx[1] = idx[2] = 1;
+    }
+    int *o = oi + 8 * (int64_t)q;
+    o[0] = comp;
+    o[1] = (int)lin(wi, wj, wk, nx + 2, ny + 2);
+    o[2] = idx[0];
+    o[3] = idx[1];
+    o[4] = idx[2];
+    o[5] = e;
+    o[6] = kind;
+    o[7] = n;
+    double *d = od + 5 * (int64_t)q;
+    d[0] = px;
+    d[1] = py;
+    d[2] = pz;
+    d[3] = pt1;
+    d[4] = sur;
+}
+
+struct LevelArgs {
+    const int *oi;       // [8] per update, level-sorted
+    const double *od;    // [5] per update
+    const int *srcs;     // [8] per update: -1 = field, else level-sorted update id
+    double *res;
+    double *F[4];        // ut, vt, wt, p
+    const double *ca, *cb, *cg;
+    Coords C;
+    int sx, sy;
+};
+
+// subForcing.f90:219-224 (velocity, u_wall = 0) and 92-98 (pressure,
+// dp/dn = 0) for the updates [lo, hi) of one dependency level
+__global__ void k_force_level(LevelArgs A, int lo, int hi) {
+    const int q = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= hi) return;
+    const int *o = A.oi + 8 * (int64_t)q;
+    const double *d = A.od + 5 * (int64_t)q;
+    const int comp = o[0], ix = o[2], iy = o[3], iz = o[4], e = o[5];
+    const int ox = comp == 0, oy = comp == 1, oz = comp == 2;
+    const int ax = ix - ox, ay = iy - oy, az = iz - oz;
+    const double *Fd = A.F[comp];
+    double f[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int s = A.srcs[8 * (int64_t)q + t];
+        f[t] = s >= 0 ? A.res[s] : Fd[lin(ax + (t & 1), ay + ((t >> 1) & 1), az + (t >> 2), A.sx, A.sy)];
+    }
+    const double na = A.ca[e], nb = A.cb[e], ng = A.cg[e];
+    double fpos, dn;
+    image_values(f, A.C.x[comp], A.C.y[comp], A.C.z[comp], ix, iy, iz, d[0], d[1], d[2], na, nb, ng, &fpos, &dn);
+    const double pt1 = d[3], sur = d[4];
+    const double n1 = pt1 + sur;
+    double val;
+    if (comp < 3) {
+        const double surf = 0.0;
+        const double cval = surf;
+        const double bval = (2.0 / n1) * (fpos - surf) - dn;
+        const double aval = dn / n1 - (fpos - surf) / (n1 * n1);
+        val = ((aval * (sur * sur)) + bval * sur) + cval;
+    } else {
+        const double dpdn = 0.0;
+        const double bval = dpdn;
+        const double aval = (dn - dpdn) / (2.0 * n1);
+        const double cval = fpos - ((dn + dpdn) * n1) * 0.5;
+        val = ((aval * (sur * sur)) + bval * sur) + cval;
+    }
+    A.res[q] = val;
+}
+
+__global__ void k_force_final(const int *loc, const int *upd, const int *comp, int n, const double *res,
+                              double *f0, double *f1, double *f2, double *f3) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double *F = comp[q] == 0 ? f0 : comp[q] == 1 ? f1 : comp[q] == 2 ? f2 : f3;
+    F[loc[q]] = res[upd[q]];
+}
+
+// ------------------------------------------------------------------------
+// surface forces (subStressCal1.f90:85-437, subStressCal2.f90:78-358)
+// ------------------------------------------------------------------------
+struct StressArgs {
+    const double *xc, *yc, *zc, *ca, *cb, *cg, *a3, *b3, *g3;
+    const double *x1, *y1, *z1;
+    Coords C;
+    const double *F[4];  // ut, vt, wt, p
+    int ne, nx, ny, nz;
+    double re;
+    int *cache;          // [5][3][ne]
+    int find;            // 1: locate the stencil cells (stressCal1)
+    double *contrib;     // [7][ne]
+    int *err;
+};
+
+__global__ void k_stress(StressArgs A) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= A.ne) return;
+    const int ne = A.ne, nx = A.nx, ny = A.ny, nz = A.nz;
+    const int sx = nx + 2, sy = ny + 2;
+    int *cell = A.cache;
+    int ic, jc, kc;
+    if (A.find) {
+        ic = bracket(A.x1, 2, nx + 1, A.xc[e]);
+        jc = bracket(A.y1, 2, ny + 1, A.yc[e]);
+        kc = bracket(A.z1, 2, nz + 1, A.zc[e]);
+        if (ic < 0 || jc < 0 || kc < 0) atomicOr(A.err, 8);
+        cell[e] = ic; cell[ne + e] = jc; cell[2 * ne + e] = kc;
+    } else {
+        ic = cell[e]; jc = cell[ne + e]; kc = cell[2 * ne + e];
+    }
+    if (ic < 0 || jc < 0 || kc < 0) return;
+    const double xs = A.xc[e], ys = A.yc[e], zs = A.zc[e];
+    const double dX = A.x1[ic + 1] - A.x1[ic], dY = A.y1[jc + 1] - A.y1[jc], dZ = A.z1[kc + 1] - A.z1[kc];
+    const double diagdis = sqrt(((dX * dX) + (dY * dY)) + (dZ * dZ));
+    const double normdis = diagdis;
+    const double na = A.ca[e], nb = A.cb[e], ng = A.cg[e];
+    const double px = xs + normdis * na, py = ys + normdis * nb, pz = zs + normdis * ng;
+    double dns[3];
+    for (int q = 0; q < 4; q++) {
+        const double *cx = A.C.x[q], *cy = A.C.y[q], *cz = A.C.z[q];
+        int *cc = cell + (int64_t)(q + 1) * 3 * ne;
+        int ix, iy, iz;
+        if (A.find) {
+            ix = bracket(cx, 2, nx + 1, px);
+            iy = bracket(cy, 2, ny + 1, py);
+            iz = bracket(cz, 2, nz + 1, pz);
+            if (ix < 0 || iy < 0 || iz < 0) atomicOr(A.err, 8);
+            cc[e] = ix; cc[ne + e] = iy; cc[2 * ne + e] = iz;
+        } else {
+            ix = cc[e]; iy = cc[ne + e]; iz = cc[2 * ne + e];
+        }
+        if (ix < 0 || iy < 0 || iz < 0) return;
+        const int ax = ix - (q == 0), ay = iy - (q == 1), az = iz - (q == 2);
+        double f[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) f[t] = A.F[q][lin(ax + (t & 1), ay + ((t >> 1) & 1), az + (t >> 2), sx, sy)];
+        double fpos, dn;
+        image_values(f, cx, cy, cz, ix, iy, iz, px, py, pz, na, nb, ng, &fpos, &dn);
+        if (q < 3) {
+            const double surf = 0.0;
+            dns[q] = (2.0 / normdis) * (fpos - surf) - dn;
+        } else {
+            const double dpdn = 0.0;
+            const double cval = fpos - ((dn + dpdn) * diagdis) * 0.5;
+            const double alen = sqrt(((A.a3[e] * A.a3[e]) + (A.b3[e] * A.b3[e])) + (A.g3[e] * A.g3[e]));
+            const double area = alen / 2.0;
+            const double area_yz = 0.5 * fabs(A.a3[e]);
+            const double area_xz = 0.5 * fabs(A.b3[e]);
+            const double proj = (dns[0] * na + dns[1] * nb) + dns[2] * ng;
+            const double stx1 = dns[0] - proj * na;
+            const double sty1 = dns[1] - proj * nb;
+            const double f_surf = cval * area;
+            double *o = A.contrib;
+            o[e] = ((1.0 / A.re) * stx1) * area;       // viscous drag
+            o[ne + e] = -f_surf * na;                  // pressure drag
+            o[2 * ne + e] = ((1.0 / A.re) * sty1) * area;  // viscous lift
+            o[3 * ne + e] = -f_surf * nb;              // pressure lift
+            o[4 * ne + e] = area;
+            o[5 * ne + e] = area_xz;
+            o[6 * ne + e] = area_yz;
+        }
+    }
+}
+
+// one block per quantity, fixed summation tree (deterministic)
+__global__ void k_sum_rows(const double *v, int n, double *out) {
+    __shared__ double sm[1024];
+    const double *row = v + (int64_t)blockIdx.x * n;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) s += row[q];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
+}
+
+// ------------------------------------------------------------------ host
+static Coords coords(const Ctx &c) {
+    Coords C;
+    C.x[0] = c.xu; C.y[0] = c.yu; C.z[0] = c.zu;
+    C.x[1] = c.xv; C.y[1] = c.yv; C.z[1] = c.zv;
+    C.x[2] = c.xw; C.y[2] = c.yw; C.z[2] = c.zw;
+    C.x[3] = c.xp; C.y[3] = c.yp; C.z[3] = c.zp;
+    return C;
+}
+
+static int read_err(Ctx &c, int *dev_err) {
+    int h = 0;
+    IB3D_CHECK(hipMemcpyAsync(&h, dev_err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    IB3D_CHECK(hipStreamSynchronize(c.stream));
+    return h;
+}
+
+int shift_nodes(Ctx &c, double xs, double ys, double zs) {
+    IB3D_REQUIRE(c.ibNodes > 0, "shiftSurfaceNodesInitial: no surface mesh");
+    hipLaunchKernelGGL(k_shift_nodes, dim3(nblocks(c.ibNodes, 256)), dim3(256), 0, c.stream, c.xnode, c.ynode,
+                       c.znode, c.xnode1, c.ynode1, c.znode1, c.ibNodes, xs, ys, zs);
+    IB3D_CHECK(hipGetLastError());
+    c.norms_ready = false;
+    return 0;
+}
+
+int surface_norm(Ctx &c, int64_t inor) {
+    IB3D_REQUIRE(c.ibElems > 0, "computeSurfaceNorm: no surface mesh");
+    hipLaunchKernelGGL(k_surface_norm, dim3(nblocks(c.ibElems, 256)), dim3(256), 0, c.stream, c.xnode1, c.ynode1,
+                       c.znode1, c.elp, c.ibElems, (double)inor, c.xcent, c.ycent, c.zcent, c.cosA, c.cosB, c.cosG,
+                       c.alpha3, c.beta3, c.gamma3);
+    IB3D_CHECK(hipGetLastError());
+    c.norms_ready = true;
+    return 0;
+}
+
+static int *g_err = nullptr;
+static int dev_err(Ctx &c) {
+    if (!g_err) IB3D_CHECK(hipMalloc(&g_err, sizeof(int)));
+    IB3D_CHECK(hipMemsetAsync(g_err, 0, sizeof(int), c.stream));
+    return 0;
+}
+void ib_release() {
+    if (g_err) (void)hipFree(g_err);
+    g_err = nullptr;
+}
+
+int count_tags(Ctx &c, int32_t out[4], bool scatter);
+
+int tagging(Ctx &c, const int64_t w[6], int32_t counts[3]) {
+    IB3D_REQUIRE(c.norms_ready, "tagging: computeSurfaceNorm has not run");
+    IB3D_REQUIRE(w[0] >= 1 && w[1] <= c.nx + 1 && w[2] >= 1 && w[3] <= c.ny + 1 && w[4] >= 1 && w[5] <= c.nz + 1 &&
+                     w[0] <= w[1] && w[2] <= w[3] && w[4] <= w[5],
+                 "tagging: search window outside the grid");
+    IB3D_CHECK(hipMemsetAsync(c.tag, 0, c.P, c.stream));
+    IB3D_CHECK(hipMemsetAsync(c.minElem, 0, c.P * sizeof(int), c.stream));
+    if (dev_err(c)) return -1;
+    const int64_t nw = (w[1] - w[0] + 1) * (w[3] - w[2] + 1) * (w[5] - w[4] + 1);
+    hipLaunchKernelGGL(k_tagging, dim3(nblocks(nw, kTagTile)), dim3(kTagTile), 0, c.stream, (int)w[0], (int)w[1],
+                       (int)w[2], (int)w[3], (int)w[4], (int)w[5], (int)c.sx, (int)c.sy, c.xp, c.yp, c.zp, c.x1,
+                       c.y1, c.z1, c.xcent, c.ycent, c.zcent, c.cosA, c.cosB, c.cosG, (int)c.ibElems, c.tag,
+                       c.minElem, g_err);
+    IB3D_CHECK(hipGetLastError());
+    IB3D_REQUIRE(read_err(c, g_err) == 0, "tagging: a window cell found no element closer than 1e14");
+    int32_t t4[4];
+    if (count_tags(c, t4, false)) return -1;
+    counts[0] = t4[2];
+    counts[1] = t4[0];
+    counts[2] = t4[1];
+    c.lists_ready = false;
+    c.normdist_ready = false;
+    c.plans_ready = false;
+    return 0;
+}
+
+// counts of fluid, solid, intercepted, red-fluid; optionally build the lists
+int count_tags(Ctx &c, int32_t out[4], bool scatter) {
+    const int nrows = (int)(c.ny * c.nz);
+    int *cnt = nullptr, *tot = nullptr;
+    IB3D_CHECK(hipMalloc(&cnt, 4 * (size_t)nrows * sizeof(int)));
+    IB3D_CHECK(hipMalloc(&tot, 4 * sizeof(int)));
+    hipLaunchKernelGGL(k_row_counts, dim3(nblocks(nrows, 256)), dim3(256), 0, c.stream, c.tag, (int)c.nx, (int)c.ny,
+                       (int)c.nz, cnt);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, cnt, nrows, tot);
+    IB3D_CHECK(hipGetLastError());
+    int h[4];
+    IB3D_CHECK(hipMemcpyAsync(h, tot, sizeof h, hipMemcpyDeviceToHost, c.stream));
+    IB3D_CHECK(hipStreamSynchronize(c.stream));
+    for (int a = 0; a < 4; a++) out[a] = h[a];
+    if (scatter) {
+        if (c.fluidList) IB3D_CHECK(hipFree(c.fluidList));
+        if (c.solidList) IB3D_CHECK(hipFree(c.solidList));
+        if (c.ibList) IB3D_CHECK(hipFree(c.ibList));
+        c.fluidList = c.solidList = c.ibList = nullptr;
+        c.fluidCount = h[0];
+        c.solidCount = h[1];
+        c.ibCount = h[2];
+        c.redCount = h[3];
+        c.blackCount = h[0] - h[3];
+        IB3D_CHECK(hipMalloc(&c.fluidList, 3 * std::max<int64_t>(1, c.fluidCount) * sizeof(int)));
+        IB3D_CHECK(hipMalloc(&c.solidList, 3 * std::max<int64_t>(1, c.solidCount) * sizeof(int)));
+        IB3D_CHECK(hipMalloc(&c.ibList, 3 * std::max<int64_t>(1, c.ibCount) * sizeof(int)));
+        hipLaunchKernelGGL(k_row_scatter, dim3(nblocks(nrows, 256)), dim3(256), 0, c.stream, c.tag, (int)c.nx,
+                           (int)c.ny, (int)c.nz, cnt, c.fluidList, (int)c.fluidCount, c.solidList, (int)c.solidCount,
+                           c.ibList, (int)c.ibCount);
+        IB3D_CHECK(hipGetLastError());
+        IB3D_CHECK(hipStreamSynchronize(c.stream));
+    }
+    IB3D_CHECK(hipFree(cnt));
+    IB3D_CHECK(hipFree(tot));
+    return 0;
+}
+
+int cell_count(Ctx &c, int32_t counts[5]) {
+    int32_t t4[4];
+    if (count_tags(c, t4, true)) return -1;
+    counts[0] = (int32_t)c.ibCount;
+    counts[1] = (int32_t)c.fluidCount;
+    counts[2] = (int32_t)c.solidCount;
+    counts[3] = (int32_t)c.redCount;
+    counts[4] = (int32_t)c.blackCount;
+    c.lists_ready = true;
+    c.normdist_ready = false;
+    c.plans_ready = false;
+    return 0;
+}
+
+int norm_distance(Ctx &c) {
+    IB3D_REQUIRE(c.lists_ready, "computeNormDistance: cellCount has not run");
+    if (c.nel) IB3D_CHECK(hipFree(c.nel));
+    if (c.normdis) IB3D_CHECK(hipFree(c.normdis));
+    const int64_t nb = std::max<int64_t>(1, c.ibCount);
+    IB3D_CHECK(hipMalloc(&c.nel, nb * sizeof(int)));
+    IB3D_CHECK(hipMalloc(&c.normdis, 7 * nb * sizeof(double)));
+    if (dev_err(c)) return -1;
+    if (c.ibCount > 0)
+        hipLaunchKernelGGL(k_norm_distance, dim3(nblocks(c.ibCount, 256)), dim3(256), 0, c.stream, c.ibList,
+                           (int)c.ibCount, c.minElem, (int)c.sx, (int)c.sy, c.xp, c.yp, c.zp, c.x1, c.y1, c.z1,
+                           c.xcent, c.ycent, c.zcent, c.cosA, c.cosB, c.cosG, c.nel, c.normdis, g_err);
+    IB3D_CHECK(hipGetLastError());
+    IB3D_REQUIRE(read_err(c, g_err) == 0, "computeNormDistance: intercepted cell outside the search window");
+    c.normdist_ready = true;
+    return build_force_plans(c);
+}
+
+static void free_plan(ForcePlan &p) {
+    if (p.upd) (void)hipFree(p.upd);
+    if (p.srcs) (void)hipFree(p.srcs);
+    if (p.pos) (void)hipFree(p.pos);
+    if (p.res) (void)hipFree(p.res);
+    if (p.final_loc) (void)hipFree(p.final_loc);
+    if (p.final_upd) (void)hipFree(p.final_upd);
+    p = ForcePlan();
+}
+
+void release_plans(Ctx &c) {
+    free_plan(c.vplan);
+    free_plan(c.pplan);
+}
+
+// final-writer table is stored as [loc | upd | comp] in final_loc (3*n ints)
+static int upload_plan(Ctx &c, ForcePlan &P, const std::vector<int> &order, const std::vector<int> &oi,
+                       const std::vector<double> &od, const std::vector<int> &src, const std::vector<int> &level,
+                       const std::vector<std::pair<int, int>> &finals, const std::vector<int> &fcomp) {
+    const int n = (int)order.size();
+    std::vector<int> pos_of(oi.size() / 8, -1);
+    for (int q = 0; q < n; q++) pos_of[order[q]] = q;
+    std::vector<int> soi(8 * (size_t)std::max(1, n)), ssrc(8 * (size_t)std::max(1, n));
+    std::vector<double> sod(5 * (size_t)std::max(1, n));
+    int maxl = -1;
+    for (int q = 0; q < n; q++) {
+        const int u = order[q];
+        for (int t = 0; t < 8; t++) {
+            soi[8 * q + t] = oi[8 * (size_t)u + t];
+            const int s = src[8 * (size_t)u + t];
+            ssrc[8 * q + t] = s >= 0 ? pos_of[s] : -1;
+        }
+        for (int t = 0; t < 5; t++) sod[5 * q + t] = od[5 * (size_t)u + t];
+        maxl = std::max(maxl, level[u]);
+    }
+    P.n_updates = n;
+    P.n_levels = maxl + 1;
+    P.level_start.assign(P.n_levels + 1, 0);
+    for (int q = 0; q < n; q++) P.level_start[level[order[q]] + 1]++;
+    for (int l = 0; l < P.n_levels; l++) P.level_start[l + 1] += P.level_start[l];
+    const size_t m = std::max(1, n);
+    IB3D_CHECK(hipMalloc(&P.upd, 8 * m * sizeof(int)));
+    IB3D_CHECK(hipMalloc(&P.srcs, 8 * m * sizeof(int)));
+    IB3D_CHECK(hipMalloc(&P.pos, 5 * m * sizeof(double)));
+    IB3D_CHECK(hipMalloc(&P.res, m * sizeof(double)));
+    IB3D_CHECK(hipMemcpy(P.upd, soi.data(), 8 * m * sizeof(int), hipMemcpyHostToDevice));
+    IB3D_CHECK(hipMemcpy(P.srcs, ssrc.data(), 8 * m * sizeof(int), hipMemcpyHostToDevice));
+    IB3D_CHECK(hipMemcpy(P.pos, sod.data(), 5 * m * sizeof(double), hipMemcpyHostToDevice));
+    P.n_final = (int)finals.size();
+    std::vector<int> fin(3 * (size_t)std::max(1, P.n_final));
+    for (int q = 0; q < P.n_final; q++) {
+        fin[q] = finals[q].first;
+        fin[P.n_final + q] = pos_of[finals[q].second];
+        fin[2 * P.n_final + q] = fcomp[q];
+    }
+    IB3D_CHECK(hipMalloc(&P.final_loc, fin.size() * sizeof(int)));
+    IB3D_CHECK(hipMemcpy(P.final_loc, fin.data(), fin.size() * sizeof(int), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Dependency levels of the serial loop over `ids` (in serial order).
+static void levels(const std::vector<int> &ids, const std::vector<int> &oi, int sx, int sy, std::vector<int> &src,
+                   std::vector<int> &level, std::vector<std::pair<int, int>> &finals, std::vector<int> &fcomp) {
+    std::unordered_map<int64_t, int> last;  // (comp, loc) -> latest writer
+    last.reserve(ids.size() * 2);
+    for (int u : ids) {
+        const int *o = &oi[8 * (size_t)u];
+        const int comp = o[0], ix = o[2], iy = o[3], iz = o[4];
+        const int ax = ix - (comp == 0), ay = iy - (comp == 1), az = iz - (comp == 2);
+        int lv = 0;
+        for (int t = 0; t < 8; t++) {
+            const int64_t loc = (int64_t)(ax + (t & 1) - 1) + (int64_t)sx * ((ay + ((t >> 1) & 1) - 1) +
+                                                                             (int64_t)sy * (az + (t >> 2) - 1));
+            const int64_t key = loc * 4 + comp;
+            auto it = last.find(key);
+            const int s = it == last.end() ? -1 : it->second;
+            src[8 * (size_t)u + t] = s;
+            if (s >= 0) lv = std::max(lv, level[s] + 1);
+        }
+        level[u] = lv;
+        last[(int64_t)o[1] * 4 + comp] = u;
+    }
+    finals.clear();
+    fcomp.clear();
+    std::vector<std::pair<int64_t, int>> tmp(last.begin(), last.end());
+    std::sort(tmp.begin(), tmp.end());
+    for (auto &kv : tmp) {
+        finals.push_back({(int)(kv.first / 4), kv.second});
+        fcomp.push_back((int)(kv.first % 4));
+    }
+}
+
+int build_force_plans(Ctx &c) {
+    release_plans(c);
+    c.plans_ready = false;
+    const int nb = (int)c.ibCount;
+    if (nb == 0) {
+        c.plans_ready = true;
+        return 0;
+    }
+    int *doi = nullptr;
+    double *dod = nullptr;
+    IB3D_CHECK(hipMalloc(&doi, 8 * 7 * (size_t)nb * sizeof(int)));
+    IB3D_CHECK(hipMalloc(&dod, 5 * 7 * (size_t)nb * sizeof(double)));
+    if (dev_err(c)) return -1;
+    StencilArgs A;
+    A.ib = c.ibList; A.nb = nb; A.nel = c.nel; A.nd = c.normdis;
+    A.dx = c.dx; A.dy = c.dy; A.dz = c.dz; A.ca = c.cosA; A.cb = c.cosB; A.cg = c.cosG;
+    A.C = coords(c);
+    A.nx = (int)c.nx; A.ny = (int)c.ny; A.nz = (int)c.nz;
+    hipLaunchKernelGGL(k_force_stencil, dim3(nblocks(7 * (int64_t)nb, 256)), dim3(256), 0, c.stream, A, doi, dod,
+                       g_err);
+    IB3D_CHECK(hipGetLastError());
+    std::vector<int> oi(8 * 7 * (size_t)nb);
+    std::vector<double> od(5 * 7 * (size_t)nb);
+    IB3D_CHECK(hipMemcpyAsync(oi.data(), doi, oi.size() * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    IB3D_CHECK(hipMemcpyAsync(od.data(), dod, od.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    const int err = read_err(c, g_err);
+    IB3D_CHECK(hipFree(doi));
+    IB3D_CHECK(hipFree(dod));
+    IB3D_REQUIRE(err == 0, "forcing: an image point has no bracketing grid lines");
+    const size_t nu = 7 * (size_t)nb;
+    std::vector<int> src(8 * nu, -1), level(nu, 0);
+    std::vector<std::pair<int, int>> finals;
+    std::vector<int> fcomp;
+    // velocity: per cell u2 u1 v2 v1 w2 w1 in serial order
+    std::vector<int> vids, pids;
+    for (int n = 0; n < nb; n++) {
+        for (int kind = 0; kind < 6; kind++) vids.push_back(7 * n + kind);
+        pids.push_back(7 * n + 6);
+    }
+    levels(vids, oi, (int)c.sx, (int)c.sy, src, level, finals, fcomp);
+    std::vector<int> vorder(vids);
+    std::stable_sort(vorder.begin(), vorder.end(), [&](int a, int b) { return level[a] < level[b]; });
+    if (upload_plan(c, c.vplan, vorder, oi, od, src, level, finals, fcomp)) return -1;
+    levels(pids, oi, (int)c.sx, (int)c.sy, src, level, finals, fcomp);
+    std::vector<int> porder(pids);
+    std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) { return level[a] < level[b]; });
+    if (upload_plan(c, c.pplan, porder, oi, od, src, level, finals, fcomp)) return -1;
+    c.plans_ready = true;
+    return 0;
+}
+
+static int run_plan(Ctx &c, ForcePlan &P) {
+    if (P.n_updates == 0) return 0;
+    LevelArgs A;
+    A.oi = P.upd; A.od = P.pos; A.srcs = P.srcs; A.res = P.res;
+    A.F[0] = c.ut; A.F[1] = c.vt; A.F[2] = c.wt; A.F[3] = c.p;
+    A.ca = c.cosA; A.cb = c.cosB; A.cg = c.cosG;
+    A.C = coords(c);
+    A.sx = (int)c.sx; A.sy = (int)c.sy;
+    for (int l = 0; l < P.n_levels; l++) {
+        const int lo = P.level_start[l], hi = P.level_start[l + 1];
+        if (hi > lo)
+            hipLaunchKernelGGL(k_force_level, dim3(nblocks(hi - lo, 128)), dim3(128), 0, c.stream, A, lo, hi);
+    }
+    const int nf = P.n_final;
+    hipLaunchKernelGGL(k_force_final, dim3(nblocks(nf, 256)), dim3(256), 0, c.stream, P.final_loc,
+                       P.final_loc + nf, P.final_loc + 2 * nf, nf, P.res, c.ut, c.vt, c.wt, c.p);
+    IB3D_CHECK(hipGetLastError());
+    return 0;
+}
+
+int velocity_forcing(Ctx &c) {
+    IB3D_REQUIRE(c.plans_ready, "velocityForcing1: computeNormDistance has not run");
+    return run_plan(c, c.vplan);
+}
+
+int pressure_forcing(Ctx &c) {
+    IB3D_REQUIRE(c.plans_ready, "pressureForcing1: computeNormDistance has not run");
+    return run_plan(c, c.pplan);
+}
+
+int stress(Ctx &c, int first, double re, double coef[4]) {
+    IB3D_REQUIRE(c.norms_ready, "stressCal: computeSurfaceNorm has not run");
+    const int ne = (int)c.ibElems;
+    if (!c.scell) IB3D_CHECK(hipMalloc(&c.scell, 15 * (size_t)ne * sizeof(int)));
+    double *contrib = nullptr;
+    IB3D_CHECK(hipMalloc(&contrib, 7 * (size_t)ne * sizeof(double)));
+    if (dev_err(c)) return -1;
+    StressArgs A;
+    A.xc = c.xcent; A.yc = c.ycent; A.zc = c.zcent; A.ca = c.cosA; A.cb = c.cosB; A.cg = c.cosG;
+    A.a3 = c.alpha3; A.b3 = c.beta3; A.g3 = c.gamma3;
+    A.x1 = c.x1; A.y1 = c.y1; A.z1 = c.z1;
+    A.C = coords(c);
+    A.F[0] = c.ut; A.F[1] = c.vt; A.F[2] = c.wt; A.F[3] = c.p;
+    A.ne = ne; A.nx = (int)c.nx; A.ny = (int)c.ny; A.nz = (int)c.nz;
+    A.re = re;
+    A.cache = c.scell;
+    A.find = (first || !c.stress_ready) ? 1 : 0;
+    A.contrib = contrib;
+    A.err = g_err;
+    hipLaunchKernelGGL(k_stress, dim3(nblocks(ne, 128)), dim3(128), 0, c.stream, A);
+    hipLaunchKernelGGL(k_sum_rows, dim3(7), dim3(1024), 0, c.stream, contrib, ne, c.scalars + 8);
+    IB3D_CHECK(hipGetLastError());
+    IB3D_CHECK(hipMemcpyAsync(c.hscal + 8, c.scalars + 8, 7 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    const int err = read_err(c, g_err);
+    IB3D_CHECK(hipFree(contrib));
+    IB3D_REQUIRE(err == 0, "stressCal: a probe point lies outside the grid");
+    c.stress_ready = true;
+    const double *s = c.hscal + 8;
+    double area_Sx = 0.5 * s[5], area_Sy = 0.5 * s[6];
+    coef[0] = 2.0 * (s[0] / area_Sx);
+    coef[1] = 2.0 * (s[1] / area_Sy);
+    coef[2] = 2.0 * (s[2] / area_Sy);
+    coef[3] = 2.0 * (s[3] / area_Sx);
+    return 0;
+}
+
+}  // namespace ib3d

@@ -1,0 +1,164 @@
+// ib3d_internal.h -- device-resident state of the IB3D hot path on MI355X.
+//
+// One process-wide context mirrors the reference's `global` module
+// (modGlobal.f90:1-106): every array lives in HBM for the whole run and the
+// host only sees it through explicit upload/download calls.
+//
+// Layouts
+//   * cell-centred and face fields: Fortran order, (nx+2)(ny+2)(nz+2), i
+//     fastest, 1-based (i,j,k) -> (i-1) + sx*((j-1) + sy*(k-1)).
+//   * tags: int8 (0 fluid, 1 solid, 2 intercepted).
+//   * Poisson unknowns: colour-compacted.  A cell (ii,jj,kk) (0-based) has
+//     colour c = (ii+jj+kk)&1 (0 = the reference's "red", i+j+k odd in
+//     1-based indices) and lives at  c-array[(jj + sy*kk)*H + (ii>>1)],
+//     H = (sx+1)/2.  Each half-sweep therefore streams two contiguous arrays.
+//     The right-hand side b/dt is stored the same way; non-fluid cells hold a
+//     NaN sentinel and are never updated (their pc stays 0, as in the
+//     reference where only fluid cells are in the red/black lists).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#define IB3D_CHECK(expr)                                                     \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess) {                                              \
+            ib3d::set_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+            return -1;                                                       \
+        }                                                                    \
+    } while (0)
+
+#define IB3D_REQUIRE(cond, msg)                  \
+    do {                                         \
+        if (!(cond)) {                           \
+            ib3d::set_error(msg);                \
+            return -1;                           \
+        }                                        \
+    } while (0)
+
+namespace ib3d {
+
+void set_error(const std::string &msg);
+
+// A forcing plan: the serial image-point updates of velocityForcing1 /
+// pressureForcing1 regrouped into dependency levels.
+struct ForcePlan {
+    int n_updates = 0;
+    int n_levels = 0;
+    std::vector<int> level_start;      // host: [n_levels+1] offsets into the level-sorted order
+    int *upd = nullptr;                // device: packed per-update record (see ib3d_ib.hip)
+    int *srcs = nullptr;               // device: 8 read sources per update (-1: the field)
+    double *pos = nullptr;             // device: image point per update
+    double *res = nullptr;             // device: value produced by each update
+    int *final_loc = nullptr;          // device: write location of each final writer
+    int *final_upd = nullptr;          // device: the update that writes it last
+    int n_final = 0;
+};
+
+struct Ctx {
+    bool ready = false;
+    int64_t nx = 0, ny = 0, nz = 0, sx = 0, sy = 0, sz = 0;
+    int64_t P = 0;        // padded cell count
+    int64_t H = 0;        // half-row length of the colour arrays
+    int64_t NC = 0;       // cells per colour array = sy*sz*H
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // 1-D grid, host copies (1-based, size n+4) and device mirrors
+    std::vector<double> hx1, hy1, hz1, hdx, hdy, hdz, hxu, hyu, hzu, hxv, hyv, hzv, hxw, hyw, hzw,
+        hxp, hyp, hzp;
+    double *x1 = nullptr, *y1 = nullptr, *z1 = nullptr, *dx = nullptr, *dy = nullptr, *dz = nullptr;
+    double *xu = nullptr, *yu = nullptr, *zu = nullptr, *xv = nullptr, *yv = nullptr, *zv = nullptr;
+    double *xw = nullptr, *yw = nullptr, *zw = nullptr, *xp = nullptr, *yp = nullptr, *zp = nullptr;
+    bool grid_ready = false;
+
+    // convection tables (6 distinct sets x 12 coefficients, 1-based)
+    std::vector<double> htab[6];
+    double *tab[6] = {};
+    int64_t tn[6] = {};
+    bool coeff_ready = false;
+
+    // Laplacian weights: Acx(n,3) folded with the boundary conditions, kept
+    // host-side in the reference's layout and on the device as per-index
+    // rows ax1/ax2/ax3 over 0-based padded index (ghost entries 0)
+    std::vector<double> hAcx, hAcy, hAcz;
+    double *ax = nullptr, *ay = nullptr, *az = nullptr;   // [3][s?]
+    bool matrix_ready = false;
+
+    // fields
+    double *u = nullptr, *v = nullptr, *w = nullptr, *ut = nullptr, *vt = nullptr, *wt = nullptr;
+    double *p = nullptr, *resi_u = nullptr, *resi_v = nullptr, *resi_w = nullptr;
+    int8_t *tag = nullptr;
+    int *minElem = nullptr;
+
+    // Poisson (colour compacted)
+    double *pc[2] = {}, *bdt[2] = {};
+    float *e32[2] = {}, *r32[2] = {};
+    double *slots = nullptr;       // per-iteration max|dpc| (reference rule)
+    int64_t n_slots = 0;
+    double *partial = nullptr;     // per-block partial sums
+    int64_t n_partial = 0;
+    double *scalars = nullptr;     // small device scalars
+    double *hscal = nullptr;       // pinned host mirror
+
+    // surface mesh
+    int64_t ibNodes = 0, ibElems = 0;
+    double *xnode = nullptr, *ynode = nullptr, *znode = nullptr;
+    double *xnode1 = nullptr, *ynode1 = nullptr, *znode1 = nullptr;
+    int *elp = nullptr;            // [3][ibElems] 1-based node ids
+    double *xcent = nullptr, *ycent = nullptr, *zcent = nullptr;
+    double *cosA = nullptr, *cosB = nullptr, *cosG = nullptr;
+    double *alpha3 = nullptr, *beta3 = nullptr, *gamma3 = nullptr;
+    bool norms_ready = false;
+
+    // lists: int32 SoA [3][count], (k,j,i) order like the reference
+    int64_t ibCount = 0, fluidCount = 0, solidCount = 0, redCount = 0, blackCount = 0;
+    int *ibList = nullptr, *fluidList = nullptr, *solidList = nullptr;
+    bool lists_ready = false;
+    int *nel = nullptr;            // one element per intercepted cell (nelp = nelu1 = ... )
+    double *normdis = nullptr;     // [7][ibCount]: p, u1, u2, v1, v2, w1, w2
+    bool normdist_ready = false;
+    ForcePlan vplan, pplan;
+    bool plans_ready = false;
+
+    // stress stencil cache [5][3][ibElems]: Elemcell, ucell, vcell, wcell, pcell
+    int *scell = nullptr;
+    bool stress_ready = false;
+
+    // Poisson configuration
+    int mode = 0;                  // 0 reference rule, 1 FP64 L2, 2 mixed FP32/FP64
+    double omega = 1.99;
+    double tol = 1e-6;
+    int check = 10;                // sweeps between L2 checks (mode 1) / inner sweeps (mode 2)
+    int warm = 0;                  // 1: start from the previous pc
+    int64_t max_sweeps = 2000000;
+    int batch = 32;                // reference rule: iterations launched per host check
+    bool have_prev = false;
+
+    // statistics of the last Poisson solve
+    int64_t last_sweeps = 0;
+    int64_t last_outer = 0;
+    double last_relres = 0.0, last_r0 = 0.0;
+    double last_sor_ms = 0.0, last_poisson_ms = 0.0;
+    hipEvent_t ev[4] = {};
+};
+
+Ctx &ctx();
+
+// launch helpers
+inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+int ensure_slots(int64_t n);
+int ensure_partial(int64_t n);
+
+// modules
+int fields_init(Ctx &c);
+int poisson_solve(Ctx &c, double deltat, double eps1, int64_t *niter, double *divmax, double *derr_std);
+int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2);
+int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);
+int build_force_plans(Ctx &c);
+int velocity_bc(Ctx &c);
+
+}  // namespace ib3d

@@ -1,0 +1,238 @@
+"""ctypes binding of the HIP library (include/ib3d.h).
+
+The library is the product path: there is no CPU fallback.  If
+``lib/libib3d_hip.so`` is missing or no GPU is visible, every call raises.
+Routine names follow the reference's subroutines (main.f90:115-177).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import cases
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libib3d_hip.so")
+
+_lib = None
+
+
+class IB3DError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise IB3DError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    I, I64, D, P = C.c_int, C.c_int64, C.c_double, C.c_void_p
+    PD, PI64, PI32 = C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int32)
+    sig = {
+        "ib3d_init": [I64, I64, I64],
+        "ib3d_finalize": [],
+        "ib3d_sync": [],
+        "ib3d_upload": [C.c_char_p, P, I64],
+        "ib3d_download": [C.c_char_p, P, I64],
+        "ib3d_set_grid": [PD, PD, PD],
+        "ib3d_set_mesh": [I64, PD, PD, PD, I64, PI64, PI64, PI64],
+        "ib3d_shiftSurfaceNodesInitial": [D, D, D],
+        "ib3d_computeSurfaceNorm": [I64],
+        "ib3d_tagging": [PI64, PI32],
+        "ib3d_cellCount": [PI32],
+        "ib3d_computeNormDistance": [],
+        "ib3d_initialConditions": [],
+        "ib3d_coefficientMatrix": [],
+        "ib3d_non_uni_coeff": [],
+        "ib3d_nsMomentum": [D, D],
+        "ib3d_velocityBC": [],
+        "ib3d_solidCellBC": [],
+        "ib3d_velocityForcing1": [],
+        "ib3d_pressureForcing1": [],
+        "ib3d_computeDiv": [D, PD],
+        "ib3d_REDBLACKSOR": [D, PI64, PD, PD],
+        "ib3d_correctPressure": [],
+        "ib3d_correctVelocity": [D],
+        "ib3d_poissonSolver": [D, D, PI64, PD, PD, PD],
+        "ib3d_stressCal1": [D, PD],
+        "ib3d_stressCal2": [D, PD],
+        "ib3d_set_poisson_mode": [I, D, D, I, I, I64],
+        "ib3d_poisson_stats": [PI64, PI64, PD, PD, PD, PD],
+        "ib3d_residual_norm": [PD, PD],
+        "ib3d_step": [I, D, D, D, D, PD],
+        "ib3d_bench_sor": [I, I, D, PD],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = I
+    L.ib3d_last_error.restype = C.c_char_p
+    L.ib3d_last_error.argtypes = []
+    L.ib3d_array_count.restype = I64
+    L.ib3d_array_count.argtypes = [C.c_char_p]
+    _lib = L
+    return L
+
+
+def _ok(rc):
+    if rc != 0:
+        raise IB3DError(lib().ib3d_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+INT_ARRAYS = {"cell", "minElemcell", "fluidIndexPtr", "solidIndexPtr", "interceptedIndexPtr",
+              "redCellIndexPtr", "blackCellIndexPtr", "nelp", "nelu1", "nelu2", "nelv1", "nelv2",
+              "nelw1", "nelw2", "Elemcell", "ucell", "vcell", "wcell", "pcell"}
+
+
+class Solver:
+    """The device-resident module of one case, driven like main.f90."""
+
+    def __init__(self, nx, ny, nz):
+        self.L = lib()
+        self.n = (nx, ny, nz)
+        _ok(self.L.ib3d_init(nx, ny, nz))
+        self.first_step = True
+
+    def close(self):
+        self.L.ib3d_finalize()
+
+    # ----------------------------------------------------------- transfers
+    def download(self, name):
+        cnt = self.L.ib3d_array_count(name.encode())
+        if cnt <= 0:
+            raise IB3DError(f"array {name} is not available")
+        dt = np.int64 if name in INT_ARRAYS else np.float64
+        out = np.empty(cnt, dtype=dt)
+        _ok(self.L.ib3d_download(name.encode(), out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def field(self, name):
+        nx, ny, nz = self.n
+        return self.download(name).reshape((nx + 2, ny + 2, nz + 2), order="F")
+
+    def upload(self, name, arr):
+        dt = np.int64 if name in INT_ARRAYS else np.float64
+        a = np.ascontiguousarray(np.asarray(arr, dtype=dt).reshape(-1, order="F"))
+        _ok(self.L.ib3d_upload(name.encode(), a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    # ----------------------------------------------------------- setup
+    @classmethod
+    def from_case(cls, directory, window=None):
+        """readInput + readSurfaceMeshGmsh (host I/O) followed by the setup
+        calls of main.f90:118-128 on the device."""
+        inp = cases.read_inputdata(os.path.join(directory, "inputdata"))
+        nx, ny, nz = inp["nx"], inp["ny"], inp["nz"]
+        s = cls(nx, ny, nz)
+        s.inp = inp
+        s.re = inp["re"]
+        s.rev = 1.0 / inp["re"]
+        s.deltat = inp["dt_order"] * 1e-3
+        s.eps1 = inp["eps1"]
+        gx = cases.read_grid(os.path.join(directory, "gridx.dat"), nx + 1)
+        gy = cases.read_grid(os.path.join(directory, "gridy.dat"), ny + 1)
+        gz = cases.read_grid(os.path.join(directory, "gridz.dat"), nz + 1)
+        _ok(s.L.ib3d_set_grid(_dp(gx), _dp(gy), _dp(gz)))
+        nodes, conn = cases.read_gmsh(os.path.join(directory, "geometries", "sphere.msh"), inp["surGeoPoints"])
+        xn, yn, zn = (np.ascontiguousarray(nodes[:, q]) for q in range(3))
+        p1, p2, p3 = (np.ascontiguousarray(conn[:, q]) for q in range(3))
+        _ok(s.L.ib3d_set_mesh(len(nodes), _dp(xn), _dp(yn), _dp(zn), len(conn), _ip(p1), _ip(p2), _ip(p3)))
+        _ok(s.L.ib3d_shiftSurfaceNodesInitial(inp["xShift"], inp["yShift"], inp["zShift"]))
+        _ok(s.L.ib3d_computeSurfaceNorm(inp["inor"]))
+        win = np.array(window or [inp["i_startSearch"], inp["i_endSearch"], inp["j_startSearch"],
+                                  inp["j_endSearch"], inp["k_startSearch"], inp["k_endSearch"]], dtype=np.int64)
+        cnt3 = np.zeros(3, dtype=np.int32)
+        _ok(s.L.ib3d_tagging(_ip(win), cnt3.ctypes.data_as(C.POINTER(C.c_int32))))
+        s.tag_counts = cnt3.copy()
+        cnt5 = np.zeros(5, dtype=np.int32)
+        _ok(s.L.ib3d_cellCount(cnt5.ctypes.data_as(C.POINTER(C.c_int32))))
+        s.counts = dict(zip(["ib", "fluid", "solid", "red", "black"], cnt5.tolist()))
+        _ok(s.L.ib3d_computeNormDistance())
+        _ok(s.L.ib3d_initialConditions())
+        _ok(s.L.ib3d_coefficientMatrix())
+        _ok(s.L.ib3d_non_uni_coeff())
+        return s
+
+    # ----------------------------------------------------------- routines
+    def nsMomentum(self):
+        _ok(self.L.ib3d_nsMomentum(self.rev, self.deltat))
+
+    def velocityBC(self):
+        _ok(self.L.ib3d_velocityBC())
+
+    def solidCellBC(self):
+        _ok(self.L.ib3d_solidCellBC())
+
+    def velocityForcing1(self):
+        _ok(self.L.ib3d_velocityForcing1())
+
+    def pressureForcing1(self):
+        _ok(self.L.ib3d_pressureForcing1())
+
+    def poissonSolver(self):
+        it = C.c_int64()
+        dm, ds, ts = C.c_double(), C.c_double(), C.c_double()
+        _ok(self.L.ib3d_poissonSolver(self.deltat, self.eps1, C.byref(it), C.byref(dm), C.byref(ds), C.byref(ts)))
+        return it.value, dm.value, ds.value, ts.value
+
+    def computeDiv(self):
+        dm = C.c_double()
+        _ok(self.L.ib3d_computeDiv(self.deltat, C.byref(dm)))
+        return dm.value
+
+    def REDBLACKSOR(self, epsi):
+        it = C.c_int64()
+        d1, d2 = C.c_double(), C.c_double()
+        _ok(self.L.ib3d_REDBLACKSOR(epsi, C.byref(it), C.byref(d1), C.byref(d2)))
+        return it.value, d2.value
+
+    def stressCal(self, first):
+        out = np.zeros(4)
+        fn = self.L.ib3d_stressCal1 if first else self.L.ib3d_stressCal2
+        _ok(fn(self.re, _dp(out)))
+        return out
+
+    def set_poisson_mode(self, mode, omega=1.7, tol=1e-6, check=10, warm=False, max_sweeps=0):
+        _ok(self.L.ib3d_set_poisson_mode(int(mode), float(omega), float(tol), int(check), int(bool(warm)),
+                                         int(max_sweeps)))
+
+    def poisson_stats(self):
+        sw, out = C.c_int64(), C.c_int64()
+        rr, r0, ts, tp = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        _ok(self.L.ib3d_poisson_stats(C.byref(sw), C.byref(out), C.byref(rr), C.byref(r0), C.byref(ts),
+                                      C.byref(tp)))
+        return {"sweeps": sw.value, "outer": out.value, "relres": rr.value, "relres0": r0.value,
+                "sor_ms": ts.value, "poisson_ms": tp.value}
+
+    def residual_norm(self):
+        r, b = C.c_double(), C.c_double()
+        _ok(self.L.ib3d_residual_norm(C.byref(r), C.byref(b)))
+        return r.value, b.value
+
+    def step(self):
+        """One time step (main.f90:152-174) on the device."""
+        out = np.zeros(8)
+        _ok(self.L.ib3d_step(int(self.first_step), self.rev, self.re, self.deltat, self.eps1, _dp(out)))
+        self.first_step = False
+        return {"niter": int(out[0]), "divmax": out[1], "derr_std": out[2], "coef": out[3:7].copy(),
+                "sor_s": out[7]}
+
+    def bench_sor(self, precision, sweeps, omega):
+        ms = C.c_double()
+        _ok(self.L.ib3d_bench_sor(int(precision), int(sweeps), float(omega), C.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        _ok(self.L.ib3d_sync())

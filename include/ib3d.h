@@ -1,0 +1,145 @@
+/*
+ * ib3d.h -- C ABI of the MI355X-native IB3D hot path.
+ *
+ * The reference exposes its hot path as argument-less Fortran subroutines
+ * that share the `global` module (modGlobal.f90:1-106).  This library keeps
+ * one device-resident copy of that module per process and exports one entry
+ * point per subroutine; module scalars the subroutine would read are passed
+ * by value, host-visible results come back through pointers.  A drop-in
+ * Fortran shim (ib3d-gpu_amd/fortran/ib3d_dropin.f90) binds these with
+ * iso_c_binding under the reference's own subroutine names.
+ *
+ * Conventions
+ *   - every function returns 0 on success and -1 on error; the message is
+ *     available from ib3d_last_error();
+ *   - plain pointers and sizes only; arrays in the reference's Fortran
+ *     layout (column-major, padded (nx+2)(ny+2)(nz+2), i fastest);
+ *   - one host thread, one HIP stream; a call returns once its host-visible
+ *     outputs are valid (device work it queued may still be running until the
+ *     next synchronising call, exactly like the fields it leaves in HBM);
+ *   - nothing here routes through a CPU implementation: without a GPU the
+ *     calls fail.
+ */
+#ifndef IB3D_H
+#define IB3D_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* Allocates the device copy of the module for an nx*ny*nz interior grid
+ * (replaces allocateArrays, subAllocateArrays.f90:1-15, plus the OpenACC
+ * `declare create` mirrors, modGlobal.f90:67-105). */
+int ib3d_init(int64_t nx, int64_t ny, int64_t nz);
+int ib3d_finalize(void);
+const char *ib3d_last_error(void);
+int ib3d_sync(void);
+
+/* Host <-> device copies of a module array by its reference name, e.g.
+ * "u", "ut", "p", "pc", "cell", "x1", "deltax", "xp", "Acx",
+ * "fluidIndexPtr", "interceptedIndexPtr", "nelp", "pNormDis", "xcent",
+ * "Elemcell" ... (replaces `!$acc update host/device`, e.g. main.f90:55).
+ * `nbytes` must equal the reference array's size in its reference kind
+ * (REAL(8) / INTEGER(8)); conversions to the device layout happen inside. */
+int ib3d_upload(const char *name, const void *host, int64_t nbytes);
+int ib3d_download(const char *name, void *host, int64_t nbytes);
+/* Element count of a named array as the reference allocates it (0 if the
+ * array does not exist yet, e.g. lists before cellCount). */
+int64_t ib3d_array_count(const char *name);
+
+/* ---- setup (host I/O stays with the caller) ----------------------------- */
+/* Grid nodes as read from gridx/y/z.dat (nx+1, ny+1, nz+1 values): builds
+ * deltax.., ghost nodes and the staggered coordinates exactly as
+ * readInput does (subReadInput.f90:56-129). */
+int ib3d_set_grid(const double *gx, const double *gy, const double *gz);
+/* Surface mesh as read by readSurfaceMeshGmsh (subReadInput.f90:147-167):
+ * node coordinates and 1-based triangle connectivity. */
+int ib3d_set_mesh(int64_t n_nodes, const double *xnode, const double *ynode, const double *znode,
+                  int64_t n_elems, const int64_t *p1, const int64_t *p2, const int64_t *p3);
+
+/* subSearch.f90:2-39 */
+int ib3d_shiftSurfaceNodesInitial(double xshift, double yshift, double zshift);
+/* subSearch.f90:80-131; inor is the normal sign read from inputdata */
+int ib3d_computeSurfaceNorm(int64_t inor);
+/* subSearch.f90:134-321; window = {i_start,i_end,j_start,j_end,k_start,k_end}
+ * (1-based, inclusive); counts = {ibCellCount, fluidCellCount, solidCellCount} */
+int ib3d_tagging(const int64_t window[6], int32_t counts[3]);
+/* subSearch.f90:515-596; counts = {ib, fluid, solid, red, black} */
+int ib3d_cellCount(int32_t counts[5]);
+/* subSearch.f90:599-678 (also prepares the forcing plans) */
+int ib3d_computeNormDistance(void);
+/* subInitialConditions.f90:2-48 */
+int ib3d_initialConditions(void);
+/* subCoefficientMatrix.f90:1-59 */
+int ib3d_coefficientMatrix(void);
+/* subNavierStokes.f90:3-619 */
+int ib3d_non_uni_coeff(void);
+
+/* ---- per-step kernels --------------------------------------------------- */
+/* subNavierStokes.f90:622-1122 */
+int ib3d_nsMomentum(double rev, double deltat);
+/* subBoundaryConditions.f90:2-195 (the live branch, 137-193) */
+int ib3d_velocityBC(void);
+/* subBoundaryConditions.f90:199-217 */
+int ib3d_solidCellBC(void);
+/* subForcing.f90:107-652 */
+int ib3d_velocityForcing1(void);
+/* subForcing.f90:2-103 */
+int ib3d_pressureForcing1(void);
+/* subPcorVcor.f90:82-101: b = div(ut) on fluid cells, divmax = max|b| */
+int ib3d_computeDiv(double deltat, double *divmax);
+/* subPcorVcor.f90:148-212: REDBLACKSOR(epsi, isum, derr, derr2) with the
+ * reference rule (omega 1.99, stop when max|pc-pco| < epsi).  Continues from
+ * the current pc (poissonSolver zeroes it first). */
+int ib3d_REDBLACKSOR(double epsi, int64_t *isum, double *derr, double *derr2);
+/* subPcorVcor.f90:105-120 */
+int ib3d_correctPressure(void);
+/* subPcorVcor.f90:125-142 */
+int ib3d_correctVelocity(double deltat);
+/* subPcorVcor.f90:2-80: zero pc, computeDiv, SOR, corrections, velocityBC,
+ * steady-state check, u=ut.  sor_seconds is the SOR-only time (the
+ * reference's cpu_time bracket, subPcorVcor.f90:33-37). */
+int ib3d_poissonSolver(double deltat, double eps1, int64_t *nIterPcor, double *divmax,
+                       double *derrStdSt, double *sor_seconds);
+/* subStressCal1.f90:2-460 / subStressCal2.f90:2-381.
+ * coef = {Cd_visc, Cd_press, Cl_visc, Cl_press} */
+int ib3d_stressCal1(double re, double coef[4]);
+int ib3d_stressCal2(double re, double coef[4]);
+
+/* ---- north-star solver modes (no reference counterpart) ------------------
+ * mode 0: the reference rule (above).
+ * mode 1: FP64 RB-SOR with the given omega until ||b/dt - A pc||_2 <=
+ *         tol*||b/dt||_2, checked every `check` sweeps.
+ * mode 2: FP32 RB-SOR sweeps (`check` per outer iteration) on the defect
+ *         equation inside an FP64 defect-correction loop with the same
+ *         FP64 stopping rule.
+ * warm != 0 starts each solve from the previous step's pc instead of 0. */
+int ib3d_set_poisson_mode(int mode, double omega, double tol, int check, int warm,
+                          int64_t max_sweeps);
+/* Statistics of the last solve: sweeps (FP64-equivalent SOR sweeps),
+ * outer iterations, final and initial relative residual, SOR ms, whole
+ * poissonSolver ms. */
+int ib3d_poisson_stats(int64_t *sweeps, int64_t *outer, double *relres, double *relres0,
+                       double *sor_ms, double *poisson_ms);
+/* FP64 residual of the current pc: ||b/dt - A pc||_2 and ||b/dt||_2 */
+int ib3d_residual_norm(double *rnorm, double *bnorm);
+
+/* One whole time step on the device in the reference's order
+ * (main.f90:156-174): nsMomentum, velocityBC, solidCellBC, velocityForcing1,
+ * velocityBC, poissonSolver, pressureForcing1, stressCal (first != 0 ->
+ * stressCal1).  out = {nIterPcor, divmax, derrStdSt, Cd_visc, Cd_press,
+ * Cl_visc, Cl_press, sor_seconds}. */
+int ib3d_step(int first, double rev, double re, double deltat, double eps1, double out[8]);
+
+/* Kernel micro-benchmark used for the roofline figure: `sweeps` RB-SOR
+ * sweeps (precision 64 or 32) on the current right-hand side, timed with
+ * HIP events on the library stream.  ms_per_sweep = average duration of
+ * one red+black sweep. */
+int ib3d_bench_sor(int precision, int sweeps, double omega, double *ms_per_sweep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

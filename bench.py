@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark: steps/s and Poisson ms/step of the IB3D hot path on one MI355X.
+
+Workload: the fixed sphere at Re = 100 on the 256x192x192 stretched grid
+(h = D/40 near the body, 20480-triangle icosphere, dt = 0.002), started from
+the reference's initial condition (u = 1 on fluid cells).  A "step" is one
+full time step of the hot path (momentum predictor, boundary conditions,
+IB velocity forcing, Poisson solve + projection, IB pressure forcing,
+surface forces), all on the GPU with inputs resident in HBM.
+
+Poisson modes (same stopping rule: ||b/dt - A p||_2 <= 1e-6 ||b/dt||_2):
+  fp64  -- FP64 red-black SOR, omega 1.7, previous-step guess
+  mixed -- FP32 red-black SOR sweeps inside an FP64 defect-correction loop,
+           omega 1.7, previous-step guess
+The headline `value` is steps/s of --mode (default mixed); the other mode is
+measured in the same run and reported beside it.
+
+Roofline: the dominant kernel is the SOR half-sweep.  Its average launch
+duration is measured with HIP events on the library's stream
+(ib3d_bench_sor); algorithmic bytes per launch = 16 B (FP64) or 8 B (FP32)
+per interior cell, i.e. 32 B / 16 B per cell per red+black sweep.
+
+CPU baseline: the reference's own Fortran (built from its sources with
+amdflang -O3, oracle/_ref) timed on this host, one thread, on a bounded
+sample: setup, one predictor/forcing/projection pass and a fixed number of
+SOR iterations on the same 256x192x192 case; the per-step time is the sum of
+the stage times with the SOR part scaled to the FP64 path's measured sweeps
+per step.
+"""
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ib3d-gpu_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner):
+    s = solver_cls.from_case(case)
+    if mode == "fp64":
+        s.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=True)
+    elif mode == "mixed":
+        s.set_poisson_mode(2, omega=omega, tol=tol, check=inner, warm=True)
+    elif mode == "reference":
+        s.set_poisson_mode(0)
+    else:
+        raise SystemExit(f"unknown mode {mode}")
+    for q in range(warmup):
+        r = s.step()
+        st = s.poisson_stats()
+        log(f"[bench] {mode} warmup {q + 1}/{warmup}: sweeps={st['sweeps']} relres={st['relres']:.2e} "
+            f"poisson={st['poisson_ms']:.1f} ms Cd={r['coef'][0] + r['coef'][1]:.4f}")
+    s.sync()
+    sweeps, sor_ms, poi_ms, cds = [], [], [], []
+    t0 = time.perf_counter()
+    for q in range(steps):
+        r = s.step()
+        st = s.poisson_stats()
+        sweeps.append(st["sweeps"])
+        sor_ms.append(st["sor_ms"])
+        poi_ms.append(st["poisson_ms"])
+        cds.append(float(r["coef"][0] + r["coef"][1]))
+        log(f"[bench] {mode} step {q + 1}/{steps}: sweeps={st['sweeps']} outer={st['outer']} "
+            f"relres={st['relres']:.2e} poisson={st['poisson_ms']:.1f} ms")
+    s.sync()
+    t1 = time.perf_counter()
+    out = {
+        "mode": mode,
+        "s_total": t1 - t0,
+        "ms_per_step": 1e3 * (t1 - t0) / steps,
+        "steps_per_s": steps / (t1 - t0),
+        "sweeps_per_step": sum(sweeps) / steps,
+        "sor_ms_per_step": sum(sor_ms) / steps,
+        "poisson_ms_per_step": sum(poi_ms) / steps,
+        "cd_last": cds[-1],
+    }
+    return s, out
+
+
+def kernel_roofline(s, precision, omega, n_cells, sweeps=200):
+    s.bench_sor(precision, 5, omega)  # warm the tables and caches
+    ms = s.bench_sor(precision, sweeps, omega)
+    bytes_per_cell = 32.0 if precision == 64 else 16.0
+    launch_s = 0.5 * ms * 1e-3             # red and black launches per sweep
+    bytes_per_launch = 0.5 * bytes_per_cell * n_cells
+    achieved = bytes_per_launch / launch_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "launch_us": round(launch_s * 1e6, 3),
+            "bytes_per_launch": bytes_per_launch}
+
+
+def cpu_baseline(case, gpu_sweeps_per_step, nsweeps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    t0 = time.perf_counter()
+    subprocess.run([exe, "sample", str(nsweeps)], cwd=case, check=True, stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL, timeout=900)
+    wall = time.perf_counter() - t0
+    vals = {}
+    with open(os.path.join(case, "harness_sample.txt")) as fh:
+        for line in fh:
+            k, v = line.split()
+            vals[k] = float(v)
+    per_step = (vals["nsMomentum"] + 2 * vals["velocityBC"] + vals["solidCellBC"] + vals["velocityForcing1+BC"] +
+                vals["computeDiv"] + gpu_sweeps_per_step * vals["sor_per_sweep"] + vals["correct+BC"] +
+                vals["pressureForcing1"] + vals["stressCal1"])
+    return {"value": round(1.0 / per_step, 6), "unit": "steps/s", "cores": 1, "kind": "reference",
+            "sample": (f"reference Fortran (amdflang -O3, 1 thread) on the same 256x192x192 case: setup, one "
+                       f"predictor/forcing/projection pass and {nsweeps} SOR iterations "
+                       f"({vals['sor_per_sweep'] * 1e3:.1f} ms each), {wall:.1f} s wall; per-step time at "
+                       f"{gpu_sweeps_per_step:.0f} sweeps/step = {per_step:.2f} s")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64", "reference"])
+    ap.add_argument("--compare", default="fp64", help="second mode measured in the same run ('' to skip)")
+    ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--omega", type=float, default=1.99)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--inner", type=int, default=20)
+    ap.add_argument("--cpu-sweeps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-sor", nargs=2, type=int, metavar=("PRECISION", "SWEEPS"),
+                    help="profiling aid: set up the case, run SWEEPS RB-SOR sweeps and one copy probe, exit")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              f"(the hot path's multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+        raise SystemExit(2)
+    from ib3d_amd import cases
+    from ib3d_amd.native import Solver
+
+    cfg = cases.CONFIGS[a.config]
+    case = tempfile.mkdtemp(prefix="ib3d_bench_")
+    if a.profile_sor:
+        try:
+            cases.write_case(cfg, case)
+            s = Solver.from_case(case)
+            s.step()  # one step so the right-hand side is the physical one
+            prec, nsw = a.profile_sor
+            ms = s.bench_sor(prec, nsw, a.omega)
+            nx, ny, nz = cfg.n
+            nbytes = (8 if prec == 64 else 4) * nx * ny * nz
+            cms = s.probe_copy(nbytes)
+            print(json.dumps({"precision": prec, "sweeps": nsw, "ms_per_sweep": ms, "copy_bytes": nbytes,
+                              "copy_ms": cms}), flush=True)
+            s.close()
+        finally:
+            shutil.rmtree(case, ignore_errors=True)
+        return
+    try:
+        cases.write_case(cfg, case)
+        nx, ny, nz = cfg.n
+        n_cells = nx * ny * nz
+        t0 = time.perf_counter()
+        s, main_res = run_mode(Solver, case, a.mode, a.steps, a.warmup, a.omega, a.tol, a.inner)
+        log(f"[bench] {a.mode}: {main_res}")
+        prec = 32 if a.mode == "mixed" else 64
+        roof = kernel_roofline(s, prec, a.omega, n_cells)
+        roof64 = kernel_roofline(s, 64, a.omega, n_cells) if prec == 32 else roof
+        s.close()
+        cmp_res = None
+        if a.compare not in ("", "none") and a.compare != a.mode:
+            s2, cmp_res = run_mode(Solver, case, a.compare, a.steps, a.warmup, a.omega, a.tol, a.inner)
+            s2.close()
+            log(f"[bench] {a.compare}: {cmp_res}")
+        gpu_wall = time.perf_counter() - t0
+        fp64_sweeps = (cmp_res if (cmp_res and cmp_res["mode"] == "fp64") else main_res)["sweeps_per_step"]
+        cpu = None if a.no_cpu else cpu_baseline(case, fp64_sweeps, a.cpu_sweeps)
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "sor_traffic.json")
+        if os.path.exists(tf):
+            with open(tf) as fh:
+                t = json.load(fh)
+            traffic = t.get("fp32" if prec == 32 else "fp64")
+        roof_out = {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac")}
+        roof_out["traffic"] = traffic
+        roof_out["kernel"] = f"k_sor<{'float' if prec == 32 else 'double'}> half-sweep"
+        roof_out["launch_us"] = roof["launch_us"]
+        line = {
+            "metric": "steps/s + Poisson ms/step to 1e-6 residual: FP64 SOR vs mixed+learned guess",
+            "value": round(main_res["steps_per_s"], 4),
+            "unit": "steps/s",
+            "n_gpus": 1,
+            "steps": a.steps,
+            "warmup": a.warmup,
+            "ms_per_step": round(main_res["ms_per_step"], 3),
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "fp32+fp64" if a.mode == "mixed" else "fp64",
+            "data": "synthetic (generated grid and icosphere, reference initial condition)",
+            "config": {"workload": f"fixed sphere Re=100, {nx}x{ny}x{nz} stretched grid, "
+                                   f"{cfg.level and 20 * 4 ** cfg.level} triangles, dt={cfg.dt}",
+                       "poisson": f"{a.mode}: RB-SOR omega={a.omega}, ||r||/||b|| <= {a.tol}, "
+                                  f"{a.inner} sweeps between residual checks, previous-step guess",
+                       "parallelism": "1 GPU"},
+            "poisson_ms_per_step": round(main_res["poisson_ms_per_step"], 3),
+            "sor_ms_per_step": round(main_res["sor_ms_per_step"], 3),
+            "sweeps_per_step": round(main_res["sweeps_per_step"], 1),
+            "cd": round(main_res["cd_last"], 5),
+            "roofline": roof_out,
+            "roofline_fp64_sweep": {k: roof64[k] for k in ("achieved", "frac", "launch_us")},
+            "cpu_baseline": cpu,
+            "gpu_wall_s": round(gpu_wall, 1),
+        }
+        if cmp_res:
+            line["compare"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cmp_res.items()}
+            line["poisson_speedup_vs_" + cmp_res["mode"]] = round(
+                cmp_res["poisson_ms_per_step"] / main_res["poisson_ms_per_step"], 3)
+        print(json.dumps(line), flush=True)
+    finally:
+        shutil.rmtree(case, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main()

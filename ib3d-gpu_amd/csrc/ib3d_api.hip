@@ -25,6 +25,7 @@ int steady_copy(Ctx &c, double dt, double *derr_std);
 int sor_solve(Ctx &c, double bnorm2, double eps1);
 int residual_norm(Ctx &c, double *rnorm, double *bnorm);
 int bench_sor(Ctx &c, int precision, int sweeps, double omega, double *ms);
+int probe_copy(Ctx &c, int64_t nbytes, double *ms);
 int sor_float_tables(Ctx &c);
 void sor_release_tables();
 int check_fastdiv(int d, int64_t nmax);
@@ -287,6 +288,7 @@ static int coefficient_matrix(Ctx &c) {
 }
 
 // device rows ax[col*s + ii] = Acx(ii, col+1) for ii = 1..n, 0 at the ghosts
+// (the x rows use the padded stride sxp so 4-vectors of them stay aligned)
 static int push_matrix(Ctx &c) {
     auto rows = [](const std::vector<double> &M, int64_t n, int64_t s) {
         std::vector<double> r(3 * s, 0.0);
@@ -294,7 +296,7 @@ static int push_matrix(Ctx &c) {
             for (int64_t ii = 1; ii <= n; ii++) r[col * s + ii] = M[(ii - 1) + n * col];
         return r;
     };
-    if (upload_vec(c.ax, rows(c.hAcx, c.nx, c.sx)) || upload_vec(c.ay, rows(c.hAcy, c.ny, c.sy)) ||
+    if (upload_vec(c.ax, rows(c.hAcx, c.nx, c.sxp)) || upload_vec(c.ay, rows(c.hAcy, c.ny, c.sy)) ||
         upload_vec(c.az, rows(c.hAcz, c.nz, c.sz)))
         return -1;
     sor_release_tables();
@@ -415,8 +417,9 @@ int ib3d_init(int64_t nx, int64_t ny, int64_t nz) {
     c.nx = nx; c.ny = ny; c.nz = nz;
     c.sx = nx + 2; c.sy = ny + 2; c.sz = nz + 2;
     c.P = c.sx * c.sy * c.sz;
-    c.H = (c.sx + 1) / 2;
+    c.H = (((c.sx + 1) / 2) + 1) & ~int64_t(1);
     c.NC = c.sy * c.sz * c.H;
+    c.sxp = (c.sx + 7) & ~int64_t(7);
     IB3D_REQUIRE(c.P < (int64_t)1 << 31, "ib3d_init: grid too large for 32-bit cell indices");
     if (check_fastdiv((int)c.H, c.NC + 1) || check_fastdiv((int)c.sy, c.NC / c.H + 1)) {
         set_error("ib3d_init: fast division self-check failed");
@@ -436,7 +439,7 @@ int ib3d_init(int64_t nx, int64_t ny, int64_t nz) {
         c.tn[t] = nn[t] + 3;
         if (dalloc(&c.tab[t], 12 * c.tn[t])) return -1;
     }
-    if (dalloc(&c.ax, 3 * c.sx) || dalloc(&c.ay, 3 * c.sy) || dalloc(&c.az, 3 * c.sz)) return -1;
+    if (dalloc(&c.ax, 3 * c.sxp) || dalloc(&c.ay, 3 * c.sy) || dalloc(&c.az, 3 * c.sz)) return -1;
     double **f[] = {&c.u, &c.v, &c.w, &c.ut, &c.vt, &c.wt, &c.p, &c.resi_u, &c.resi_v, &c.resi_w};
     for (auto p : f) if (dalloc(p, c.P)) return -1;
     if (dalloc(&c.tag, c.P) || dalloc(&c.minElem, c.P)) return -1;
@@ -867,6 +870,13 @@ int ib3d_bench_sor(int precision, int sweeps, double omega, double *ms_per_sweep
     IB3D_REQUIRE(precision == 64 || precision == 32, "bench_sor: precision must be 64 or 32");
     IB3D_REQUIRE(sweeps >= 1, "bench_sor: sweeps must be >= 1");
     return bench_sor(c, precision, sweeps, omega, ms_per_sweep);
+}
+
+int ib3d_probe_copy(int64_t nbytes, double *ms) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(c.ready, "probe_copy: not initialised");
+    IB3D_REQUIRE(nbytes >= 32, "probe_copy: nbytes must be >= 32");
+    return probe_copy(c, nbytes, ms);
 }
 
 }  // extern "C"

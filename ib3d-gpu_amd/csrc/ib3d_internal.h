@@ -11,7 +11,7 @@
 //   * Poisson unknowns: colour-compacted.  A cell (ii,jj,kk) (0-based) has
 //     colour c = (ii+jj+kk)&1 (0 = the reference's "red", i+j+k odd in
 //     1-based indices) and lives at  c-array[(jj + sy*kk)*H + (ii>>1)],
-//     H = (sx+1)/2.  Each half-sweep therefore streams two contiguous arrays.
+//     H = (sx+1)/2 rounded up to even (so cell pairs are 16-byte aligned).  Each half-sweep therefore streams two contiguous arrays.
 //     The right-hand side b/dt is stored the same way; non-fluid cells hold a
 //     NaN sentinel and are never updated (their pc stays 0, as in the
 //     reference where only fluid cells are in the red/black lists).
@@ -64,6 +64,7 @@ struct Ctx {
     int64_t P = 0;        // padded cell count
     int64_t H = 0;        // half-row length of the colour arrays
     int64_t NC = 0;       // cells per colour array = sy*sz*H
+    int64_t sxp = 0;      // row stride of the x Laplacian rows (sx rounded up to 8)
     hipStream_t stream = nullptr;
     int device = 0;
 
@@ -85,7 +86,7 @@ struct Ctx {
     // host-side in the reference's layout and on the device as per-index
     // rows ax1/ax2/ax3 over 0-based padded index (ghost entries 0)
     std::vector<double> hAcx, hAcy, hAcz;
-    double *ax = nullptr, *ay = nullptr, *az = nullptr;   // [3][s?]
+    double *ax = nullptr, *ay = nullptr, *az = nullptr;   // [3][sxp], [3][sy], [3][sz]
     bool matrix_ready = false;
 
     // fields
@@ -101,6 +102,7 @@ struct Ctx {
     int64_t n_slots = 0;
     double *partial = nullptr;     // per-block partial sums
     int64_t n_partial = 0;
+    double bnorm2 = 0.0;           // ||b/dt||^2 of the last computeDiv
     double *scalars = nullptr;     // small device scalars
     double *hscal = nullptr;       // pinned host mirror
 

@@ -9,19 +9,33 @@
 // values.  max|pc-pco| is the max over all updates of |new-old|, fused into
 // the sweeps.
 //
-// HBM traffic per cell and sweep (red + black): own value read + write, the
-// other colour read once as neighbours, b/dt read once = 32 B (FP64) or
-// 16 B (FP32).  Blocks are remapped so that each XCD streams one contiguous
-// eighth of the array; y/z neighbour rows then stay in that XCD's L2.
+// Algorithmic HBM traffic per interior cell and red+black sweep: own value
+// read + write, the other colour read once as neighbours, b/dt read once =
+// 32 B (FP64) or 16 B (FP32).
+//
+// Kernels
+//   k_sor_pz  the half-sweep (FP64 and FP32).  A thread owns two neighbouring
+//             cells of one colour row and walks kZch planes in z, keeping the
+//             other colour's values below / at / above in registers (the "at"
+//             pair also supplies the x neighbours); colour rows have even
+//             length so every pair is an aligned vector access, and the
+//             coefficients are hoisted out of the z loop.
+//   k_sor     one cell per thread (kept for reference; not launched).
+//   k_resid   FP64 residual b/dt - A p over both colours (optionally of
+//             p + e), written as FP32 for the next inner solve.
+//
+// Host loops never wait on every iteration: sweeps are queued in batches and
+// each kernel first reads a device-side convergence flag of the previous
+// iteration, so a batch that overshoots does no work and changes nothing.
 #include "ib3d_internal.h"
 
-#include <chrono>
 #include <cmath>
 #include <cstring>
 
 namespace ib3d {
 
 static constexpr int kSorBlock = 256;
+static constexpr int kZch = 8;
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give XCD x a
 // contiguous range of logical blocks.
@@ -58,7 +72,7 @@ __device__ __forceinline__ void block_max_to(double v, double *dst) {
     }
 }
 
-// deterministic block sum -> partial[blockIdx]
+// deterministic block sum -> partial[slot]
 __device__ __forceinline__ void block_sum_to(double v, double *partial, int64_t slot) {
     __shared__ double sm[kSorBlock / 64];
     v = wave_sum(v);
@@ -72,6 +86,9 @@ __device__ __forceinline__ void block_sum_to(double v, double *partial, int64_t 
     }
 }
 
+// skip this launch when the previous iteration already converged
+__device__ __forceinline__ bool gated(const double *gate, double thr) { return gate && *gate < thr; }
+
 __global__ void k_sum_partials(const double *partial, int64_t n, double *out) {
     __shared__ double sm[1024];
     double s = 0.0;
@@ -83,6 +100,34 @@ __global__ void k_sum_partials(const double *partial, int64_t n, double *out) {
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = sm[0];
+}
+
+// ||r|| from block partials; rel[o] = ||r||/||b/dt||, conv[o] = 1 while
+// rel > tol (the loop continues), 0 once converged.  When gated the previous
+// iteration's values are carried forward.
+__global__ void k_finish_norm(const double *partial, int64_t n, double bn, double tol, double *rel, double *conv,
+                              const double *gate) {
+    __shared__ double sm[1024];
+    if (gate && *gate < 0.5) {
+        if (threadIdx.x == 0) {
+            rel[0] = rel[-1];
+            conv[0] = 0.0;
+        }
+        return;
+    }
+    double s = 0.0;
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) s += partial[q];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double r = bn > 0.0 ? sqrt(sm[0]) / bn : 0.0;
+        rel[0] = r;
+        conv[0] = (r > tol || r != r) ? 1.0 : 0.0;
+    }
 }
 
 // Division by a launch-invariant divisor d (2 <= d < 2^31) for numerators
@@ -115,6 +160,7 @@ __host__ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) 
 
 struct SorGeom {
     int H, NC, sy, sx, sz, plane;  // plane = H*sy
+    int sxp;                       // row stride of the x coefficient table (multiple of 8)
     FastDiv dH, dSy;
 };
 
@@ -126,24 +172,33 @@ __device__ __forceinline__ void decode(const SorGeom &g, int f, int c, int &ii, 
     ii = 2 * h + ((c + jj + kk) & 1);
 }
 
-// One colour half-sweep.  T is the storage/arithmetic type of the unknowns.
+// FP64 keeps the reference's correctly rounded division; the FP32 smoother
+// of the defect-correction path multiplies by the hardware reciprocal (its
+// rounding only perturbs the inner iterate; the FP64 outer residual decides).
+__device__ __forceinline__ double sor_div(double a, double b) { return a / b; }
+__device__ __forceinline__ float sor_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// One colour half-sweep, one cell per thread.
 // FIRST: the unknown starts from zero (own old value and, for the red
 // colour, the other colour are taken as 0 without reading them).
-template <typename T, bool TRACK, bool FIRST>
+// APPLY (with FIRST): the previous inner solve's increment still held in
+// `mine` is first added to the FP64 unknown `acc` (defect correction).
+template <typename T, bool TRACK, bool FIRST, bool APPLY>
 __global__ void __launch_bounds__(kSorBlock)
 k_sor(T *__restrict__ mine, const T *__restrict__ other, const T *__restrict__ rhs,
       const T *__restrict__ axc, const T *__restrict__ ayc, const T *__restrict__ azc, SorGeom g,
-      int colour, T omega, T omm, double *slot, const double *gate, double eps) {
-    if (gate && *gate < eps) return;  // the previous iteration already converged
+      int colour, T omega, T omm, double *slot, const double *gate, double thr, double *__restrict__ acc) {
+    if (gated(gate, thr)) return;
     const int f = xcd_block() * kSorBlock + (int)threadIdx.x;
     double d = 0.0;
     if (f < g.NC) {
         const T r = rhs[f];
         if (r == r) {  // NaN sentinel marks cells that are not updated
+            if (APPLY) acc[f] += (double)mine[f];
             int ii, jj, kk;
             decode(g, f, colour, ii, jj, kk);
             const int q = (colour + jj + kk) & 1;
-            const T a1x = axc[ii], a2x = axc[g.sx + ii], a3x = axc[2 * g.sx + ii];
+            const T a1x = axc[ii], a2x = axc[g.sxp + ii], a3x = axc[2 * g.sxp + ii];
             const T a1y = ayc[jj], a2y = ayc[g.sy + jj], a3y = ayc[2 * g.sy + jj];
             const T a1z = azc[kk], a2z = azc[g.sz + kk], a3z = azc[2 * g.sz + kk];
             T pw, pe, ps, pn, pb, pt, pold;
@@ -158,8 +213,8 @@ k_sor(T *__restrict__ mine, const T *__restrict__ other, const T *__restrict__ r
                 pt = other[f + g.plane];
             }
             pold = FIRST ? T(0) : mine[f];
-            const T gs = ((((((r - a1x * pw) - a3x * pe) - a1y * ps) - a3y * pn) - a1z * pb) - a3z * pt) /
-                         ((a2x + a2y) + a2z);
+            const T gs = sor_div(((((((r - a1x * pw) - a3x * pe) - a1y * ps) - a3y * pn) - a1z * pb) - a3z * pt),
+                                 ((a2x + a2y) + a2z));
             const T pnew = omm * pold + omega * gs;
             mine[f] = pnew;
             if (TRACK) d = (double)fabs(pnew - pold);
@@ -168,22 +223,109 @@ k_sor(T *__restrict__ mine, const T *__restrict__ other, const T *__restrict__ r
     if (TRACK) block_max_to(d, slot);
 }
 
-// FP64 residual r = b/dt - A p over both colours.  ADD: p <- p + e first
-// (e is the FP32 defect-correction increment).  Writes r as FP32 (NaN
-// sentinel elsewhere) when r32 != nullptr, and per-block partial sums of r^2.
+// Half-sweep on cell pairs marching in z (see the file comment).  Cell 0 of
+// a pair sits at ii = 2h+q, cell 1 at ii = 2h+2+q (q = row parity); their x
+// neighbours are the other colour's pair at the same h plus one value on the
+// west (q = 0) or east (q = 1) side.  The x coefficients of both parities are
+// loaded once (h is even, so 2h..2h+3 is one aligned 4-vector), the y
+// coefficients once per thread and the z coefficients are uniform per plane.
+// TRACK: max |new - old| (reference stopping rule).  FIRST: the unknown starts
+// from zero.  APPLY (with FIRST): the previous increment, still in `mine`, is
+// first added to the FP64 unknown `acc` (defect correction).
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+template <typename T> struct Vec;
+template <> struct Vec<double> { typedef d2v v2; typedef d4v v4; };
+template <> struct Vec<float> { typedef f2v v2; typedef f4v v4; };
+
+template <typename T, bool TRACK, bool FIRST, bool APPLY>
+__global__ void __launch_bounds__(kSorBlock)
+k_sor_pz(T *__restrict__ mine, const T *__restrict__ other, const T *__restrict__ rhs, const T *__restrict__ axc,
+         const T *__restrict__ ayc, const T *__restrict__ azc, SorGeom g, int colour, T omega, T omm,
+         double *slot, const double *gate, double thr, double *__restrict__ acc) {
+    typedef typename Vec<T>::v2 V2;
+    typedef typename Vec<T>::v4 V4;
+    if (gated(gate, thr)) return;
+    const int p = 2 * (int)(blockIdx.x * kSorBlock + threadIdx.x);
+    double d = 0.0;
+    const int jj = (int)fdiv((uint32_t)p, g.dH);
+    const int h = p - jj * g.H;
+    if (p < g.plane && jj >= 1 && jj <= g.sy - 2) {
+        const int k0 = 1 + (int)blockIdx.y * kZch;
+        const int k1 = min(k0 + kZch, g.sz - 1);
+        const V4 c1 = *reinterpret_cast<const V4 *>(axc + 2 * h);
+        const V4 c2 = *reinterpret_cast<const V4 *>(axc + g.sxp + 2 * h);
+        const V4 c3 = *reinterpret_cast<const V4 *>(axc + 2 * g.sxp + 2 * h);
+        const T a1y = ayc[jj], a2y = ayc[g.sy + jj], a3y = ayc[2 * g.sy + jj];
+        int f = p + k0 * g.plane;
+        const V2 zero = V2(T(0));
+        const bool no_other = FIRST && colour == 0;
+        V2 below = no_other ? zero : *reinterpret_cast<const V2 *>(other + f - g.plane);
+        V2 cur = no_other ? zero : *reinterpret_cast<const V2 *>(other + f);
+        for (int kk = k0; kk < k1; kk++, f += g.plane) {
+            const V2 above = no_other ? zero : *reinterpret_cast<const V2 *>(other + f + g.plane);
+            const V2 r2 = *reinterpret_cast<const V2 *>(rhs + f);
+            const bool ok0 = r2.x == r2.x, ok1 = r2.y == r2.y;
+            if (ok0 || ok1) {
+                const int q = (colour + jj + kk) & 1;
+                const T side = no_other ? T(0) : (q ? other[f + 2] : other[f - 1]);
+                const V2 ps = no_other ? zero : *reinterpret_cast<const V2 *>(other + f - g.H);
+                const V2 pn = no_other ? zero : *reinterpret_cast<const V2 *>(other + f + g.H);
+                const V2 po = (FIRST && !APPLY) ? zero : *reinterpret_cast<const V2 *>(mine + f);
+                if (APPLY) {
+                    if (ok0) acc[f] += (double)po.x;
+                    if (ok1) acc[f + 1] += (double)po.y;
+                }
+                const V2 pold = FIRST ? zero : po;
+                const T a1z = azc[kk], a2z = azc[g.sz + kk], a3z = azc[2 * g.sz + kk];
+                const T a1x0 = q ? c1.y : c1.x, a2x0 = q ? c2.y : c2.x, a3x0 = q ? c3.y : c3.x;
+                const T a1x1 = q ? c1.w : c1.z, a2x1 = q ? c2.w : c2.z, a3x1 = q ? c3.w : c3.z;
+                V2 out = po;
+                if (ok0) {
+                    const T pw = q ? cur.x : side, pe = q ? cur.y : cur.x;
+                    const T gs = sor_div(((((((r2.x - a1x0 * pw) - a3x0 * pe) - a1y * ps.x) - a3y * pn.x) -
+                                           a1z * below.x) - a3z * above.x),
+                                         ((a2x0 + a2y) + a2z));
+                    out.x = omm * pold.x + omega * gs;
+                    if (TRACK) d = fmax(d, (double)fabs(out.x - pold.x));
+                }
+                if (ok1) {
+                    const T pw = q ? cur.y : cur.x, pe = q ? side : cur.y;
+                    const T gs = sor_div(((((((r2.y - a1x1 * pw) - a3x1 * pe) - a1y * ps.y) - a3y * pn.y) -
+                                           a1z * below.y) - a3z * above.y),
+                                         ((a2x1 + a2y) + a2z));
+                    out.y = omm * pold.y + omega * gs;
+                    if (TRACK) d = fmax(d, (double)fabs(out.y - pold.y));
+                }
+                *reinterpret_cast<V2 *>(mine + f) = out;
+            }
+            below = cur;
+            cur = above;
+        }
+    }
+    if (TRACK) block_max_to(d, slot);
+}
+
+// FP64 residual r = b/dt - A p over both colours.  ADD: of p + e (e is the
+// FP32 defect-correction increment not yet added).  Writes r as FP32 (NaN
+// sentinel elsewhere) when r0 != nullptr, and per-block partial sums of r^2.
 template <bool ADD>
 __global__ void __launch_bounds__(kSorBlock)
-k_resid(double *__restrict__ p0, double *__restrict__ p1, const float *__restrict__ e0,
+k_resid(const double *__restrict__ p0, const double *__restrict__ p1, const float *__restrict__ e0,
         const float *__restrict__ e1, const double *__restrict__ b0, const double *__restrict__ b1,
         float *__restrict__ r0, float *__restrict__ r1, const double *__restrict__ axc,
-        const double *__restrict__ ayc, const double *__restrict__ azc, SorGeom g, double *partial) {
+        const double *__restrict__ ayc, const double *__restrict__ azc, SorGeom g, double *partial,
+        const double *gate, double thr) {
+    if (gated(gate, thr)) return;
     const int lb = xcd_block();
     const int f2 = lb * kSorBlock + (int)threadIdx.x;  // over 2*NC
     double rr = 0.0;
     if (f2 < 2 * g.NC) {
         const int colour = f2 >= g.NC;
         const int f = colour ? f2 - g.NC : f2;
-        double *pm = colour ? p1 : p0;
+        const double *pm = colour ? p1 : p0;
         const double *po = colour ? p0 : p1;
         const float *em = colour ? e1 : e0;
         const float *eo = colour ? e0 : e1;
@@ -193,7 +335,7 @@ k_resid(double *__restrict__ p0, double *__restrict__ p1, const float *__restric
             int ii, jj, kk;
             decode(g, f, colour, ii, jj, kk);
             const int q = (colour + jj + kk) & 1;
-            const int sx = g.sx, sy = g.sy, nzp = g.sz;
+            const int sx = g.sxp, sy = g.sy, nzp = g.sz;
             double pw = po[f - 1 + q], pe = po[f + q], ps = po[f - g.H], pn = po[f + g.H];
             double pb = po[f - g.plane], pt = po[f + g.plane], ps0 = pm[f];
             if (ADD) {
@@ -261,6 +403,14 @@ k_div(const double *__restrict__ ut, const double *__restrict__ vt, const double
     block_sum_to(bb, partial, blockIdx.x);
 }
 
+// plain streaming copy of known size: the reference point for converting
+// hardware fetch/write counters into bytes
+__global__ void __launch_bounds__(kSorBlock) k_copy_probe(const double4 *__restrict__ a, double4 *__restrict__ b,
+                                                          int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
+    if (q < n) b[q] = a[q];
+}
+
 // ---------------------------------------------------------------- host side
 static SorGeom geom(const Ctx &c) {
     SorGeom g;
@@ -270,6 +420,7 @@ static SorGeom geom(const Ctx &c) {
     g.sx = (int)c.sx;
     g.sz = (int)c.sz;
     g.plane = (int)(c.H * c.sy);
+    g.sxp = (int)c.sxp;
     g.dH = make_fastdiv(g.H);
     g.dSy = make_fastdiv(g.sy);
     return g;
@@ -312,7 +463,7 @@ static int64_t g_f_sx = 0, g_f_sy = 0, g_f_sz = 0;
 int sor_float_tables(Ctx &c) {
     if (g_axf && g_f_sx == c.sx && g_f_sy == c.sy && g_f_sz == c.sz) return 0;
     if (g_axf) { (void)hipFree(g_axf); (void)hipFree(g_ayf); (void)hipFree(g_azf); }
-    std::vector<double> hx(3 * c.sx), hy(3 * c.sy), hz(3 * c.sz);
+    std::vector<double> hx(3 * c.sxp), hy(3 * c.sy), hz(3 * c.sz);
     IB3D_CHECK(hipMemcpy(hx.data(), c.ax, hx.size() * 8, hipMemcpyDeviceToHost));
     IB3D_CHECK(hipMemcpy(hy.data(), c.ay, hy.size() * 8, hipMemcpyDeviceToHost));
     IB3D_CHECK(hipMemcpy(hz.data(), c.az, hz.size() * 8, hipMemcpyDeviceToHost));
@@ -346,16 +497,49 @@ int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2) {
     IB3D_CHECK(hipStreamSynchronize(c.stream));
     *divmax = c.hscal[0];
     if (bnorm2) *bnorm2 = c.hscal[1];
+    c.bnorm2 = c.hscal[1];
     return 0;
 }
 
-template <typename T, bool TRACK, bool FIRST>
-static void launch_sweep(Ctx &c, T *mine, const T *other, const T *rhs, const T *axc, const T *ayc,
-                         const T *azc, int colour, double omega, double *slot, const double *gate,
-                         double eps) {
-    const unsigned nb = nblocks(c.NC, kSorBlock);
-    hipLaunchKernelGGL((k_sor<T, TRACK, FIRST>), dim3(nb), dim3(kSorBlock), 0, c.stream, mine, other, rhs, axc,
-                       ayc, azc, geom(c), colour, (T)omega, (T)(1.0 - omega), slot, gate, eps);
+static dim3 pz_grid(const Ctx &c) {
+    return dim3(nblocks((c.H * c.sy) / 2, kSorBlock), (unsigned)((c.sz - 2 + kZch - 1) / kZch));
+}
+
+// FP64 sweep (both colours) gated on *gate < thr
+static void sweep64(Ctx &c, double omega, bool track, double *slot, const double *gate, double thr) {
+    const SorGeom g = geom(c);
+    for (int col = 0; col < 2; col++) {
+        if (track)
+            hipLaunchKernelGGL((k_sor_pz<double, true, false, false>), pz_grid(c), dim3(kSorBlock), 0, c.stream,
+                               c.pc[col], c.pc[1 - col], c.bdt[col], c.ax, c.ay, c.az, g, col, omega, 1.0 - omega,
+                               slot, gate, thr, nullptr);
+        else
+            hipLaunchKernelGGL((k_sor_pz<double, false, false, false>), pz_grid(c), dim3(kSorBlock), 0, c.stream,
+                               c.pc[col], c.pc[1 - col], c.bdt[col], c.ax, c.ay, c.az, g, col, omega, 1.0 - omega,
+                               slot, gate, thr, nullptr);
+    }
+}
+
+// FP32 sweep (both colours) on the increment e; first: start from zero,
+// apply: first add the previous increment to the FP64 pc
+static void sweep32(Ctx &c, double omega, bool first, bool apply, const double *gate, double thr) {
+    const SorGeom g = geom(c);
+    const float om = (float)omega, omm = (float)(1.0 - omega);
+    for (int col = 0; col < 2; col++) {
+        float *mine = c.e32[col];
+        const float *other = c.e32[1 - col];
+        const float *rhs = c.r32[col];
+        if (first && apply)
+            hipLaunchKernelGGL((k_sor_pz<float, false, true, true>), pz_grid(c), dim3(kSorBlock), 0, c.stream,
+                               mine, other, rhs, g_axf, g_ayf, g_azf, g, col, om, omm, nullptr, gate, thr,
+                               c.pc[col]);
+        else if (first)
+            hipLaunchKernelGGL((k_sor_pz<float, false, true, false>), pz_grid(c), dim3(kSorBlock), 0, c.stream,
+                               mine, other, rhs, g_axf, g_ayf, g_azf, g, col, om, omm, nullptr, gate, thr, nullptr);
+        else
+            hipLaunchKernelGGL((k_sor_pz<float, false, false, false>), pz_grid(c), dim3(kSorBlock), 0, c.stream,
+                               mine, other, rhs, g_axf, g_ayf, g_azf, g, col, om, omm, nullptr, gate, thr, nullptr);
+    }
 }
 
 // subPcorVcor.f90:148-212 with the reference stopping rule.  Iterations are
@@ -372,10 +556,7 @@ int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2) {
         for (int q = 0; q < batch; q++) {
             const int64_t m = done + q;
             const double *gate = m > 0 ? c.slots + m - 1 : nullptr;
-            launch_sweep<double, true, false>(c, c.pc[0], c.pc[1], c.bdt[0], c.ax, c.ay, c.az, 0, omega,
-                                              c.slots + m, gate, epsi);
-            launch_sweep<double, true, false>(c, c.pc[1], c.pc[0], c.bdt[1], c.ax, c.ay, c.az, 1, omega,
-                                              c.slots + m, gate, epsi);
+            sweep64(c, omega, true, c.slots + m, gate, epsi);
         }
         IB3D_CHECK(hipGetLastError());
         std::vector<double> h(batch);
@@ -399,85 +580,86 @@ int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2) {
     }
 }
 
-static int residual(Ctx &c, bool add, bool write_r32, double *rnorm2) {
+// residual of the current p (+e when add) into r32 (when write_r32) and
+// partial sums; then rel[o] and conv[o], gated on conv[o-1]
+static void residual_async(Ctx &c, bool add, bool write_r32, double bn, double *rel, double *conv,
+                           const double *gate) {
     const unsigned nb = nblocks(2 * c.NC, kSorBlock);
-    if (ensure_partial(nb)) return -1;
+    const SorGeom g = geom(c);
+    float *r0 = write_r32 ? c.r32[0] : nullptr, *r1 = write_r32 ? c.r32[1] : nullptr;
     if (add)
         hipLaunchKernelGGL(k_resid<true>, dim3(nb), dim3(kSorBlock), 0, c.stream, c.pc[0], c.pc[1], c.e32[0],
-                           c.e32[1], c.bdt[0], c.bdt[1], write_r32 ? c.r32[0] : nullptr,
-                           write_r32 ? c.r32[1] : nullptr, c.ax, c.ay, c.az, geom(c), c.partial);
+                           c.e32[1], c.bdt[0], c.bdt[1], r0, r1, c.ax, c.ay, c.az, g, c.partial, gate, 0.5);
     else
         hipLaunchKernelGGL(k_resid<false>, dim3(nb), dim3(kSorBlock), 0, c.stream, c.pc[0], c.pc[1], c.e32[0],
-                           c.e32[1], c.bdt[0], c.bdt[1], write_r32 ? c.r32[0] : nullptr,
-                           write_r32 ? c.r32[1] : nullptr, c.ax, c.ay, c.az, geom(c), c.partial);
-    if (add) {
-        // the residual above read p+e everywhere; now commit p <- p+e
-        hipLaunchKernelGGL(k_add_e, dim3(nblocks(2 * c.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream, c.pc[0],
-                           c.pc[1], c.e32[0], c.e32[1], c.bdt[0], c.bdt[1], c.NC);
-    }
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, c.stream, c.partial, (int64_t)nb, c.scalars + 2);
-    IB3D_CHECK(hipGetLastError());
-    IB3D_CHECK(hipMemcpyAsync(c.hscal + 2, c.scalars + 2, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    IB3D_CHECK(hipStreamSynchronize(c.stream));
-    *rnorm2 = c.hscal[2];
-    return 0;
+                           c.e32[1], c.bdt[0], c.bdt[1], r0, r1, c.ax, c.ay, c.az, g, c.partial, gate, 0.5);
+    hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, c.stream, c.partial, (int64_t)nb, bn, c.tol, rel,
+                       conv, gate);
 }
 
-// FP64 RB-SOR until the relative L2 residual reaches tol (checked every
-// c.check sweeps).
-static int sor_l2_fp64(Ctx &c, double bn, int64_t *sweeps, double *rel, double *rel0) {
-    double rn2;
-    if (residual(c, false, false, &rn2)) return -1;
-    double r = bn > 0 ? std::sqrt(rn2) / bn : 0.0;
-    *rel0 = r;
-    int64_t it = 0;
-    while (r > c.tol && it < c.max_sweeps) {
-        for (int q = 0; q < c.check; q++) {
-            launch_sweep<double, false, false>(c, c.pc[0], c.pc[1], c.bdt[0], c.ax, c.ay, c.az, 0, c.omega,
-                                               nullptr, nullptr, 0.0);
-            launch_sweep<double, false, false>(c, c.pc[1], c.pc[0], c.bdt[1], c.ax, c.ay, c.az, 1, c.omega,
-                                               nullptr, nullptr, 0.0);
+static void add_increment(Ctx &c) {
+    hipLaunchKernelGGL(k_add_e, dim3(nblocks(2 * c.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream, c.pc[0], c.pc[1],
+                       c.e32[0], c.e32[1], c.bdt[0], c.bdt[1], c.NC);
+}
+
+// FP64 or mixed L2 solve.  Outer iteration o (1, 2, ...): `check` sweeps
+// (FP64 on pc, or FP32 on the increment e from zero, the first of which also
+// adds the previous increment to pc), then the FP64 residual.  Everything of
+// iteration o is gated on conv[o-1]; the host reads rel/conv once per batch
+// of outer iterations.  Index 0 holds the initial residual.
+static int sor_l2(Ctx &c, bool mixed, double bn, int64_t *sweeps, int64_t *outer, double *rel, double *rel0) {
+    if (mixed && sor_float_tables(c)) return -1;
+    if (ensure_partial(nblocks(2 * c.NC, kSorBlock))) return -1;
+    const int64_t max_outer = std::max<int64_t>(1, c.max_sweeps / c.check);
+    if (ensure_slots(2 * (max_outer + 2))) return -1;
+    double *relv = c.slots, *conv = c.slots + c.n_slots / 2;
+    residual_async(c, false, mixed, bn, relv, conv, nullptr);
+    int64_t o = 0;     // last queued outer iteration
+    int64_t seen = 0;  // first index not yet inspected on the host
+    int batch = 4;
+    std::vector<double> hr, hc;
+    for (;;) {
+        for (int q = 0; q < batch && o < max_outer; q++) {
+            o++;
+            const double *gate = conv + (o - 1);
+            for (int s = 0; s < c.check; s++) {
+                if (mixed)
+                    sweep32(c, c.omega, s == 0, s == 0 && o > 1, gate, 0.5);
+                else
+                    sweep64(c, c.omega, false, nullptr, gate, 0.5);
+            }
+            residual_async(c, mixed, mixed, bn, relv + o, conv + o, gate);
         }
-        it += c.check;
-        if (residual(c, false, false, &rn2)) return -1;
-        r = bn > 0 ? std::sqrt(rn2) / bn : 0.0;
-    }
-    *sweeps = it;
-    *rel = r;
-    return 0;
-}
-
-// FP32 RB-SOR sweeps on A e = r inside an FP64 defect-correction loop.
-static int sor_l2_mixed(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double *rel, double *rel0) {
-    if (sor_float_tables(c)) return -1;
-    double rn2;
-    if (residual(c, false, true, &rn2)) return -1;
-    double r = bn > 0 ? std::sqrt(rn2) / bn : 0.0;
-    *rel0 = r;
-    int64_t it = 0, out = 0;
-    while (r > c.tol && it < c.max_sweeps) {
-        for (int q = 0; q < c.check; q++) {
-            if (q == 0) {
-                launch_sweep<float, false, true>(c, c.e32[0], c.e32[1], c.r32[0], g_axf, g_ayf, g_azf, 0, c.omega,
-                                                 nullptr, nullptr, 0.0);
-                launch_sweep<float, false, true>(c, c.e32[1], c.e32[0], c.r32[1], g_axf, g_ayf, g_azf, 1, c.omega,
-                                                 nullptr, nullptr, 0.0);
-            } else {
-                launch_sweep<float, false, false>(c, c.e32[0], c.e32[1], c.r32[0], g_axf, g_ayf, g_azf, 0, c.omega,
-                                                  nullptr, nullptr, 0.0);
-                launch_sweep<float, false, false>(c, c.e32[1], c.e32[0], c.r32[1], g_axf, g_ayf, g_azf, 1, c.omega,
-                                                  nullptr, nullptr, 0.0);
+        IB3D_CHECK(hipGetLastError());
+        const int64_t n = o - seen + 1;
+        hr.resize(n);
+        hc.resize(n);
+        IB3D_CHECK(hipMemcpyAsync(hr.data(), relv + seen, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        IB3D_CHECK(hipMemcpyAsync(hc.data(), conv + seen, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        IB3D_CHECK(hipStreamSynchronize(c.stream));
+        if (seen == 0) *rel0 = hr[0];
+        for (int64_t q = 0; q < n; q++) {
+            if (hc[q] < 0.5) {
+                const int64_t oc = seen + q;  // converged after oc outer iterations
+                if (mixed && oc > 0) add_increment(c);
+                IB3D_CHECK(hipGetLastError());
+                *sweeps = oc * c.check;
+                *outer = oc;
+                *rel = hr[q];
+                return 0;
             }
         }
-        it += c.check;
-        out++;
-        if (residual(c, true, true, &rn2)) return -1;
-        r = bn > 0 ? std::sqrt(rn2) / bn : 0.0;
+        if (o >= max_outer) {
+            if (mixed && o > 0) add_increment(c);
+            IB3D_CHECK(hipGetLastError());
+            *sweeps = o * c.check;
+            *outer = o;
+            *rel = hr[n - 1];
+            return 0;
+        }
+        seen = o + 1;
+        batch = std::min(batch * 2, 64);
     }
-    *sweeps = it;
-    *outer = out;
-    *rel = r;
-    return 0;
 }
 
 int sor_solve(Ctx &c, double bnorm2, double eps1) {
@@ -493,11 +675,7 @@ int sor_solve(Ctx &c, double bnorm2, double eps1) {
     }
     int64_t sw = 0, out = 0;
     double rel = 0, rel0 = 0;
-    if (c.mode == 1) {
-        if (sor_l2_fp64(c, bn, &sw, &rel, &rel0)) return -1;
-    } else {
-        if (sor_l2_mixed(c, bn, &sw, &out, &rel, &rel0)) return -1;
-    }
+    if (sor_l2(c, c.mode == 2, bn, &sw, &out, &rel, &rel0)) return -1;
     c.last_sweeps = sw;
     c.last_outer = out;
     c.last_relres = rel;
@@ -506,12 +684,41 @@ int sor_solve(Ctx &c, double bnorm2, double eps1) {
 }
 
 int residual_norm(Ctx &c, double *rnorm, double *bnorm) {
-    double rn2;
-    if (residual(c, false, false, &rn2)) return -1;
-    // ||b/dt||^2 from the residual kernel with p = 0 would need another pass;
-    // reuse the value cached by the last computeDiv
-    *rnorm = std::sqrt(rn2);
-    *bnorm = std::sqrt(c.hscal[1]);
+    const unsigned nb = nblocks(2 * c.NC, kSorBlock);
+    if (ensure_partial(nb)) return -1;
+    hipLaunchKernelGGL(k_resid<false>, dim3(nb), dim3(kSorBlock), 0, c.stream, c.pc[0], c.pc[1], c.e32[0],
+                       c.e32[1], c.bdt[0], c.bdt[1], nullptr, nullptr, c.ax, c.ay, c.az, geom(c), c.partial,
+                       nullptr, 0.0);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, c.stream, c.partial, (int64_t)nb, c.scalars + 2);
+    IB3D_CHECK(hipGetLastError());
+    IB3D_CHECK(hipMemcpyAsync(c.hscal + 2, c.scalars + 2, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    IB3D_CHECK(hipStreamSynchronize(c.stream));
+    *rnorm = std::sqrt(c.hscal[2]);
+    *bnorm = std::sqrt(c.bnorm2);
+    return 0;
+}
+
+int probe_copy(Ctx &c, int64_t nbytes, double *ms) {
+    const int64_t n = nbytes / 32;
+    double4 *a = nullptr, *b = nullptr;
+    IB3D_CHECK(hipMalloc(&a, n * 32));
+    IB3D_CHECK(hipMalloc(&b, n * 32));
+    IB3D_CHECK(hipMemsetAsync(a, 0, n * 32, c.stream));
+    hipEvent_t e0, e1;
+    IB3D_CHECK(hipEventCreate(&e0));
+    IB3D_CHECK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_copy_probe, dim3(nblocks(n, kSorBlock)), dim3(kSorBlock), 0, c.stream, a, b, n);
+    IB3D_CHECK(hipEventRecord(e0, c.stream));
+    hipLaunchKernelGGL(k_copy_probe, dim3(nblocks(n, kSorBlock)), dim3(kSorBlock), 0, c.stream, a, b, n);
+    IB3D_CHECK(hipEventRecord(e1, c.stream));
+    IB3D_CHECK(hipEventSynchronize(e1));
+    float t = 0;
+    IB3D_CHECK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    IB3D_CHECK(hipFree(a));
+    IB3D_CHECK(hipFree(b));
     return 0;
 }
 
@@ -521,22 +728,18 @@ int bench_sor(Ctx &c, int precision, int sweeps, double omega, double *ms) {
     IB3D_CHECK(hipEventCreate(&b));
     if (precision == 32) {
         if (sor_float_tables(c)) return -1;
-        double rn2;
-        if (residual(c, false, true, &rn2)) return -1;
+        const unsigned nb = nblocks(2 * c.NC, kSorBlock);
+        if (ensure_partial(nb)) return -1;
+        hipLaunchKernelGGL(k_resid<false>, dim3(nb), dim3(kSorBlock), 0, c.stream, c.pc[0], c.pc[1], c.e32[0],
+                           c.e32[1], c.bdt[0], c.bdt[1], c.r32[0], c.r32[1], c.ax, c.ay, c.az, geom(c), c.partial,
+                           nullptr, 0.0);
     }
     IB3D_CHECK(hipEventRecord(a, c.stream));
     for (int q = 0; q < sweeps; q++) {
-        if (precision == 32) {
-            launch_sweep<float, false, false>(c, c.e32[0], c.e32[1], c.r32[0], g_axf, g_ayf, g_azf, 0, omega,
-                                              nullptr, nullptr, 0.0);
-            launch_sweep<float, false, false>(c, c.e32[1], c.e32[0], c.r32[1], g_axf, g_ayf, g_azf, 1, omega,
-                                              nullptr, nullptr, 0.0);
-        } else {
-            launch_sweep<double, false, false>(c, c.pc[0], c.pc[1], c.bdt[0], c.ax, c.ay, c.az, 0, omega, nullptr,
-                                               nullptr, 0.0);
-            launch_sweep<double, false, false>(c, c.pc[1], c.pc[0], c.bdt[1], c.ax, c.ay, c.az, 1, omega, nullptr,
-                                               nullptr, 0.0);
-        }
+        if (precision == 32)
+            sweep32(c, omega, false, false, nullptr, 0.0);
+        else
+            sweep64(c, omega, false, nullptr, nullptr, 0.0);
     }
     IB3D_CHECK(hipEventRecord(b, c.stream));
     IB3D_CHECK(hipEventSynchronize(b));

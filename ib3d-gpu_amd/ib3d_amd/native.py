@@ -66,6 +66,7 @@ def lib():
         "ib3d_residual_norm": [PD, PD],
         "ib3d_step": [I, D, D, D, D, PD],
         "ib3d_bench_sor": [I, I, D, PD],
+        "ib3d_probe_copy": [I64, PD],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -232,6 +233,11 @@ class Solver:
     def bench_sor(self, precision, sweeps, omega):
         ms = C.c_double()
         _ok(self.L.ib3d_bench_sor(int(precision), int(sweeps), float(omega), C.byref(ms)))
+        return ms.value
+
+    def probe_copy(self, nbytes):
+        ms = C.c_double()
+        _ok(self.L.ib3d_probe_copy(int(nbytes), C.byref(ms)))
         return ms.value
 
     def sync(self):

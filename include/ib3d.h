@@ -138,6 +138,10 @@ int ib3d_step(int first, double rev, double re, double deltat, double eps1, doub
  * HIP events on the library stream.  ms_per_sweep = average duration of
  * one red+black sweep. */
 int ib3d_bench_sor(int precision, int sweeps, double omega, double *ms_per_sweep);
+/* Diagnostics (no reference counterpart): one streaming device copy of
+ * `nbytes` (read nbytes + write nbytes), the calibration point for turning
+ * hardware fetch/write counters into bytes; ms = its duration. */
+int ib3d_probe_copy(int64_t nbytes, double *ms);
 
 #ifdef __cplusplus
 }

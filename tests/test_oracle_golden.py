@@ -1,0 +1,135 @@
+"""The oracle against the reference's own outputs.
+
+tests/golden/*_reference.json hold SHA-256 digests of every array the
+reference (built from its sources, oracle/_ref) dumped, plus its per-step
+SOR iteration counts, divmax and Cd/Cl.  A digest match is a bitwise match.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def load(name):
+    with open(os.path.join(HERE, "golden", name)) as fh:
+        return json.load(fh)
+
+
+def dig(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def input_digests(case):
+    out = {}
+    for f in ["inputdata", "gridx.dat", "gridy.dat", "gridz.dat", os.path.join("geometries", "sphere.msh")]:
+        with open(os.path.join(case, f), "rb") as fh:
+            out[f] = hashlib.sha256(fh.read()).hexdigest()
+    return out
+
+
+SETUP = {"cell": "cell.i64", "interceptedIndexPtr": "interceptedIndexPtr.i64",
+         "fluidIndexPtr": "fluidIndexPtr.i64", "solidIndexPtr": "solidIndexPtr.i64",
+         "redCellIndexPtr": "redCellIndexPtr.i64", "blackCellIndexPtr": "blackCellIndexPtr.i64",
+         "nelp": "nelp.i64", "pNormDis": "pNormDis.f64", "u1NormDis": "u1NormDis.f64",
+         "u2NormDis": "u2NormDis.f64", "v1NormDis": "v1NormDis.f64", "v2NormDis": "v2NormDis.f64",
+         "w1NormDis": "w1NormDis.f64", "w2NormDis": "w2NormDis.f64", "Acx": "Acx.f64", "Acy": "Acy.f64",
+         "Acz": "Acz.f64", "xcent": "xcent.f64", "cosAlpha": "cosAlpha.f64", "cosBeta": "cosBeta.f64",
+         "cosGamma": "cosGamma.f64", "alpha3": "alpha3.f64", "x1": "x1.f64", "xu": "xu.f64", "yv": "yv.f64",
+         "zw": "zw.f64", "xp": "xp.f64", "deltax": "deltax.f64", "xnode1": "xnode1.f64"}
+
+
+def check_setup(o, g):
+    assert dig(o.cell_tagging) == g["arrays"]["cell_tagging.i64"]
+    assert dig(o.minElemcell) == g["arrays"]["minElemcell.i64"]
+    for name, key in SETUP.items():
+        assert dig(o.array(name)) == g["arrays"][key], name
+    nx, ny, nz = o.n
+    for setn, n in [("uu", nx), ("uv", nx), ("vv", ny), ("vu", ny), ("ww", nz), ("wu", nz)]:
+        t = o.array("tab_" + setn).reshape(12, n + 3)[:, 2:n + 1].reshape(-1)
+        assert dig(t) == g["arrays"][f"ca_{setn}.f64"], setn
+    c = o.counts()
+    for k in ["ib", "fluid", "solid", "red", "black", "ibElems"]:
+        assert c[k] == g["counts"][k], k
+
+
+def run_steps_and_check(o, g, nsteps):
+    for s in range(1, nsteps + 1):
+        dump = s in g["dump_steps"]
+        pre = f"s{s:04d}_"
+        o.call("ns_momentum")
+        if dump:
+            for f in ["ut", "vt", "wt", "resi_u", "resi_v", "resi_w"]:
+                assert dig(o.array(f)) == g["arrays"][pre + f"mom_{f}.f64"], (s, "mom", f)
+        o.call("velocity_bc")
+        o.call("solid_cell_bc")
+        if dump:
+            for f in ["ut", "vt", "wt", "p"]:
+                assert dig(o.array(f)) == g["arrays"][pre + f"sol_{f}.f64"], (s, "sol", f)
+        o.call("velocity_forcing")
+        if dump:
+            for f in ["ut", "vt", "wt"]:
+                assert dig(o.array(f)) == g["arrays"][pre + f"frc_{f}.f64"], (s, "frc", f)
+        o.call("velocity_bc")
+        it, dm, _ = o.poisson()
+        ref = g["steps"][s - 1]
+        assert it == ref["nIterPcor"], (s, it, ref["nIterPcor"])
+        assert dm == ref["divmax"], (s, dm, ref["divmax"])
+        if dump:
+            for f in ["u", "v", "w", "p", "pc"]:
+                assert dig(o.array(f)) == g["arrays"][pre + f"poi_{f}.f64"], (s, "poi", f)
+        o.call("pressure_forcing")
+        if dump:
+            assert dig(o.array("p")) == g["arrays"][pre + "prf_p.f64"], (s, "prf")
+        c = o.stress(s == 1)
+        assert [c[0], c[1]] == g["drag"][s - 1][:2], (s, c, g["drag"][s - 1])
+        assert [c[2], c[3]] == g["lift"][s - 1][:2], (s, c, g["lift"][s - 1])
+        if s == 1:
+            for nm in ["Elemcell", "ucell", "vcell", "wcell", "pcell"]:
+                assert dig(o.array(nm)) == g["arrays"][nm + ".i64"], nm
+
+
+def test_tiny_matches_reference_bitwise(case_dir):
+    import pyoracle
+    g = load("tiny_reference.json")
+    d = case_dir("tiny")
+    assert input_digests(d) == g["inputs"], "case generator drifted from the fixture"
+    o = pyoracle.Oracle.from_case(d)
+    check_setup(o, g)
+    run_steps_and_check(o, g, g["nsteps"])
+
+
+def test_tiny_arrays_verbatim(case_dir):
+    """Readable copy of a few reference arrays (same data as the digests)."""
+    import pyoracle
+    a = np.load(os.path.join(HERE, "golden", "tiny_reference_arrays.npz"))
+    o = pyoracle.Oracle.from_case(case_dir("tiny"))
+    np.testing.assert_array_equal(o.array("cell").astype(np.int8), a["cell"])
+    np.testing.assert_array_equal(o.array("interceptedIndexPtr"), a["interceptedIndexPtr"])
+    np.testing.assert_array_equal(o.array("pNormDis"), a["pNormDis"])
+    np.testing.assert_array_equal(o.array("Acx"), a["Acx"])
+
+
+def test_cfg1_setup_and_first_step(case_dir):
+    """128x64x64, 5120 triangles: setup arrays and the whole first step."""
+    import pyoracle
+    g = load("cfg1_reference.json")
+    d = case_dir("cfg1")
+    assert input_digests(d) == g["inputs"]
+    o = pyoracle.Oracle.from_case(d)
+    check_setup(o, g)
+    run_steps_and_check(o, g, 1)
+
+
+def test_cfg1_reference_trajectory_is_physical():
+    """The 200-step reference trajectory stored as the GPU parity target:
+    the drag coefficient settles in the Schiller-Naumann band for Re = 100
+    (24/Re (1 + 0.15 Re^0.687) = 1.09, a sanity band, not a parity target)."""
+    g = load("cfg1_reference.json")
+    cd = [d[0] + d[1] for d in g["drag"]]
+    assert len(cd) == 200
+    assert 0.9 < cd[-1] < 1.3
+    assert all(s["nIterPcor"] > 0 for s in g["steps"])

@@ -223,8 +223,10 @@ def search_window(nodes_1d: np.ndarray, centre: float, radius: float, h: float, 
     return int(sel[0]) + 2, int(sel[-1]) + 2
 
 
-def write_case(cfg: CaseConfig, directory: str) -> dict:
-    """Write inputdata, grid?.dat and geometries/sphere.msh; return metadata."""
+def write_case(cfg: CaseConfig, directory: str, istart: int = 0) -> dict:
+    """Write inputdata, grid?.dat and geometries/sphere.msh; return metadata.
+    ``istart`` = 1 makes the driver restart from a ``result`` file
+    (main.f90:15-16)."""
     os.makedirs(os.path.join(directory, "geometries"), exist_ok=True)
     axes = []
     ratios = []
@@ -246,7 +248,7 @@ def write_case(cfg: CaseConfig, directory: str) -> dict:
         fh.write("      0., 0., 0.,\n")
         fh.write(f"      {cfg.n_points}, 0, 0., 0, 0.,\n")
         fh.write(f"      {cfg.shift[0]!r}, {cfg.shift[1]!r}, {cfg.shift[2]!r},\n")
-        fh.write(f"      0, {cfg.dt_order!r}, {cfg.inor},\n")
+        fh.write(f"      {int(istart)}, {cfg.dt_order!r}, {cfg.inor},\n")
         fh.write(f"      {win[0][0]}, {win[0][1]},\n")
         fh.write(f"      {win[1][0]}, {win[1][1]},\n")
         fh.write(f"      {win[2][0]}, {win[2][1]}\n")
@@ -334,3 +336,64 @@ def read_gmsh(path: str, sur_geo_points: int):
         p = next(it).split()
         conn[n] = [int(p[5]), int(p[6]), int(p[7])]
     return nodes, conn
+
+
+# --------------------------------------------------------------------------
+# restart file
+# --------------------------------------------------------------------------
+def write_result(path: str, u, v, w, p, totime: float, ita: int, ita1: int) -> None:
+    """The restart file of subWriteOutput.f90:58-72: one record per padded
+    cell, i fastest, holding ``u v w p totime ita ita1``.  Values are
+    written in shortest round-trip form, so a list-directed read gets the
+    same doubles back.  Fields are (nx+2, ny+2, nz+2) arrays indexed [i,j,k]."""
+    u, v, w, p = (np.asarray(a, dtype=np.float64) for a in (u, v, w, p))
+    uf, vf, wf, pf = (a.reshape(-1, order="F") for a in (u, v, w, p))
+    tail = f" {float(totime)!r} {int(ita)} {int(ita1)}\n"
+    with open(path, "w") as fh:
+        for q in range(uf.size):
+            fh.write(f" {float(uf[q])!r} {float(vf[q])!r} {float(wf[q])!r} {float(pf[q])!r}{tail}")
+
+
+def read_result(path: str, n: tuple):
+    """Read a restart file (subLastConditions.f90:24-30); returns
+    (u, v, w, p, totime, ita, ita1) with fields shaped (nx+2, ny+2, nz+2).
+    Fortran exponent letters (D) are accepted."""
+    shape = (n[0] + 2, n[1] + 2, n[2] + 2)
+    cnt = shape[0] * shape[1] * shape[2]
+    cols = [np.empty(cnt) for _ in range(4)]
+    totime, ita, ita1 = 0.0, 0, 0
+    with open(path) as fh:
+        for q in range(cnt):
+            t = fh.readline().replace(",", " ").replace("D", "E").replace("d", "e").split()
+            if len(t) < 7:
+                raise ValueError(f"{path}: record {q + 1} holds {len(t)} values, 7 needed")
+            for c in range(4):
+                cols[c][q] = float(t[c])
+            totime, ita, ita1 = float(t[4]), int(t[5]), int(t[6])
+    fields = [c.reshape(shape, order="F") for c in cols]
+    return fields[0], fields[1], fields[2], fields[3], totime, ita, ita1
+
+
+def read_iter_dat(path: str):
+    """Rows of iter.dat (subPcorVcor.f90:60-62, FORMAT(' ',I8,I10,4E15.6)):
+    (ita, nIterPcor, divmax, derrStdSt, sor_seconds)."""
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            if not line.strip():
+                continue
+            rows.append((int(line[1:9]), int(line[9:19]), float(line[19:34]), float(line[34:49]),
+                         float(line[49:64])))
+    return rows
+
+
+def read_coeff_dat(path: str):
+    """Rows of dragcoff.dat / liftcoff.dat (list-directed, three values:
+    viscous part, pressure part, totime; subStressCal2.f90:374-379)."""
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            t = line.replace("D", "E").split()
+            if t:
+                rows.append(tuple(float(x) for x in t[:3]))
+    return rows

@@ -69,3 +69,52 @@ def test_reads_the_reference_shipped_case():
     assert (inp["i_startSearch"], inp["k_endSearch"]) == (78, 166)
     gx = cases.read_grid(os.path.join(REFERENCE, "gridx.dat"), 283)
     assert gx[0] == 0.0 and np.all(np.diff(gx) > 0)
+
+
+def test_result_file_round_trip(tmp_path):
+    """write_result/read_result keep every double (shortest round-trip text)
+    and the reference's record order (i fastest, subWriteOutput.f90:65-69)."""
+    from ib3d_amd import cases
+    rng = np.random.default_rng(7)
+    n = (5, 4, 3)
+    shape = (n[0] + 2, n[1] + 2, n[2] + 2)
+    f = [rng.standard_normal(shape) * 10.0 ** rng.integers(-12, 12, size=shape) for _ in range(4)]
+    path = str(tmp_path / "result")
+    cases.write_result(path, *f, 0.125, 7, 9)
+    with open(path) as fh:
+        first = fh.readline().split()
+        second = fh.readline().split()
+    assert float(first[0]) == f[0][0, 0, 0] and float(second[0]) == f[0][1, 0, 0]
+    got = cases.read_result(path, n)
+    for a, b in zip(got[:4], f):
+        assert np.array_equal(a, b)
+    assert got[4:] == (0.125, 7, 9)
+
+
+def test_iter_dat_columns(tmp_path):
+    """FORMAT(' ',I8,I10,4E15.6) of subPcorVcor.f90:62, as Fortran writes it."""
+    from ib3d_amd import cases
+    path = str(tmp_path / "iter.dat")
+    with open(path, "w") as fh:
+        fh.write(" " + "%8d" % 12 + "%10d" % 2380 + "%15s" % "0.707429E+01" + "%15s" % "0.123457E-02"
+                 + "%15s" % "0.456000E+00" + "\n")
+    (ita, nit, dm, ds, t), = cases.read_iter_dat(path)
+    assert (ita, nit) == (12, 2380)
+    assert dm == 7.07429 and ds == 0.00123457 and t == 0.456
+
+
+def test_fortran_driver_fails_loudly_without_a_gpu(tmp_path):
+    """bin/ib3d_main stops with the library's message and a non-zero status
+    where no GPU is visible (there is no CPU path behind the drop-in)."""
+    import os
+    import subprocess
+    from ib3d_amd import cases
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    driver = os.path.join(root, "ib3d-gpu_amd", "bin", "ib3d_main")
+    if not os.path.exists(driver):
+        pytest.skip("Fortran driver not built")
+    cases.write_case(cases.CONFIGS["tiny"], str(tmp_path))
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+    r = subprocess.run([driver, "1"], cwd=str(tmp_path), capture_output=True, text=True, env=env, timeout=120)
+    assert r.returncode != 0
+    assert "ib3d: ib3d_init" in r.stdout + r.stderr

@@ -43,7 +43,7 @@ void ib_release();
 template <typename T>
 static int dalloc(T **p, int64_t n) {
     IB3D_CHECK(hipMalloc((void **)p, std::max<int64_t>(1, n) * sizeof(T)));
-    IB3D_CHECK(hipMemset(*p, 0, std::max<int64_t>(1, n) * sizeof(T)));
+    IB3D_CHECK(dzero(*p, std::max<int64_t>(1, n) * sizeof(T)));
     return 0;
 }
 template <typename T>
@@ -106,7 +106,7 @@ static void free_all(Ctx &c) {
     for (int t = 0; t < 6; t++) dfree(c.tab[t]);
     dfree(c.e32[0]); dfree(c.e32[1]); dfree(c.r32[0]); dfree(c.r32[1]);
     dfree(c.tag); dfree(c.minElem); dfree(c.elp);
-    dfree(c.ibList); dfree(c.fluidList); dfree(c.solidList); dfree(c.nel); dfree(c.scell);
+    dfree(c.ibList); dfree(c.fluidList); dfree(c.solidList); dfree(c.nel); dfree(c.scell); dfree(c.scontrib);
     release_plans(c);
     if (c.hscal) (void)hipHostFree(c.hscal);
     c.hscal = nullptr;
@@ -132,7 +132,7 @@ static void grid_axis(int64_t n, const double *g, std::vector<double> &x1, std::
 }
 
 static int upload_vec(double *dst, const std::vector<double> &v) {
-    IB3D_CHECK(hipMemcpy(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    IB3D_CHECK(h2d(dst, v.data(), v.size() * sizeof(double)));
     return 0;
 }
 
@@ -491,10 +491,10 @@ int ib3d_download(const char *name, void *host, int64_t nbytes) {
     IB3D_CHECK(hipStreamSynchronize(c.stream));
     switch (e.kind) {
         case F64_FIELD:
-            IB3D_CHECK(hipMemcpy(host, (double *)e.dev + e.offset, nbytes, hipMemcpyDeviceToHost));
+            IB3D_CHECK(d2h(host, (double *)e.dev + e.offset, nbytes));
             break;
         case IB_F64:
-            IB3D_CHECK(hipMemcpy(host, (double *)e.dev + e.offset, nbytes, hipMemcpyDeviceToHost));
+            IB3D_CHECK(d2h(host, (double *)e.dev + e.offset, nbytes));
             break;
         case PC_FIELD: {
             double *tmp = nullptr;
@@ -502,20 +502,20 @@ int ib3d_download(const char *name, void *host, int64_t nbytes) {
             hipLaunchKernelGGL(k_gather_pc, dim3(nblocks(c.P, 256)), dim3(256), 0, c.stream, c.pc[0], c.pc[1], tmp,
                                (int)c.sx, (int)c.sy, (int)c.sz, c.H);
             IB3D_CHECK(hipStreamSynchronize(c.stream));
-            IB3D_CHECK(hipMemcpy(host, tmp, nbytes, hipMemcpyDeviceToHost));
+            IB3D_CHECK(d2h(host, tmp, nbytes));
             IB3D_CHECK(hipFree(tmp));
             break;
         }
         case TAG_FIELD: {
             std::vector<int8_t> t(c.P);
-            IB3D_CHECK(hipMemcpy(t.data(), c.tag, c.P, hipMemcpyDeviceToHost));
+            IB3D_CHECK(d2h(t.data(), c.tag, c.P));
             int64_t *o = (int64_t *)host;
             for (int64_t q = 0; q < c.P; q++) o[q] = t[q];
             break;
         }
         case I32_FIELD: case IB_NEL: case ELEM_I32: {
             std::vector<int> t(e.count);
-            IB3D_CHECK(hipMemcpy(t.data(), (int *)e.dev + e.offset, e.count * sizeof(int), hipMemcpyDeviceToHost));
+            IB3D_CHECK(d2h(t.data(), (int *)e.dev + e.offset, e.count * sizeof(int)));
             int64_t *o = (int64_t *)host;
             for (int64_t q = 0; q < e.count; q++) o[q] = t[q];
             break;
@@ -524,14 +524,14 @@ int ib3d_download(const char *name, void *host, int64_t nbytes) {
             int64_t *o = (int64_t *)host;
             if (e.set == 0) {
                 std::vector<int> t(e.count);
-                if (e.count) IB3D_CHECK(hipMemcpy(t.data(), e.dev, e.count * sizeof(int), hipMemcpyDeviceToHost));
+                if (e.count) IB3D_CHECK(d2h(t.data(), e.dev, e.count * sizeof(int)));
                 for (int64_t q = 0; q < e.count; q++) o[q] = t[q];
             } else {
                 // red/black lists: the fluid list split by mod(i+j+k,2)
                 // (subSearch.f90:557-589)
                 const int64_t nf = c.fluidCount, cnt = e.count / 3;
                 std::vector<int> t(3 * nf);
-                if (nf) IB3D_CHECK(hipMemcpy(t.data(), c.fluidList, 3 * nf * sizeof(int), hipMemcpyDeviceToHost));
+                if (nf) IB3D_CHECK(d2h(t.data(), c.fluidList, 3 * nf * sizeof(int)));
                 int64_t r = 0;
                 for (int64_t q = 0; q < nf; q++) {
                     const int odd = (t[q] + t[nf + q] + t[2 * nf + q]) % 2 == 1;
@@ -570,12 +570,12 @@ int ib3d_upload(const char *name, const void *host, int64_t nbytes) {
     IB3D_CHECK(hipStreamSynchronize(c.stream));
     switch (e.kind) {
         case F64_FIELD:
-            IB3D_CHECK(hipMemcpy((double *)e.dev + e.offset, host, nbytes, hipMemcpyHostToDevice));
+            IB3D_CHECK(h2d((double *)e.dev + e.offset, host, nbytes));
             break;
         case PC_FIELD: {
             double *tmp = nullptr;
             IB3D_CHECK(hipMalloc(&tmp, c.P * sizeof(double)));
-            IB3D_CHECK(hipMemcpy(tmp, host, nbytes, hipMemcpyHostToDevice));
+            IB3D_CHECK(h2d(tmp, host, nbytes));
             hipLaunchKernelGGL(k_scatter_pc, dim3(nblocks(c.P, 256)), dim3(256), 0, c.stream, c.pc[0], c.pc[1], tmp,
                                (int)c.sx, (int)c.sy, (int)c.sz, c.H);
             IB3D_CHECK(hipStreamSynchronize(c.stream));
@@ -590,13 +590,13 @@ int ib3d_upload(const char *name, const void *host, int64_t nbytes) {
                 IB3D_REQUIRE(in[q] >= 0 && in[q] <= 2, "ib3d_upload: cell tags must be 0, 1 or 2");
                 t[q] = (int8_t)in[q];
             }
-            IB3D_CHECK(hipMemcpy(c.tag, t.data(), c.P, hipMemcpyHostToDevice));
+            IB3D_CHECK(h2d(c.tag, t.data(), c.P));
             c.lists_ready = c.normdist_ready = c.plans_ready = false;
             break;
         }
         case F64_VEC: {
             std::memcpy(e.host->data() + 1, host, nbytes);
-            IB3D_CHECK(hipMemcpy(e.dev, e.host->data(), e.host->size() * sizeof(double), hipMemcpyHostToDevice));
+            IB3D_CHECK(h2d(e.dev, e.host->data(), e.host->size() * sizeof(double)));
             break;
         }
         case HOST_VEC: {
@@ -643,14 +643,14 @@ int ib3d_set_mesh(int64_t n_nodes, const double *xn, const double *yn, const dou
     double **ed[] = {&c.xcent, &c.ycent, &c.zcent, &c.cosA, &c.cosB, &c.cosG, &c.alpha3, &c.beta3, &c.gamma3};
     for (auto p : ed) dfree(*p);
     dfree(c.elp);
-    dfree(c.scell);
+    dfree(c.scell); dfree(c.scontrib);
     for (auto p : nd) if (dalloc(p, n_nodes)) return -1;
     for (auto p : ed) if (dalloc(p, n_elems)) return -1;
     if (dalloc(&c.elp, 3 * n_elems)) return -1;
-    IB3D_CHECK(hipMemcpy(c.xnode, xn, n_nodes * 8, hipMemcpyHostToDevice));
-    IB3D_CHECK(hipMemcpy(c.ynode, yn, n_nodes * 8, hipMemcpyHostToDevice));
-    IB3D_CHECK(hipMemcpy(c.znode, zn, n_nodes * 8, hipMemcpyHostToDevice));
-    IB3D_CHECK(hipMemcpy(c.elp, el.data(), el.size() * sizeof(int), hipMemcpyHostToDevice));
+    IB3D_CHECK(h2d(c.xnode, xn, n_nodes * 8));
+    IB3D_CHECK(h2d(c.ynode, yn, n_nodes * 8));
+    IB3D_CHECK(h2d(c.znode, zn, n_nodes * 8));
+    IB3D_CHECK(h2d(c.elp, el.data(), el.size() * sizeof(int)));
     c.ibNodes = n_nodes;
     c.ibElems = n_elems;
     c.norms_ready = c.stress_ready = false;

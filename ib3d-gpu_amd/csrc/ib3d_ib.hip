@@ -723,9 +723,9 @@ static int upload_plan(Ctx &c, ForcePlan &P, const std::vector<int> &order, cons
     IB3D_CHECK(hipMalloc(&P.srcs, 8 * m * sizeof(int)));
     IB3D_CHECK(hipMalloc(&P.pos, 5 * m * sizeof(double)));
     IB3D_CHECK(hipMalloc(&P.res, m * sizeof(double)));
-    IB3D_CHECK(hipMemcpy(P.upd, soi.data(), 8 * m * sizeof(int), hipMemcpyHostToDevice));
-    IB3D_CHECK(hipMemcpy(P.srcs, ssrc.data(), 8 * m * sizeof(int), hipMemcpyHostToDevice));
-    IB3D_CHECK(hipMemcpy(P.pos, sod.data(), 5 * m * sizeof(double), hipMemcpyHostToDevice));
+    IB3D_CHECK(h2d(P.upd, soi.data(), 8 * m * sizeof(int)));
+    IB3D_CHECK(h2d(P.srcs, ssrc.data(), 8 * m * sizeof(int)));
+    IB3D_CHECK(h2d(P.pos, sod.data(), 5 * m * sizeof(double)));
     P.n_final = (int)finals.size();
     std::vector<int> fin(3 * (size_t)std::max(1, P.n_final));
     for (int q = 0; q < P.n_final; q++) {
@@ -734,7 +734,7 @@ static int upload_plan(Ctx &c, ForcePlan &P, const std::vector<int> &order, cons
         fin[2 * P.n_final + q] = fcomp[q];
     }
     IB3D_CHECK(hipMalloc(&P.final_loc, fin.size() * sizeof(int)));
-    IB3D_CHECK(hipMemcpy(P.final_loc, fin.data(), fin.size() * sizeof(int), hipMemcpyHostToDevice));
+    IB3D_CHECK(h2d(P.final_loc, fin.data(), fin.size() * sizeof(int)));
     return 0;
 }
 
@@ -855,8 +855,8 @@ int stress(Ctx &c, int first, double re, double coef[4]) {
     IB3D_REQUIRE(c.norms_ready, "stressCal: computeSurfaceNorm has not run");
     const int ne = (int)c.ibElems;
     if (!c.scell) IB3D_CHECK(hipMalloc(&c.scell, 15 * (size_t)ne * sizeof(int)));
-    double *contrib = nullptr;
-    IB3D_CHECK(hipMalloc(&contrib, 7 * (size_t)ne * sizeof(double)));
+    if (!c.scontrib) IB3D_CHECK(hipMalloc(&c.scontrib, 7 * (size_t)ne * sizeof(double)));
+    double *contrib = c.scontrib;
     if (dev_err(c)) return -1;
     StressArgs A;
     A.xc = c.xcent; A.yc = c.ycent; A.zc = c.zcent; A.ca = c.cosA; A.cb = c.cosB; A.cg = c.cosG;
@@ -875,7 +875,6 @@ int stress(Ctx &c, int first, double re, double coef[4]) {
     IB3D_CHECK(hipGetLastError());
     IB3D_CHECK(hipMemcpyAsync(c.hscal + 8, c.scalars + 8, 7 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     const int err = read_err(c, g_err);
-    IB3D_CHECK(hipFree(contrib));
     IB3D_REQUIRE(err == 0, "stressCal: a probe point lies outside the grid");
     c.stress_ready = true;
     const double *s = c.hscal + 8;

@@ -128,6 +128,7 @@ struct Ctx {
 
     // stress stencil cache [5][3][ibElems]: Elemcell, ucell, vcell, wcell, pcell
     int *scell = nullptr;
+    double *scontrib = nullptr;   // per-element stress contributions (7 x ibElems)
     bool stress_ready = false;
 
     // Poisson configuration
@@ -152,6 +153,25 @@ Ctx &ctx();
 
 // launch helpers
 inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Every transfer and fill goes through the library's stream, which is
+// created non-blocking and therefore not ordered with the null stream: a
+// blocking hipMemcpy/hipMemset would race with kernels queued on it.  These
+// return after the operation has completed.
+inline hipError_t h2d(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx().stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(ctx().stream);
+}
+inline hipError_t d2h(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx().stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(ctx().stream);
+}
+inline hipError_t dzero(void *dst, size_t bytes) {
+    hipError_t e = hipMemsetAsync(dst, 0, bytes, ctx().stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(ctx().stream);
+}
 int ensure_slots(int64_t n);
 int ensure_partial(int64_t n);
 

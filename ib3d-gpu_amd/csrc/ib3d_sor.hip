@@ -331,9 +331,11 @@ k_resid(const double *__restrict__ p0, const double *__restrict__ p1, const floa
         const float *eo = colour ? e0 : e1;
         const double bd = (colour ? b1 : b0)[f];
         float *rout = r0 ? (colour ? r1 : r0) : nullptr;
-        if (bd == bd) {
-            int ii, jj, kk;
-            decode(g, f, colour, ii, jj, kk);
+        int ii, jj, kk;
+        decode(g, f, colour, ii, jj, kk);
+        // live cells carry a finite b/dt; the explicit interior test keeps
+        // every neighbour access inside the arrays whatever the sentinel holds
+        if (bd == bd && ii >= 1 && ii <= g.sx - 2 && jj >= 1 && jj <= g.sy - 2 && kk >= 1 && kk <= g.sz - 2) {
             const int q = (colour + jj + kk) & 1;
             const int sx = g.sxp, sy = g.sy, nzp = g.sz;
             double pw = po[f - 1 + q], pe = po[f + q], ps = po[f - g.H], pn = po[f + g.H];
@@ -437,18 +439,30 @@ int check_fastdiv(int d, int64_t nmax) {
     return 0;
 }
 
+// grow the slot array, keeping its contents (a batch may gate on the last
+// slot of the previous one)
 int ensure_slots(int64_t n) {
     Ctx &c = ctx();
     if (n <= c.n_slots) return 0;
-    if (c.slots) IB3D_CHECK(hipFree(c.slots));
-    c.n_slots = std::max<int64_t>(n, 4096);
-    IB3D_CHECK(hipMalloc(&c.slots, c.n_slots * sizeof(double)));
+    const int64_t nn = std::max<int64_t>(2 * n, 4096);
+    double *fresh = nullptr;
+    IB3D_CHECK(hipMalloc(&fresh, nn * sizeof(double)));
+    IB3D_CHECK(hipMemsetAsync(fresh, 0, nn * sizeof(double), c.stream));
+    if (c.slots) {
+        IB3D_CHECK(hipMemcpyAsync(fresh, c.slots, c.n_slots * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        IB3D_CHECK(hipStreamSynchronize(c.stream));
+        IB3D_CHECK(hipFree(c.slots));
+    }
+    IB3D_CHECK(hipStreamSynchronize(c.stream));
+    c.slots = fresh;
+    c.n_slots = nn;
     return 0;
 }
 
 int ensure_partial(int64_t n) {
     Ctx &c = ctx();
     if (n <= c.n_partial) return 0;
+    IB3D_CHECK(hipStreamSynchronize(c.stream));
     if (c.partial) IB3D_CHECK(hipFree(c.partial));
     c.n_partial = n;
     IB3D_CHECK(hipMalloc(&c.partial, n * sizeof(double)));
@@ -464,16 +478,16 @@ int sor_float_tables(Ctx &c) {
     if (g_axf && g_f_sx == c.sx && g_f_sy == c.sy && g_f_sz == c.sz) return 0;
     if (g_axf) { (void)hipFree(g_axf); (void)hipFree(g_ayf); (void)hipFree(g_azf); }
     std::vector<double> hx(3 * c.sxp), hy(3 * c.sy), hz(3 * c.sz);
-    IB3D_CHECK(hipMemcpy(hx.data(), c.ax, hx.size() * 8, hipMemcpyDeviceToHost));
-    IB3D_CHECK(hipMemcpy(hy.data(), c.ay, hy.size() * 8, hipMemcpyDeviceToHost));
-    IB3D_CHECK(hipMemcpy(hz.data(), c.az, hz.size() * 8, hipMemcpyDeviceToHost));
+    IB3D_CHECK(d2h(hx.data(), c.ax, hx.size() * 8));
+    IB3D_CHECK(d2h(hy.data(), c.ay, hy.size() * 8));
+    IB3D_CHECK(d2h(hz.data(), c.az, hz.size() * 8));
     std::vector<float> fx(hx.begin(), hx.end()), fy(hy.begin(), hy.end()), fz(hz.begin(), hz.end());
     IB3D_CHECK(hipMalloc(&g_axf, fx.size() * 4));
     IB3D_CHECK(hipMalloc(&g_ayf, fy.size() * 4));
     IB3D_CHECK(hipMalloc(&g_azf, fz.size() * 4));
-    IB3D_CHECK(hipMemcpy(g_axf, fx.data(), fx.size() * 4, hipMemcpyHostToDevice));
-    IB3D_CHECK(hipMemcpy(g_ayf, fy.data(), fy.size() * 4, hipMemcpyHostToDevice));
-    IB3D_CHECK(hipMemcpy(g_azf, fz.data(), fz.size() * 4, hipMemcpyHostToDevice));
+    IB3D_CHECK(h2d(g_axf, fx.data(), fx.size() * 4));
+    IB3D_CHECK(h2d(g_ayf, fy.data(), fy.size() * 4));
+    IB3D_CHECK(h2d(g_azf, fz.data(), fz.size() * 4));
     g_f_sx = c.sx; g_f_sy = c.sy; g_f_sz = c.sz;
     return 0;
 }

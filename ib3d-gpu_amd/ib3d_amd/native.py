@@ -2,7 +2,7 @@
 
 The library is the product path: there is no CPU fallback.  If
 ``lib/libib3d_hip.so`` is missing or no GPU is visible, every call raises.
-Routine names follow the reference's subroutines (main.f90:115-177).
+Routine names follow the reference's subroutines (main.f90:5-67).
 """
 from __future__ import annotations
 

@@ -1,0 +1,76 @@
+"""Accuracy of the L2-residual Poisson modes against the FP64 path that is
+bit-identical to the reference (reference stopping rule, zero guess).
+
+The fixed sphere at 128x64x64 runs 40 steps three times on the GPU:
+- reference rule (FP64, max|dpc| < eps1, zero guess), itself checked bit for
+  bit against the reference's own run (test_gpu_reference_trajectory);
+- FP64 RB-SOR to ||r||/||b|| <= 1e-6 from the previous step's correction;
+- FP32 RB-SOR inside FP64 defect correction to the same FP64 residual.
+Both L2 modes must keep Cd within 0.5% of the reference's Cd series, and the
+field differences are printed (they are the per-configuration errors stated
+in the documentation).  The differences come from the stopping rule, not
+from FP32: the mixed and the FP64 L2 solutions agree far more closely with
+each other than with the reference rule.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+NSTEPS = 40
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run(case, mode):
+    from ib3d_amd.native import Solver
+    s = Solver.from_case(case)
+    if mode == "fp64":
+        s.set_poisson_mode(1, omega=1.99, tol=1e-6, check=20, warm=True)
+    elif mode == "mixed":
+        s.set_poisson_mode(2, omega=1.99, tol=1e-6, check=20, warm=True)
+    cds, relres = [], []
+    for _ in range(NSTEPS):
+        r = s.step()
+        cds.append(float(r["coef"][0] + r["coef"][1]))
+        if mode != "reference":
+            relres.append(s.poisson_stats()["relres"])
+    out = {f: s.download(f) for f in ("u", "v", "w", "p")}
+    s.close()
+    return out, np.array(cds), relres
+
+
+def relmax(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def test_l2_modes_against_the_reference_rule(case_dir):
+    with open(os.path.join(HERE, "golden", "cfg1_reference.json")) as fh:
+        g = json.load(fh)
+    cd_ref_golden = np.array([d[0] + d[1] for d in g["drag"][:NSTEPS]])
+    case = case_dir("cfg1")
+    _say(f"[gpu] accuracy: reference rule, {NSTEPS} steps at 128x64x64")
+    ref, cd_ref, _ = run(case, "reference")
+    np.testing.assert_allclose(cd_ref, cd_ref_golden, rtol=1e-10)
+    res = {}
+    for mode in ("fp64", "mixed"):
+        _say(f"[gpu] accuracy: {mode}")
+        f, cd, rr = run(case, mode)
+        assert max(rr) <= 1e-6
+        errs = {k: relmax(f[k], ref[k]) for k in f}
+        cd_err = float(np.max(np.abs(cd[5:] - cd_ref[5:]) / np.abs(cd_ref[5:])))
+        res[mode] = (f, errs, cd_err)
+        _say(f"[gpu] {mode}: max rel field error vs reference rule after {NSTEPS} steps: "
+             + ", ".join(f"{k}={v:.3e}" for k, v in errs.items()) + f"; Cd max rel err (steps 6-{NSTEPS}) {cd_err:.3e}")
+        assert cd_err < 5e-3          # the 0.5% Cd bar
+        assert errs["u"] < 1e-4 and errs["p"] < 1e-3
+    d = {k: relmax(res["mixed"][0][k], res["fp64"][0][k]) for k in ("u", "v", "w", "p")}
+    _say("[gpu] mixed vs FP64 at the same 1e-6 residual: " + ", ".join(f"{k}={v:.3e}" for k, v in d.items()))
+    assert d["u"] < 1e-9 and d["p"] < 1e-8

@@ -183,4 +183,10 @@ int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);
 int build_force_plans(Ctx &c);
 int velocity_bc(Ctx &c);
 
+// learned initial guess (ib3d_cnn.hip)
+bool cnn_active();
+int cnn_before_solve(Ctx &c, const double *rel_prev, double *rel_guess, int (*resid)(Ctx &, double *));
+int cnn_after_solve(Ctx &c, const double *rel_prev, const double *rel_guess);
+void cnn_release();
+
 }  // namespace ib3d

@@ -621,12 +621,28 @@ static void add_increment(Ctx &c) {
 // adds the previous increment to pc), then the FP64 residual.  Everything of
 // iteration o is gated on conv[o-1]; the host reads rel/conv once per batch
 // of outer iterations.  Index 0 holds the initial residual.
+static double g_bn = 0.0;
+static int resid_rel(Ctx &c, double *rel) {
+    residual_async(c, false, false, g_bn, rel, c.scalars + 19, nullptr);
+    IB3D_CHECK(hipGetLastError());
+    return 0;
+}
+
 static int sor_l2(Ctx &c, bool mixed, double bn, int64_t *sweeps, int64_t *outer, double *rel, double *rel0) {
     if (mixed && sor_float_tables(c)) return -1;
     if (ensure_partial(nblocks(2 * c.NC, kSorBlock))) return -1;
     const int64_t max_outer = std::max<int64_t>(1, c.max_sweeps / c.check);
     if (ensure_slots(2 * (max_outer + 2))) return -1;
     double *relv = c.slots, *conv = c.slots + c.n_slots / 2;
+    // learned initial guess / training-pair recording: residual of p_prev
+    // (FP32 copy in r32) into scalars[16], the guess's into scalars[18]
+    const bool cnn = cnn_active();
+    if (cnn) {
+        g_bn = bn;
+        residual_async(c, false, true, bn, c.scalars + 16, c.scalars + 17, nullptr);
+        IB3D_CHECK(hipGetLastError());
+        if (cnn_before_solve(c, c.scalars + 16, c.scalars + 18, resid_rel)) return -1;
+    }
     residual_async(c, false, mixed, bn, relv, conv, nullptr);
     int64_t o = 0;     // last queued outer iteration
     int64_t seen = 0;  // first index not yet inspected on the host
@@ -657,6 +673,7 @@ static int sor_l2(Ctx &c, bool mixed, double bn, int64_t *sweeps, int64_t *outer
                 const int64_t oc = seen + q;  // converged after oc outer iterations
                 if (mixed && oc > 0) add_increment(c);
                 IB3D_CHECK(hipGetLastError());
+                if (cnn && cnn_after_solve(c, c.scalars + 16, c.scalars + 18)) return -1;
                 *sweeps = oc * c.check;
                 *outer = oc;
                 *rel = hr[q];
@@ -666,6 +683,7 @@ static int sor_l2(Ctx &c, bool mixed, double bn, int64_t *sweeps, int64_t *outer
         if (o >= max_outer) {
             if (mixed && o > 0) add_increment(c);
             IB3D_CHECK(hipGetLastError());
+            if (cnn && cnn_after_solve(c, c.scalars + 16, c.scalars + 18)) return -1;
             *sweeps = o * c.check;
             *outer = o;
             *rel = hr[n - 1];

@@ -33,6 +33,7 @@ def lib():
     L = C.CDLL(LIB_PATH)
     I, I64, D, P = C.c_int, C.c_int64, C.c_double, C.c_void_p
     PD, PI64, PI32 = C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int32)
+    PF = C.POINTER(C.c_float)
     sig = {
         "ib3d_init": [I64, I64, I64],
         "ib3d_finalize": [],
@@ -67,6 +68,17 @@ def lib():
         "ib3d_step": [I, D, D, D, D, PD],
         "ib3d_bench_sor": [I, I, D, PD],
         "ib3d_probe_copy": [I64, PD],
+        "ib3d_cnn_init": [C.c_uint64],
+        "ib3d_cnn_release": [],
+        "ib3d_cnn_set_params": [PF, I64],
+        "ib3d_cnn_get_params": [PF, I64],
+        "ib3d_cnn_forward": [PF, I64, I64, I64, I64, PF],
+        "ib3d_cnn_gradient": [PF, PF, I64, I64, I64, I64, PF],
+        "ib3d_cnn_adam": [PF, D, D, D, D],
+        "ib3d_cnn_config": [I, I],
+        "ib3d_cnn_clear_pairs": [],
+        "ib3d_cnn_train": [I, I, D, C.c_uint64, PD],
+        "ib3d_cnn_stats": [PI64, PI64, PD, PD, PD],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -76,6 +88,9 @@ def lib():
     L.ib3d_last_error.argtypes = []
     L.ib3d_array_count.restype = I64
     L.ib3d_array_count.argtypes = [C.c_char_p]
+    for name in ("ib3d_cnn_param_count", "ib3d_cnn_pair_count"):
+        getattr(L, name).restype = I64
+        getattr(L, name).argtypes = []
     _lib = L
     return L
 
@@ -91,6 +106,10 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 INT_ARRAYS = {"cell", "minElemcell", "fluidIndexPtr", "solidIndexPtr", "interceptedIndexPtr",
@@ -133,7 +152,7 @@ class Solver:
     @classmethod
     def from_case(cls, directory, window=None):
         """readInput + readSurfaceMeshGmsh (host I/O) followed by the setup
-        calls of main.f90:118-128 on the device."""
+        calls of main.f90:9-18 on the device."""
         inp = cases.read_inputdata(os.path.join(directory, "inputdata"))
         nx, ny, nz = inp["nx"], inp["ny"], inp["nz"]
         s = cls(nx, ny, nz)
@@ -223,7 +242,7 @@ class Solver:
         return r.value, b.value
 
     def step(self):
-        """One time step (main.f90:152-174) on the device."""
+        """One time step (main.f90:26-39 for the first, 41-66 after) on the device."""
         out = np.zeros(8)
         _ok(self.L.ib3d_step(int(self.first_step), self.rev, self.re, self.deltat, self.eps1, _dp(out)))
         self.first_step = False
@@ -242,3 +261,69 @@ class Solver:
 
     def sync(self):
         _ok(self.L.ib3d_sync())
+
+
+class Net:
+    """The learned initial guess held by the library (one per process).
+
+    Parameters are a flat FP32 vector of 15601 values: per layer
+    W[tap][cin][cout] then bias[cout], layers 3->16, 16->16, 16->16, 16->1."""
+
+    N_PARAM = 15601
+
+    def __init__(self, seed=1234):
+        self.L = lib()
+        _ok(self.L.ib3d_cnn_init(int(seed)))
+
+    def release(self):
+        _ok(self.L.ib3d_cnn_release())
+
+    def params(self):
+        out = np.empty(self.N_PARAM, dtype=np.float32)
+        _ok(self.L.ib3d_cnn_get_params(_fp(out), out.size))
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        _ok(self.L.ib3d_cnn_set_params(_fp(a), a.size))
+
+    def forward(self, x):
+        """x: (nb, vz, vy, vx, 3) -> (nb, vz, vy, vx)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        nb, vz, vy, vx, _ = x.shape
+        y = np.empty((nb, vz, vy, vx), dtype=np.float32)
+        _ok(self.L.ib3d_cnn_forward(_fp(x), nb, vx, vy, vz, _fp(y)))
+        return y
+
+    def gradient(self, x, dy):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        dy = np.ascontiguousarray(dy, dtype=np.float32)
+        nb, vz, vy, vx, _ = x.shape
+        g = np.empty(self.N_PARAM, dtype=np.float32)
+        _ok(self.L.ib3d_cnn_gradient(_fp(x), _fp(dy), nb, vx, vy, vz, _fp(g)))
+        return g
+
+    def adam(self, grad, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        g = np.ascontiguousarray(grad, dtype=np.float32)
+        _ok(self.L.ib3d_cnn_adam(_fp(g), lr, beta1, beta2, eps))
+
+    def config(self, record=False, guess=False):
+        _ok(self.L.ib3d_cnn_config(int(bool(record)), int(bool(guess))))
+
+    def pair_count(self):
+        return int(self.L.ib3d_cnn_pair_count())
+
+    def clear_pairs(self):
+        _ok(self.L.ib3d_cnn_clear_pairs())
+
+    def train(self, epochs=20, batch=4, lr=1e-3, seed=7):
+        loss = np.zeros(epochs)
+        _ok(self.L.ib3d_cnn_train(int(epochs), int(batch), float(lr), int(seed), _dp(loss)))
+        return loss
+
+    def stats(self):
+        n, k = C.c_int64(), C.c_int64()
+        rp, rg, ms = C.c_double(), C.c_double(), C.c_double()
+        _ok(self.L.ib3d_cnn_stats(C.byref(n), C.byref(k), C.byref(rp), C.byref(rg), C.byref(ms)))
+        return {"solves": n.value, "kept": k.value, "rel_prev": rp.value, "rel_guess": rg.value,
+                "guess_ms": ms.value}

@@ -143,6 +143,43 @@ int ib3d_bench_sor(int precision, int sweeps, double omega, double *ms_per_sweep
  * hardware fetch/write counters into bytes; ms = its duration. */
 int ib3d_probe_copy(int64_t nbytes, double *ms);
 
+/* ---- learned initial guess (no counterpart in the reference) -------------
+ * A 3-D CNN (3 -> 16 -> 16 -> 16 -> 1 channels, 3x3x3, ReLU) predicts the
+ * start of each L2-mode Poisson solve from the warm start's defect
+ * r0 = b/dt - A p_prev (scaled by 1/|A_diag|), the fluid mask and p_prev.
+ * The solve keeps the prediction only when its FP64 residual is below the
+ * previous-step guess's and iterates to the same tolerance either way.
+ * Weights are BF16 on MFMA with FP32 accumulation; the FP32 master copy is
+ * trained by Adam in the library's own kernels. */
+/* allocate the 15601 parameters (He-normal weights from `seed`, zero biases) */
+int ib3d_cnn_init(uint64_t seed);
+int ib3d_cnn_release(void);
+int64_t ib3d_cnn_param_count(void);
+/* parameter vector: per layer W[tap][cin][cout] (tap = kz*9 + ky*3 + kx,
+ * offset -1..1 each) then bias[cout]; layers 3->16, 16->16, 16->16, 16->1 */
+int ib3d_cnn_set_params(const float *params, int64_t n);
+int ib3d_cnn_get_params(float *params, int64_t n);
+/* forward on nb volumes (nb x vz x vy x vx x 3 FP32, channels last, zero
+ * padding outside each volume) -> nb x vz x vy x vx FP32 */
+int ib3d_cnn_forward(const float *x, int64_t nb, int64_t vx, int64_t vy, int64_t vz, float *y);
+/* gradient of sum(dy * forward(x)) with respect to the parameters */
+int ib3d_cnn_gradient(const float *x, const float *dy, int64_t nb, int64_t vx, int64_t vy, int64_t vz,
+                      float *grad);
+/* one Adam step with the given gradient */
+int ib3d_cnn_adam(const float *grad, double lr, double beta1, double beta2, double eps);
+/* record = 1: every L2-mode solve stores (input, converged correction) as a
+ * training pair on the device; guess = 1: every L2-mode solve starts from the
+ * network's guess when it lowers the residual */
+int ib3d_cnn_config(int record, int guess);
+int64_t ib3d_cnn_pair_count(void);
+int ib3d_cnn_clear_pairs(void);
+/* Adam over 64^3 tiles (halo 4) of the recorded pairs; mean loss per epoch */
+int ib3d_cnn_train(int epochs, int batch, double lr, uint64_t seed, double *loss_per_epoch);
+/* solves that used the network, how many kept its guess, and the last
+ * solve's relative residuals of the previous-step guess and of the network's
+ * guess, and the guess's cost in ms */
+int ib3d_cnn_stats(int64_t *solves, int64_t *kept, double *rel_prev, double *rel_guess, double *guess_ms);
+
 #ifdef __cplusplus
 }
 #endif

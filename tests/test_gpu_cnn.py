@@ -1,0 +1,128 @@
+"""The learned initial guess on the GPU: BF16 MFMA forward, backward and
+Adam against the FP64 oracle (oracle/cnn_oracle.py, with the oracle rounding
+what the GPU stores in BF16), and an ablation on the 128x64x64 sphere: the
+network trained on pairs the solver dumped, then solves with and without its
+guess from the same state."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def net():
+    from ib3d_amd.native import Net, Solver
+    # the network lives beside a device context; a tiny one suffices here
+    s = Solver(8, 8, 8)
+    n = Net(seed=11)
+    yield n
+    n.release()
+    s.close()
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(np.asarray(a, dtype=np.float64) - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def test_forward_matches_oracle(net):
+    import cnn_oracle as co
+    rng = np.random.default_rng(21)
+    p = net.params().astype(np.float64)
+    for layer in range(3):
+        p[co.SLICES[layer][1]] = rng.uniform(-0.2, 0.5, co.COUT[layer])
+    p[co.SLICES[3][1]] = 0.1
+    net.set_params(p.astype(np.float32))
+    x = rng.standard_normal((2, 9, 10, 37, 3)).astype(np.float32)
+    _say("[gpu] cnn forward 2x9x10x37")
+    y = net.forward(x)
+    ref, _ = co.forward(net.params(), x, bf16=True)
+    e = rel_err(y, ref)
+    _say(f"[gpu] cnn forward max err / max |y| = {e:.2e}")
+    assert e < 5e-3
+
+
+def test_gradient_matches_oracle(net):
+    import cnn_oracle as co
+    rng = np.random.default_rng(22)
+    x = rng.standard_normal((2, 8, 11, 35, 3)).astype(np.float32)
+    dy = rng.standard_normal((2, 8, 11, 35)).astype(np.float32)
+    _say("[gpu] cnn gradient 2x8x11x35")
+    g = net.gradient(x, dy)
+    ref = co.gradient(net.params(), x, dy, bf16=True)
+    for layer in range(4):
+        ws, bs = co.SLICES[layer]
+        ew, eb = rel_err(g[ws], ref[ws]), rel_err(g[bs], ref[bs])
+        _say(f"[gpu] layer {layer + 1}: weight grad err {ew:.2e}, bias grad err {eb:.2e}")
+        assert ew < 1e-2 and eb < 1e-2
+
+
+def test_adam_matches_oracle(net):
+    import cnn_oracle as co
+    rng = np.random.default_rng(23)
+    p0 = net.params().astype(np.float64)
+    net.set_params(p0.astype(np.float32))   # also resets the moments
+    p, m, v = p0.copy(), np.zeros_like(p0), np.zeros_like(p0)
+    for t in (1, 2, 3):
+        g = rng.standard_normal(p0.size).astype(np.float32)
+        net.adam(g, lr=1e-3)
+        p, m, v = co.adam_step(p, g.astype(np.float64), m, v, t)
+    np.testing.assert_allclose(net.params(), p, rtol=1e-5, atol=1e-7)
+
+
+def run_steps(case, n_before, n_after, guess, net, record=False):
+    from ib3d_amd.native import Solver
+    s = Solver.from_case(case)
+    s.set_poisson_mode(1, omega=1.99, tol=1e-6, check=20, warm=True)
+    net.config(record=record, guess=False)
+    for _ in range(n_before):
+        s.step()
+    net.config(record=False, guess=guess)
+    out = []
+    for _ in range(n_after):
+        s.step()
+        st = s.poisson_stats()
+        row = {"sweeps": st["sweeps"], "relres": st["relres"], "relres0": st["relres0"]}
+        if guess:
+            row.update(net.stats())
+        out.append(row)
+    net.config(False, False)
+    s.close()
+    return out
+
+
+def test_learned_guess_ablation(case_dir, net):
+    """Train on pairs dumped by the first 40 solves of the 128x64x64 case, then
+    compare steps 41-60 started from the network's guess against the
+    previous-step guess.  The fallback rule must hold on every solve; the
+    gain is printed (sweeps per solve with and without the network)."""
+    from ib3d_amd.native import Net
+    case = case_dir("cfg1")
+    net.release()
+    n2 = Net(seed=5)
+    _say("[gpu] ablation: recording 40 solves")
+    run_steps(case, 40, 0, False, n2, record=True)
+    assert n2.pair_count() == 40
+    _say("[gpu] ablation: training")
+    loss = n2.train(epochs=20, batch=4, lr=1e-3, seed=3)
+    _say("[gpu] loss per epoch: " + " ".join(f"{l:.3e}" for l in loss))
+    assert np.all(np.isfinite(loss)) and loss[-1] < loss[0]
+    n2.clear_pairs()
+    base = run_steps(case, 40, 20, False, n2)
+    learned = run_steps(case, 40, 20, True, n2)
+    for r in learned:
+        assert r["relres"] <= 1e-6
+        assert r["relres0"] <= r["rel_prev"] * (1 + 1e-12)   # the guess never starts worse
+    sb = np.array([r["sweeps"] for r in base])
+    sl = np.array([r["sweeps"] for r in learned])
+    kept = learned[-1]["kept"]
+    _say(f"[gpu] ablation: sweeps per solve, previous-step guess {sb.mean():.1f}, learned guess {sl.mean():.1f} "
+         f"(kept {kept}/{len(learned)}); initial residual prev {np.mean([r['rel_prev'] for r in learned]):.3e}, "
+         f"learned {np.mean([r['rel_guess'] for r in learned]):.3e}; guess cost {learned[-1]['guess_ms']:.2f} ms")
+    n2.release()

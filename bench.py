@@ -41,6 +41,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ib3d-gpu_amd"))
 
@@ -51,13 +53,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner):
+def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None):
     s = solver_cls.from_case(case)
+    if net is not None:
+        net.config(record=False, guess=mode == "mixed+learned")
     if mode == "fp64":
         s.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=True)
     elif mode == "fp64-cold":
         s.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=False)
-    elif mode == "mixed":
+    elif mode in ("mixed", "mixed+learned"):
         s.set_poisson_mode(2, omega=omega, tol=tol, check=inner, warm=True)
     elif mode == "reference":
         s.set_poisson_mode(0)
@@ -82,6 +86,8 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner):
             f"relres={st['relres']:.2e} poisson={st['poisson_ms']:.1f} ms")
     s.sync()
     t1 = time.perf_counter()
+    if net is not None:
+        net.config(False, False)
     out = {
         "mode": mode,
         "s_total": t1 - t0,
@@ -92,7 +98,43 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner):
         "poisson_ms_per_step": sum(poi_ms) / steps,
         "cd_last": cds[-1],
     }
+    if mode == "mixed+learned":
+        st = net.stats()
+        out["learned_guess"] = {"solves": st["solves"], "kept": st["kept"], "guess_ms": round(st["guess_ms"], 3)}
     return s, out
+
+
+def train_guess(cfg_name, rec_steps, epochs, batch, lr, omega, tol, inner):
+    """Record (input, converged correction) pairs from `rec_steps` FP64
+    solves of the given case, then train the network on 64^3 tiles of them."""
+    from ib3d_amd import cases
+    from ib3d_amd.native import Net, Solver
+    case = tempfile.mkdtemp(prefix="ib3d_train_")
+    try:
+        cases.write_case(cases.CONFIGS[cfg_name], case)
+        t0 = time.perf_counter()
+        s = Solver.from_case(case)
+        s.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=True)
+        net = Net(seed=20251)
+        net.config(record=True, guess=False)
+        for q in range(rec_steps):
+            s.step()
+            if (q + 1) % 100 == 0:
+                log(f"[bench] recorded {q + 1}/{rec_steps} solves")
+        net.config(False, False)
+        s.close()
+        t1 = time.perf_counter()
+        pairs = net.pair_count()
+        loss = net.train(epochs=epochs, batch=batch, lr=lr, seed=7)
+        t2 = time.perf_counter()
+        net.clear_pairs()
+        log(f"[bench] trained on {pairs} solves in {t2 - t1:.1f} s, loss {loss[0]:.3e} -> {loss[-1]:.3e}")
+        info = {"train_case": cfg_name, "recorded_solves": pairs, "epochs": epochs, "batch": batch, "lr": lr,
+                "record_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 1),
+                "loss_first": float(loss[0]), "loss_last": float(loss[-1])}
+        return net, info
+    finally:
+        shutil.rmtree(case, ignore_errors=True)
 
 
 def kernel_roofline(s, precision, omega, n_cells, sweeps=200):
@@ -136,8 +178,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64", "fp64-cold", "reference"])
-    ap.add_argument("--compare", default="fp64-cold,fp64",
+    ap.add_argument("--mode", default="mixed+learned", choices=["mixed+learned", "mixed", "fp64", "fp64-cold",
+                                                                 "reference"])
+    ap.add_argument("--compare", default="fp64-cold,fp64,mixed",
                     help="comma-separated modes measured in the same run for comparison ('none' to skip)")
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--omega", type=float, default=1.99)
@@ -145,6 +188,9 @@ def main():
     ap.add_argument("--inner", type=int, default=20)
     ap.add_argument("--cpu-sweeps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--train-config", default="cfg1", help="case whose solves are recorded for training")
+    ap.add_argument("--rec-steps", type=int, default=1000, help="recorded solves (two 64^3 tiles each)")
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--profile-sor", nargs=2, type=int, metavar=("PRECISION", "SWEEPS"),
                     help="profiling aid: set up the case, run SWEEPS RB-SOR sweeps and one copy probe, exit")
     a = ap.parse_args()
@@ -177,16 +223,26 @@ def main():
         cases.write_case(cfg, case)
         nx, ny, nz = cfg.n
         n_cells = nx * ny * nz
+        modes = [a.mode] + [x for x in a.compare.split(",") if x and x != "none" and x != a.mode]
+        net, net_info = None, None
+        if "mixed+learned" in modes:
+            net, net_info = train_guess(a.train_config, a.rec_steps, a.epochs, 4, 1e-3, a.omega, a.tol, a.inner)
         t0 = time.perf_counter()
-        s, main_res = run_mode(Solver, case, a.mode, a.steps, a.warmup, a.omega, a.tol, a.inner)
+        s, main_res = run_mode(Solver, case, a.mode, a.steps, a.warmup, a.omega, a.tol, a.inner, net)
+        main_fields = {f: s.download(f) for f in ("u", "p")}
         log(f"[bench] {a.mode}: {main_res}")
-        prec = 32 if a.mode == "mixed" else 64
+        prec = 32 if a.mode.startswith("mixed") else 64
         roof = kernel_roofline(s, prec, a.omega, n_cells)
         roof64 = kernel_roofline(s, 64, a.omega, n_cells) if prec == 32 else roof
         s.close()
         cmp_all = []
-        for m in [x for x in a.compare.split(",") if x and x != "none" and x != a.mode]:
-            s2, r2 = run_mode(Solver, case, m, a.steps, a.warmup, a.omega, a.tol, a.inner)
+        errors = {}
+        for m in modes[1:]:
+            s2, r2 = run_mode(Solver, case, m, a.steps, a.warmup, a.omega, a.tol, a.inner, net)
+            if m.startswith("fp64"):
+                # field difference of the headline mode against this FP64 run after the same steps
+                errors["vs_" + m] = {f: float(np.max(np.abs(main_fields[f] - s2.download(f))) /
+                                          np.max(np.abs(s2.download(f)))) for f in ("u", "p")}
             s2.close()
             log(f"[bench] {m}: {r2}")
             cmp_all.append(r2)
@@ -215,12 +271,14 @@ def main():
             "higher_is_better": True,
             "scaling": "weak",
             "vs_baseline": None,
-            "dtype": "fp32+fp64" if a.mode == "mixed" else "fp64",
+            "dtype": "fp32+fp64" if a.mode.startswith("mixed") else "fp64",
             "data": "synthetic (generated grid and icosphere, reference initial condition)",
             "config": {"workload": f"fixed sphere Re=100, {nx}x{ny}x{nz} stretched grid, "
                                    f"{cfg.level and 20 * 4 ** cfg.level} triangles, dt={cfg.dt}",
                        "poisson": f"{a.mode}: RB-SOR omega={a.omega}, ||r||/||b|| <= {a.tol}, "
-                                  f"{a.inner} sweeps between residual checks, previous-step guess",
+                                  f"{a.inner} sweeps between residual checks, "
+                                  + ("CNN guess with previous-step fallback" if a.mode == "mixed+learned"
+                                     else "previous-step guess" if a.mode in ("mixed", "fp64") else "zero guess"),
                        "parallelism": "1 GPU"},
             "poisson_ms_per_step": round(main_res["poisson_ms_per_step"], 3),
             "sor_ms_per_step": round(main_res["sor_ms_per_step"], 3),
@@ -230,10 +288,14 @@ def main():
             "roofline_fp64_sweep": {k: roof64[k] for k in ("achieved", "frac", "launch_us")},
             "cpu_baseline": cpu,
             "gpu_wall_s": round(gpu_wall, 1),
+            "field_error": errors,
         }
+        if net_info:
+            net_info.update(main_res.get("learned_guess", {}))
+            line["learned_guess"] = net_info
         if cmp_all:
-            line["compare"] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
-                               for r in cmp_all]
+            line["compare"] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()
+                                if k != "learned_guess"} for r in cmp_all]
             for r in cmp_all:
                 line["poisson_speedup_vs_" + r["mode"]] = round(
                     r["poisson_ms_per_step"] / main_res["poisson_ms_per_step"], 3)

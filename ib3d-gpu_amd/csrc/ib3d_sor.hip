@@ -362,6 +362,96 @@ k_resid(const double *__restrict__ p0, const double *__restrict__ p1, const floa
     block_sum_to(rr, partial, lb);
 }
 
+// FP64 residual on cell pairs marching in z, both colours per thread (the
+// same neighbour pattern as k_sor_pz).  ADD: of p + e.  WRITE_R: the residual
+// as FP32 (NaN where b/dt is NaN) for the next inner solve.  One partial sum
+// of r^2 per block.
+template <bool ADD, bool WRITE_R>
+__global__ void __launch_bounds__(kSorBlock)
+k_resid_pz(const double *__restrict__ p0, const double *__restrict__ p1, const float *__restrict__ e0,
+           const float *__restrict__ e1, const double *__restrict__ b0, const double *__restrict__ b1,
+           float *__restrict__ r0, float *__restrict__ r1, const double *__restrict__ axc,
+           const double *__restrict__ ayc, const double *__restrict__ azc, SorGeom g, double *partial,
+           const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    const int p = 2 * (int)(blockIdx.x * kSorBlock + threadIdx.x);
+    const int jj = (int)fdiv((uint32_t)p, g.dH);
+    const int h = p - jj * g.H;
+    double rr = 0.0;
+    if (p < g.plane && jj >= 1 && jj <= g.sy - 2) {
+        const int k0 = 1 + (int)blockIdx.y * kZch;
+        const int k1 = min(k0 + kZch, g.sz - 1);
+        const d4v c1 = *reinterpret_cast<const d4v *>(axc + 2 * h);
+        const d4v c2 = *reinterpret_cast<const d4v *>(axc + g.sxp + 2 * h);
+        const d4v c3 = *reinterpret_cast<const d4v *>(axc + 2 * g.sxp + 2 * h);
+        const double a1y = ayc[jj], a2y = ayc[g.sy + jj], a3y = ayc[2 * g.sy + jj];
+        const double *pc_[2] = {p0, p1};
+        const float *ec_[2] = {e0, e1};
+        const double *bc_[2] = {b0, b1};
+        float *rc_[2] = {r0, r1};
+        auto ld2 = [&](int c, int idx) -> d2v {
+            d2v v = *reinterpret_cast<const d2v *>(pc_[c] + idx);
+            if (ADD) {
+                const f2v e = *reinterpret_cast<const f2v *>(ec_[c] + idx);
+                v.x += (double)e.x;
+                v.y += (double)e.y;
+            }
+            return v;
+        };
+        auto ld1 = [&](int c, int idx) -> double {
+            double v = pc_[c][idx];
+            if (ADD) v += (double)ec_[c][idx];
+            return v;
+        };
+        int f = p + k0 * g.plane;
+        d2v below[2] = {ld2(0, f - g.plane), ld2(1, f - g.plane)};
+        d2v cur[2] = {ld2(0, f), ld2(1, f)};
+        for (int kk = k0; kk < k1; kk++, f += g.plane) {
+            d2v above[2] = {ld2(0, f + g.plane), ld2(1, f + g.plane)};
+            const double a1z = azc[kk], a2z = azc[g.sz + kk], a3z = azc[2 * g.sz + kk];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int o = 1 - c;
+                const d2v bd = *reinterpret_cast<const d2v *>(bc_[c] + f);
+                const bool ok0 = bd.x == bd.x, ok1 = bd.y == bd.y;
+                f2v rout = {__builtin_nanf(""), __builtin_nanf("")};
+                if (ok0 || ok1) {
+                    const int q = (c + jj + kk) & 1;
+                    const double side = q ? ld1(o, f + 2) : ld1(o, f - 1);
+                    const d2v ps = ld2(o, f - g.H), pn = ld2(o, f + g.H);
+                    const d2v co = cur[o], own = cur[c];
+                    const double a1x0 = q ? c1.y : c1.x, a2x0 = q ? c2.y : c2.x, a3x0 = q ? c3.y : c3.x;
+                    const double a1x1 = q ? c1.w : c1.z, a2x1 = q ? c2.w : c2.z, a3x1 = q ? c3.w : c3.z;
+                    if (ok0) {
+                        const double pw = q ? co.x : side, pe = q ? co.y : co.x;
+                        const double ap = (((((a1x0 * pw + a3x0 * pe) + a1y * ps.x) + a3y * pn.x) +
+                                            a1z * below[o].x) + a3z * above[o].x) +
+                                          ((a2x0 + a2y) + a2z) * own.x;
+                        const double r = bd.x - ap;
+                        rr += r * r;
+                        rout.x = (float)r;
+                    }
+                    if (ok1) {
+                        const double pw = q ? co.y : co.x, pe = q ? side : co.y;
+                        const double ap = (((((a1x1 * pw + a3x1 * pe) + a1y * ps.y) + a3y * pn.y) +
+                                            a1z * below[o].y) + a3z * above[o].y) +
+                                          ((a2x1 + a2y) + a2z) * own.y;
+                        const double r = bd.y - ap;
+                        rr += r * r;
+                        rout.y = (float)r;
+                    }
+                }
+                if (WRITE_R) *reinterpret_cast<f2v *>(rc_[c] + f) = rout;
+            }
+            below[0] = cur[0];
+            below[1] = cur[1];
+            cur[0] = above[0];
+            cur[1] = above[1];
+        }
+    }
+    block_sum_to(rr, partial, (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
 // p <- p + e on both colours (fluid cells only)
 __global__ void __launch_bounds__(kSorBlock)
 k_add_e(double *p0, double *p1, const float *e0, const float *e1, const double *b0, const double *b1, int64_t NC) {
@@ -598,17 +688,19 @@ int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2) {
 // partial sums; then rel[o] and conv[o], gated on conv[o-1]
 static void residual_async(Ctx &c, bool add, bool write_r32, double bn, double *rel, double *conv,
                            const double *gate) {
-    const unsigned nb = nblocks(2 * c.NC, kSorBlock);
+    const dim3 grid = pz_grid(c);
+    const int64_t nb = (int64_t)grid.x * grid.y;
     const SorGeom g = geom(c);
-    float *r0 = write_r32 ? c.r32[0] : nullptr, *r1 = write_r32 ? c.r32[1] : nullptr;
-    if (add)
-        hipLaunchKernelGGL(k_resid<true>, dim3(nb), dim3(kSorBlock), 0, c.stream, c.pc[0], c.pc[1], c.e32[0],
-                           c.e32[1], c.bdt[0], c.bdt[1], r0, r1, c.ax, c.ay, c.az, g, c.partial, gate, 0.5);
-    else
-        hipLaunchKernelGGL(k_resid<false>, dim3(nb), dim3(kSorBlock), 0, c.stream, c.pc[0], c.pc[1], c.e32[0],
-                           c.e32[1], c.bdt[0], c.bdt[1], r0, r1, c.ax, c.ay, c.az, g, c.partial, gate, 0.5);
-    hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, c.stream, c.partial, (int64_t)nb, bn, c.tol, rel,
-                       conv, gate);
+    float *r0 = c.r32[0], *r1 = c.r32[1];
+#define IB3D_RESID(A, W)                                                                                         \
+    hipLaunchKernelGGL((k_resid_pz<A, W>), grid, dim3(kSorBlock), 0, c.stream, c.pc[0], c.pc[1], c.e32[0],         \
+                       c.e32[1], c.bdt[0], c.bdt[1], r0, r1, c.ax, c.ay, c.az, g, c.partial, gate, 0.5)
+    if (add && write_r32) IB3D_RESID(true, true);
+    else if (add) IB3D_RESID(true, false);
+    else if (write_r32) IB3D_RESID(false, true);
+    else IB3D_RESID(false, false);
+#undef IB3D_RESID
+    hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, c.stream, c.partial, nb, bn, c.tol, rel, conv, gate);
 }
 
 static void add_increment(Ctx &c) {

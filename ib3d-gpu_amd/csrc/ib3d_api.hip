@@ -107,6 +107,7 @@ static void free_all(Ctx &c) {
     dfree(c.e32[0]); dfree(c.e32[1]); dfree(c.r32[0]); dfree(c.r32[1]);
     dfree(c.tag); dfree(c.minElem); dfree(c.elp);
     dfree(c.ibList); dfree(c.fluidList); dfree(c.solidList); dfree(c.nel); dfree(c.scell); dfree(c.scontrib);
+    dfree(c.rt_flag); dfree(c.rt_list); dfree(c.rt_count);
     release_plans(c);
     if (c.hscal) (void)hipHostFree(c.hscal);
     c.hscal = nullptr;
@@ -661,6 +662,31 @@ int ib3d_shiftSurfaceNodesInitial(double xs, double ys, double zs) {
     Ctx &c = ctx();
     IB3D_REQUIRE(c.ready, "not initialised");
     return shift_nodes(c, xs, ys, zs);
+}
+
+int ib3d_shiftSurfaceNodesMovingbody(double xshift, double yshift, double zshift, double dx, double dy,
+                                     double dz) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(c.ready, "shiftSurfaceNodesMovingbody: not initialised");
+    const double base[3] = {xshift, yshift, zshift}, d[3] = {dx, dy, dz};
+    return shift_nodes_moving(c, base, d);
+}
+
+int ib3d_selectiveRetagging(int32_t counts[3]) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(c.ready, "selectiveRetagging: not initialised");
+    return selective_retagging(c, counts);
+}
+
+int ib3d_set_wall_motion(const double vel[3], const double acc[3]) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(c.ready, "set_wall_motion: not initialised");
+    for (int q = 0; q < 3; q++) {
+        IB3D_REQUIRE(std::isfinite(vel[q]) && std::isfinite(acc[q]), "set_wall_motion: non-finite value");
+        c.wall_vel[q] = vel[q];
+        c.wall_acc[q] = acc[q];
+    }
+    return 0;
 }
 
 int ib3d_computeSurfaceNorm(int64_t inor) {

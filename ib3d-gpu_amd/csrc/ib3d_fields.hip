@@ -234,14 +234,14 @@ __global__ void k_bc_z(double *ut, double *vt, double *wt, int nx, int ny, int n
 
 // subBoundaryConditions.f90:207-215
 __global__ void k_solid_bc(const int *list, int64_t n, double *ut, double *vt, double *wt, double *p, int nx,
-                           int ny) {
+                           int ny, double wu, double wv, double ww) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int i = list[q], j = list[n + q], k = list[2 * n + q];
     const int64_t c = (i - 1) + (int64_t)(nx + 2) * ((j - 1) + (int64_t)(ny + 2) * (k - 1));
-    ut[c] = 0.0;
-    vt[c] = 0.0;
-    wt[c] = 0.0;
+    ut[c] = wu;   // 0 in the reference (a body at rest); the body's velocity when it moves
+    vt[c] = wv;
+    wt[c] = ww;
     p[c] = 0.0;
 }
 
@@ -344,7 +344,8 @@ int solid_bc(Ctx &c) {
     IB3D_REQUIRE(c.lists_ready, "solidCellBC: cellCount has not run");
     if (c.solidCount == 0) return 0;
     hipLaunchKernelGGL(k_solid_bc, dim3(nblocks(c.solidCount, 256)), dim3(256), 0, c.stream, c.solidList,
-                       c.solidCount, c.ut, c.vt, c.wt, c.p, (int)c.nx, (int)c.ny);
+                       c.solidCount, c.ut, c.vt, c.wt, c.p, (int)c.nx, (int)c.ny, c.wall_vel[0], c.wall_vel[1],
+                       c.wall_vel[2]);
     IB3D_CHECK(hipGetLastError());
     return 0;
 }

@@ -35,6 +35,18 @@ __global__ void k_shift_nodes(const double *x, const double *y, const double *z,
     z1[q] = z[q] + zs;
 }
 
+// prescribed rigid displacement of the initially placed body (replaces the
+// free motion of subSearch.f90:41-76): x1 = (x + shift) + d
+__global__ void k_shift_nodes_moving(const double *x, const double *y, const double *z, double *x1, double *y1,
+                                     double *z1, int64_t n, double xs, double ys, double zs, double dx, double dy,
+                                     double dz) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    x1[q] = (x[q] + xs) + dx;
+    y1[q] = (y[q] + ys) + dy;
+    z1[q] = (z[q] + zs) + dz;
+}
+
 // subSearch.f90:95-125 (node ids are 1-based)
 __global__ void k_surface_norm(const double *xn, const double *yn, const double *zn, const int *elp, int64_t ne,
                                double inor, double *xc, double *yc, double *zc, double *ca, double *cb, double *cg,
@@ -134,6 +146,108 @@ k_tagging(int i0, int i1, int j0, int j1, int k0, int k1, int sx, int sy, const 
     }
     const int64_t c = lin(i, j, k, sx, sy);
     tag[c] = all_in ? 1 : (all_out ? 0 : 2);
+    minElem[c] = nel;
+}
+
+// subSearch.f90:360-363: queue the 6x6x6 block (i1-2..i1+3, ...) around every
+// intercepted cell; each cell enters the list once (the reference tags
+// overlapping blocks repeatedly, always with the same result).  Cells
+// outside the padded index range 1..n+2 are not queued.
+__global__ void k_retag_mark(const int *ib, int nb, int nx, int ny, int nz, uint8_t *flag, int *list, int *count) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nb * 216) return;
+    const int n = (int)(q / 216), o = (int)(q % 216);
+    const int i = ib[n] - 2 + o % 6, j = ib[nb + n] - 2 + (o / 6) % 6, k = ib[2 * nb + n] - 2 + o / 36;
+    if (i < 1 || i > nx + 2 || j < 1 || j > ny + 2 || k < 1 || k > nz + 2) return;
+    const int64_t c = lin(i, j, k, nx + 2, ny + 2);
+    unsigned int *word = reinterpret_cast<unsigned int *>(flag + (c & ~(int64_t)3));
+    const unsigned int bit = 1u << (8 * (c & 3));
+    const unsigned int old = atomicOr(word, bit);
+    if (!(old & bit)) list[atomicAdd(count, 1)] = (int)c;
+}
+
+// subSearch.f90:364-439 for the queued cells: nearest centroid and the
+// 8-corner test, the same arithmetic as k_tagging.  A cell that was solid
+// and is no longer takes the wall velocity (the reference's body never
+// moves, so it has no rule for such cells).  Clears its queue flag.
+__global__ void __launch_bounds__(kTagTile)
+k_retag_list(const int *list, const int *count, int sx, int sy, const double *__restrict__ xp,
+             const double *__restrict__ yp, const double *__restrict__ zp, const double *__restrict__ x1,
+             const double *__restrict__ y1, const double *__restrict__ z1, const double *__restrict__ xc,
+             const double *__restrict__ yc, const double *__restrict__ zc, const double *__restrict__ ca,
+             const double *__restrict__ cb, const double *__restrict__ cg, int ne, int8_t *tag, int *minElem,
+             uint8_t *flag, double *u, double *v, double *w, double wu, double wv, double ww, int *err) {
+    __shared__ double sx_[kTagTile], sy_[kTagTile], sz_[kTagTile];
+    const int nq = *count;
+    if ((int)(blockIdx.x * blockDim.x) >= nq) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = q < nq;
+    int i = 0, j = 0, k = 0;
+    int64_t c = 0;
+    if (active) {
+        c = list[q];
+        i = (int)(c % sx) + 1;
+        const int64_t r = c / sx;
+        j = (int)(r % sy) + 1;
+        k = (int)(r / sy) + 1;
+    }
+    const double n1x = active ? xp[i] : 0.0, n1y = active ? yp[j] : 0.0, n1z = active ? zp[k] : 0.0;
+    double minDis = 1e14, smin = __builtin_huge_val();
+    int nel = 0;
+    for (int base = 0; base < ne; base += kTagTile) {
+        const int m = base + threadIdx.x;
+        __syncthreads();
+        if (m < ne) {
+            sx_[threadIdx.x] = xc[m];
+            sy_[threadIdx.x] = yc[m];
+            sz_[threadIdx.x] = zc[m];
+        }
+        __syncthreads();
+        const int cnt = min(kTagTile, ne - base);
+        for (int t = 0; t < cnt; t++) {
+            const double dy = n1y - sy_[t], dx = n1x - sx_[t], dz = n1z - sz_[t];
+            const double s = ((dy * dy) + (dx * dx)) + (dz * dz);
+            if (s < smin) {
+                const double dis = sqrt(s);
+                if (dis < minDis) {
+                    minDis = dis;
+                    smin = s;
+                    nel = base + t + 1;
+                }
+            }
+        }
+    }
+    if (!active) return;
+    flag[c] = 0;
+    if (nel < 1) {
+        atomicOr(err, 1);
+        return;
+    }
+    const int e = nel - 1;
+    const double cx = xc[e], cy = yc[e], cz = zc[e], a = ca[e], b = cb[e], g = cg[e];
+    const double n2x = x1[i], n3x = x1[i + 1], n2y = y1[j], n3y = y1[j + 1], n2z = z1[k], n3z = z1[k + 1];
+    double d[8];
+    d[0] = ((n2x - cx) * a + (n2y - cy) * b) + (n2z - cz) * g;
+    d[1] = ((n2x - cx) * a + (n3y - cy) * b) + (n2z - cz) * g;
+    d[2] = ((n3x - cx) * a + (n2y - cy) * b) + (n2z - cz) * g;
+    d[3] = ((n3x - cx) * a + (n3y - cy) * b) + (n2z - cz) * g;
+    d[4] = ((n2x - cx) * a + (n2y - cy) * b) + (n3z - cz) * g;
+    d[5] = ((n2x - cx) * a + (n3y - cy) * b) + (n3z - cz) * g;
+    d[6] = ((n3x - cx) * a + (n2y - cy) * b) + (n3z - cz) * g;
+    d[7] = ((n3x - cx) * a + (n3y - cy) * b) + (n3z - cz) * g;
+    bool all_in = true, all_out = true;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        all_in = all_in && (d[t] <= -1e-16);
+        all_out = all_out && (d[t] > -1e-16);
+    }
+    const int8_t nt = all_in ? 1 : (all_out ? 0 : 2);
+    if (tag[c] == 1 && nt != 1) {
+        u[c] = wu;
+        v[c] = wv;
+        w[c] = ww;
+    }
+    tag[c] = nt;
     minElem[c] = nel;
 }
 
@@ -383,10 +497,19 @@ struct LevelArgs {
     const double *ca, *cb, *cg;
     Coords C;
     int sx, sy;
+    double wall[3], acc[3];
 };
 
-// subForcing.f90:219-224 (velocity, u_wall = 0) and 92-98 (pressure,
-// dp/dn = 0) for the updates [lo, hi) of one dependency level
+// dp/dn at the wall from the body acceleration a: -(a . n); 0 for a body at
+// rest (subForcing.f90:11 and the disabled moving-body form at line 20)
+__device__ __forceinline__ double wall_dpdn(const double acc[3], double na, double nb, double ng) {
+    if (acc[0] == 0.0 && acc[1] == 0.0 && acc[2] == 0.0) return 0.0;
+    return -(((acc[0] * na) + (acc[1] * nb)) + (acc[2] * ng));
+}
+
+// subForcing.f90:219-224 (velocity; u_wall = 0 in the reference, the body's
+// velocity when it moves) and 92-98 (pressure; dp/dn = 0, or the body's
+// acceleration) for the updates [lo, hi) of one dependency level
 __global__ void k_force_level(LevelArgs A, int lo, int hi) {
     const int q = lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= hi) return;
@@ -409,13 +532,13 @@ __global__ void k_force_level(LevelArgs A, int lo, int hi) {
     const double n1 = pt1 + sur;
     double val;
     if (comp < 3) {
-        const double surf = 0.0;
+        const double surf = A.wall[comp];
         const double cval = surf;
         const double bval = (2.0 / n1) * (fpos - surf) - dn;
         const double aval = dn / n1 - (fpos - surf) / (n1 * n1);
         val = ((aval * (sur * sur)) + bval * sur) + cval;
     } else {
-        const double dpdn = 0.0;
+        const double dpdn = wall_dpdn(A.acc, na, nb, ng);
         const double bval = dpdn;
         const double aval = (dn - dpdn) / (2.0 * n1);
         const double cval = fpos - ((dn + dpdn) * n1) * 0.5;
@@ -446,6 +569,7 @@ struct StressArgs {
     int find;            // 1: locate the stencil cells (stressCal1)
     double *contrib;     // [7][ne]
     int *err;
+    double wall[3], acc[3];
 };
 
 __global__ void k_stress(StressArgs A) {
@@ -493,10 +617,10 @@ __global__ void k_stress(StressArgs A) {
         double fpos, dn;
         image_values(f, cx, cy, cz, ix, iy, iz, px, py, pz, na, nb, ng, &fpos, &dn);
         if (q < 3) {
-            const double surf = 0.0;
+            const double surf = A.wall[q];
             dns[q] = (2.0 / normdis) * (fpos - surf) - dn;
         } else {
-            const double dpdn = 0.0;
+            const double dpdn = wall_dpdn(A.acc, na, nb, ng);
             const double cval = fpos - ((dn + dpdn) * diagdis) * 0.5;
             const double alen = sqrt(((A.a3[e] * A.a3[e]) + (A.b3[e] * A.b3[e])) + (A.g3[e] * A.g3[e]));
             const double area = alen / 2.0;
@@ -569,6 +693,16 @@ int surface_norm(Ctx &c, int64_t inor) {
     return 0;
 }
 
+int shift_nodes_moving(Ctx &c, const double base[3], const double disp[3]) {
+    IB3D_REQUIRE(c.ibNodes > 0, "shiftSurfaceNodesMovingbody: no surface mesh");
+    hipLaunchKernelGGL(k_shift_nodes_moving, dim3(nblocks(c.ibNodes, 256)), dim3(256), 0, c.stream, c.xnode,
+                       c.ynode, c.znode, c.xnode1, c.ynode1, c.znode1, c.ibNodes, base[0], base[1], base[2], disp[0],
+                       disp[1], disp[2]);
+    IB3D_CHECK(hipGetLastError());
+    c.norms_ready = false;
+    return 0;
+}
+
 static int *g_err = nullptr;
 static int dev_err(Ctx &c) {
     if (!g_err) IB3D_CHECK(hipMalloc(&g_err, sizeof(int)));
@@ -581,6 +715,46 @@ void ib_release() {
 }
 
 int count_tags(Ctx &c, int32_t out[4], bool scatter);
+
+// subSearch.f90:327-463 selectiveRetagging
+int selective_retagging(Ctx &c, int32_t counts[3]) {
+    IB3D_REQUIRE(c.norms_ready, "selectiveRetagging: computeSurfaceNorm has not run");
+    IB3D_REQUIRE(c.lists_ready, "selectiveRetagging: cellCount has not run");
+    const int64_t nb = c.ibCount;
+    if (!c.rt_flag) {
+        IB3D_CHECK(hipMalloc(&c.rt_flag, (c.P + 4) * sizeof(uint8_t)));
+        IB3D_CHECK(hipMemsetAsync(c.rt_flag, 0, (c.P + 4) * sizeof(uint8_t), c.stream));
+        IB3D_CHECK(hipMalloc(&c.rt_count, sizeof(int)));
+    }
+    const int64_t cap = std::min<int64_t>(c.P, std::max<int64_t>(1, 216 * nb));
+    if (cap > c.rt_cap) {
+        IB3D_CHECK(hipStreamSynchronize(c.stream));
+        if (c.rt_list) IB3D_CHECK(hipFree(c.rt_list));
+        IB3D_CHECK(hipMalloc(&c.rt_list, cap * sizeof(int)));
+        c.rt_cap = cap;
+    }
+    IB3D_CHECK(hipMemsetAsync(c.minElem, 0, c.P * sizeof(int), c.stream));
+    IB3D_CHECK(hipMemsetAsync(c.rt_count, 0, sizeof(int), c.stream));
+    if (dev_err(c)) return -1;
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_retag_mark, dim3(nblocks(216 * nb, 256)), dim3(256), 0, c.stream, c.ibList, (int)nb,
+                           (int)c.nx, (int)c.ny, (int)c.nz, c.rt_flag, c.rt_list, c.rt_count);
+        hipLaunchKernelGGL(k_retag_list, dim3(nblocks(cap, kTagTile)), dim3(kTagTile), 0, c.stream, c.rt_list,
+                           c.rt_count, (int)c.sx, (int)c.sy, c.xp, c.yp, c.zp, c.x1, c.y1, c.z1, c.xcent, c.ycent,
+                           c.zcent, c.cosA, c.cosB, c.cosG, (int)c.ibElems, c.tag, c.minElem, c.rt_flag, c.u, c.v,
+                           c.w, c.wall_vel[0], c.wall_vel[1], c.wall_vel[2], g_err);
+    }
+    IB3D_CHECK(hipGetLastError());
+    IB3D_REQUIRE(read_err(c, g_err) == 0, "selectiveRetagging: a cell found no element closer than 1e14");
+    int32_t t4[4];
+    if (count_tags(c, t4, false)) return -1;
+    counts[0] = t4[2];
+    counts[1] = t4[0];
+    counts[2] = t4[1];
+    c.lists_ready = c.normdist_ready = c.plans_ready = false;
+    c.stress_ready = false;
+    return 0;
+}
 
 int tagging(Ctx &c, const int64_t w[6], int32_t counts[3]) {
     IB3D_REQUIRE(c.norms_ready, "tagging: computeSurfaceNorm has not run");
@@ -829,6 +1003,10 @@ static int run_plan(Ctx &c, ForcePlan &P) {
     A.ca = c.cosA; A.cb = c.cosB; A.cg = c.cosG;
     A.C = coords(c);
     A.sx = (int)c.sx; A.sy = (int)c.sy;
+    for (int q = 0; q < 3; q++) {
+        A.wall[q] = c.wall_vel[q];
+        A.acc[q] = c.wall_acc[q];
+    }
     for (int l = 0; l < P.n_levels; l++) {
         const int lo = P.level_start[l], hi = P.level_start[l + 1];
         if (hi > lo)
@@ -870,6 +1048,10 @@ int stress(Ctx &c, int first, double re, double coef[4]) {
     A.find = (first || !c.stress_ready) ? 1 : 0;
     A.contrib = contrib;
     A.err = g_err;
+    for (int q = 0; q < 3; q++) {
+        A.wall[q] = c.wall_vel[q];
+        A.acc[q] = c.wall_acc[q];
+    }
     hipLaunchKernelGGL(k_stress, dim3(nblocks(ne, 128)), dim3(128), 0, c.stream, A);
     hipLaunchKernelGGL(k_sum_rows, dim3(7), dim3(1024), 0, c.stream, contrib, ne, c.scalars + 8);
     IB3D_CHECK(hipGetLastError());

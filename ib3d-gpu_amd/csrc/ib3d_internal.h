@@ -131,6 +131,15 @@ struct Ctx {
     double *scontrib = nullptr;   // per-element stress contributions (7 x ibElems)
     bool stress_ready = false;
 
+    // rigid-body motion (zero for a fixed body): wall velocity used by the
+    // forcing, the solid cells and the surface forces; body acceleration
+    // for dp/dn at the wall; retagging scratch
+    double wall_vel[3] = {0.0, 0.0, 0.0}, wall_acc[3] = {0.0, 0.0, 0.0};
+    uint8_t *rt_flag = nullptr;    // P flags: cell queued for retagging
+    int *rt_list = nullptr;        // unique queued cells (linear index)
+    int *rt_count = nullptr;
+    int64_t rt_cap = 0;
+
     // Poisson configuration
     int mode = 0;                  // 0 reference rule, 1 FP64 L2, 2 mixed FP32/FP64
     double omega = 1.99;
@@ -181,6 +190,8 @@ int poisson_solve(Ctx &c, double deltat, double eps1, int64_t *niter, double *di
 int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2);
 int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);
 int build_force_plans(Ctx &c);
+int shift_nodes_moving(Ctx &c, const double base[3], const double disp[3]);
+int selective_retagging(Ctx &c, int32_t counts[3]);
 int velocity_bc(Ctx &c);
 
 // learned initial guess (ib3d_cnn.hip)

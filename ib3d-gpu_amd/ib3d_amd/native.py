@@ -68,6 +68,9 @@ def lib():
         "ib3d_step": [I, D, D, D, D, PD],
         "ib3d_bench_sor": [I, I, D, PD],
         "ib3d_probe_copy": [I64, PD],
+        "ib3d_shiftSurfaceNodesMovingbody": [D, D, D, D, D, D],
+        "ib3d_selectiveRetagging": [PI32],
+        "ib3d_set_wall_motion": [PD, PD],
         "ib3d_cnn_init": [C.c_uint64],
         "ib3d_cnn_release": [],
         "ib3d_cnn_set_params": [PF, I64],
@@ -184,6 +187,25 @@ class Solver:
         _ok(s.L.ib3d_coefficientMatrix())
         _ok(s.L.ib3d_non_uni_coeff())
         return s
+
+    def move_body(self, disp, vel=(0.0, 0.0, 0.0), acc=(0.0, 0.0, 0.0)):
+        """Prescribed rigid motion: nodes at (initial placement + disp), then
+        computeSurfaceNorm, selectiveRetagging, cellCount and
+        computeNormDistance; the wall moves with velocity `vel` and
+        acceleration `acc` in the forcing, solid cells and surface forces."""
+        inp = self.inp
+        v = np.array(vel, dtype=np.float64)
+        a = np.array(acc, dtype=np.float64)
+        _ok(self.L.ib3d_set_wall_motion(_dp(v), _dp(a)))
+        _ok(self.L.ib3d_shiftSurfaceNodesMovingbody(inp["xShift"], inp["yShift"], inp["zShift"], float(disp[0]),
+                                                    float(disp[1]), float(disp[2])))
+        _ok(self.L.ib3d_computeSurfaceNorm(inp["inor"]))
+        c3 = np.zeros(3, dtype=np.int32)
+        _ok(self.L.ib3d_selectiveRetagging(c3.ctypes.data_as(C.POINTER(C.c_int32))))
+        cnt5 = np.zeros(5, dtype=np.int32)
+        _ok(self.L.ib3d_cellCount(cnt5.ctypes.data_as(C.POINTER(C.c_int32))))
+        self.counts = dict(zip(["ib", "fluid", "solid", "red", "black"], cnt5.tolist()))
+        _ok(self.L.ib3d_computeNormDistance())
 
     # ----------------------------------------------------------- routines
     def nsMomentum(self):

@@ -61,6 +61,23 @@ int ib3d_set_mesh(int64_t n_nodes, const double *xnode, const double *ynode, con
 
 /* subSearch.f90:2-39 */
 int ib3d_shiftSurfaceNodesInitial(double xshift, double yshift, double zshift);
+/* Moving body.  The reference's shiftSurfaceNodesMovingbody (subSearch.f90:
+ * 41-76) integrates a free motion; this entry point places the body at a
+ * prescribed rigid displacement (dx,dy,dz) from its initial placement:
+ * xnode1 = (xnode + xshift) + dx.  The per-step sequence is then
+ * computeSurfaceNorm, selectiveRetagging, cellCount, computeNormDistance. */
+int ib3d_shiftSurfaceNodesMovingbody(double xshift, double yshift, double zshift, double dx, double dy,
+                                     double dz);
+/* subSearch.f90:327-512: re-tag the 6x6x6 block around every intercepted
+ * cell of the current lists (cells that stop being solid take the wall
+ * velocity); counts = {ib, fluid, solid} */
+int ib3d_selectiveRetagging(int32_t counts[3]);
+/* Wall velocity (used instead of the reference's usurf = vsurf = wsurf = 0,
+ * subForcing.f90:144-145, in the ghost-cell fit, the solid cells and the
+ * surface shear) and body acceleration (dp/dn = -a.n at the wall instead of
+ * 0, subForcing.f90:11 and 20).  Zero keeps the reference's fixed body bit for
+ * bit. */
+int ib3d_set_wall_motion(const double vel[3], const double acc[3]);
 /* subSearch.f90:80-131; inor is the normal sign read from inputdata */
 int ib3d_computeSurfaceNorm(int64_t inor);
 /* subSearch.f90:134-321; window = {i_start,i_end,j_start,j_end,k_start,k_end}

@@ -53,6 +53,11 @@ struct orc_state {
     double *pNormDis, *u1NormDis, *u2NormDis, *v1NormDis, *v2NormDis, *w1NormDis, *w2NormDis;
     int64_t *Elemcell, *ucell, *vcell, *wcell, *pcell;
     int stress_ready;
+    /* rigid-body motion (zero for a fixed body): wall velocity used by the
+     * forcing, the solid cells and the surface forces; body acceleration
+     * for dp/dn at the wall */
+    double wall_vel[3], wall_acc[3];
+    int64_t *cell_old;
     char err[256];
 };
 
@@ -116,6 +121,7 @@ static void free_mesh(orc_state *s) {
 }
 
 void orc_free(orc_state *s) {
+    if (s) free(s->cell_old);
     if (!s) return;
     free_lists(s);
     free_mesh(s);
@@ -315,6 +321,71 @@ int orc_tagging(orc_state *s, const int64_t win[6]) {
     return 0;
 }
 
+/* subSearch.f90:41-76 shiftSurfaceNodesMovingbody with the free motion
+ * replaced by a prescribed rigid displacement (dx,dy,dz) of the initially
+ * placed body: xnode1 = (xnode + xShift) + dx */
+int orc_shift_nodes_moving(orc_state *s, double xs, double ys, double zs, double dx, double dy, double dz) {
+    for (int64_t i = 1; i <= s->ibNodes; i++) {
+        s->xnode1[i] = (s->xnode[i] + xs) + dx;
+        s->ynode1[i] = (s->ynode[i] + ys) + dy;
+        s->znode1[i] = (s->znode[i] + zs) + dz;
+    }
+    return 0;
+}
+
+/* subSearch.f90:338-463 selectiveRetagging: the 6x6x6 block
+ * (i1-2..i1+3, j1-2..j1+3, k1-2..k1+3) around every intercepted cell of the
+ * current list is tagged again against the moved surface; every other cell
+ * keeps its tag.  minElemcell is cleared first (338-340).  Block cells outside
+ * the padded index range 1..n+2 are skipped (the reference would index its
+ * coordinate arrays outside their bounds there).  Cells that were solid and
+ * are no longer take the wall velocity (no counterpart in the reference,
+ * whose body never moves). */
+int orc_selective_retagging(orc_state *s) {
+    size_t P = (size_t)s->sx * s->sy * s->sz;
+    if (!s->cell_old) s->cell_old = zalloc(P, 8);
+    memcpy(s->cell_old, s->cell, P * 8);
+    memset(s->minElemcell, 0, P * 8);
+    int64_t nb = s->ibCount, nel = 0;
+    for (int64_t nn = 1; nn <= nb; nn++) {
+        int64_t i1 = LST(s->ibList, nb, nn, 1), j1 = LST(s->ibList, nb, nn, 2), k1 = LST(s->ibList, nb, nn, 3);
+        for (int64_t k = k1 - 2; k <= k1 + 3; k++)
+            for (int64_t j = j1 - 2; j <= j1 + 3; j++)
+                for (int64_t i = i1 - 2; i <= i1 + 3; i++) {
+                    if (i < 1 || i > s->nx + 2 || j < 1 || j > s->ny + 2 || k < 1 || k > s->nz + 2) continue;
+                    nel = nearest_element(s, s->xp[i], s->yp[j], s->zp[k], nel);
+                    if (nel < 1) return fail(s, "no element closer than 1e14");
+                    F(s->minElemcell, i, j, k) = nel;
+                    F(s->cell, i, j, k) = corner_tag(s, nel, s->x1[i], s->x1[i + 1], s->y1[j], s->y1[j + 1],
+                                                     s->z1[k], s->z1[k + 1]);
+                }
+    }
+    for (size_t q = 0; q < P; q++) {
+        if (s->cell_old[q] == 1 && s->cell[q] != 1) {
+            s->u[q] = s->wall_vel[0];
+            s->v[q] = s->wall_vel[1];
+            s->w[q] = s->wall_vel[2];
+        }
+    }
+    count_tags(s);
+    return 0;
+}
+
+int orc_set_wall_motion(orc_state *s, const double vel[3], const double acc[3]) {
+    for (int q = 0; q < 3; q++) {
+        s->wall_vel[q] = vel[q];
+        s->wall_acc[q] = acc[q];
+    }
+    return 0;
+}
+
+/* dp/dn at the wall from the body acceleration a: -(a . n); the reference's
+ * disabled form for a transverse motion is -(dv/dt) cosBeta (subForcing.f90:20) */
+static inline double wall_dpdn(const orc_state *s, double na, double nb, double ng) {
+    if (s->wall_acc[0] == 0.0 && s->wall_acc[1] == 0.0 && s->wall_acc[2] == 0.0) return 0.0;
+    return -(((s->wall_acc[0] * na) + (s->wall_acc[1] * nb)) + (s->wall_acc[2] * ng));
+}
+
 /* subSearch.f90:515-596 cellCount: lists in (k,j,i) order, red = mod(i+j+k,2)==1 */
 int orc_cell_count(orc_state *s) {
     free_lists(s);
@@ -375,6 +446,15 @@ int orc_cell_count(orc_state *s) {
 int orc_compute_norm_distance(orc_state *s) {
     int64_t nb = s->ibCount;
     size_t m = (size_t)nb + 1;
+    int64_t **ia[] = {&s->nelp, &s->nelu1, &s->nelu2, &s->nelv1, &s->nelv2, &s->nelw1, &s->nelw2};
+    double **da[] = {&s->pNormDis, &s->u1NormDis, &s->u2NormDis, &s->v1NormDis, &s->v2NormDis, &s->w1NormDis,
+                     &s->w2NormDis};
+    for (int q = 0; q < 7; q++) {
+        free(*ia[q]);
+        free(*da[q]);
+        *ia[q] = NULL;
+        *da[q] = NULL;
+    }
     s->nelp = zalloc(m, 8); s->nelu1 = zalloc(m, 8); s->nelu2 = zalloc(m, 8);
     s->nelv1 = zalloc(m, 8); s->nelv2 = zalloc(m, 8); s->nelw1 = zalloc(m, 8); s->nelw2 = zalloc(m, 8);
     s->pNormDis = zalloc(m, 8); s->u1NormDis = zalloc(m, 8); s->u2NormDis = zalloc(m, 8);
@@ -770,9 +850,9 @@ int orc_solid_cell_bc(orc_state *s) {
     int64_t ns = s->solidCount;
     for (int64_t n = 1; n <= ns; n++) {
         int64_t i = LST(s->solidList, ns, n, 1), j = LST(s->solidList, ns, n, 2), k = LST(s->solidList, ns, n, 3);
-        F(s->ut, i, j, k) = 0.0;
-        F(s->vt, i, j, k) = 0.0;
-        F(s->wt, i, j, k) = 0.0;
+        F(s->ut, i, j, k) = s->wall_vel[0];
+        F(s->vt, i, j, k) = s->wall_vel[1];
+        F(s->wt, i, j, k) = s->wall_vel[2];
         F(s->p, i, j, k) = 0.0;
     }
     return 0;
@@ -836,7 +916,7 @@ static void image_interp(const orc_state *s, const double *Fd, const comp_t *c, 
 static int ghost_face(orc_state *s, double *Fd, const comp_t *c, int64_t i, int64_t j, int64_t k,
                       double nx0, double ny0, double nz0, double sur, int64_t e, int64_t sxh, int64_t syh,
                       int64_t szh, int clamp_axis, int64_t from, int64_t to, int64_t idx[3], double *out) {
-    const double surf = 0.0;
+    const double surf = s->wall_vel[c->ox ? 0 : c->oy ? 1 : 2];
     double pt1 = 1.5 * sqrt(((s->deltax[i] * s->deltax[i]) + (s->deltay[j] * s->deltay[j])) +
                             (s->deltaz[k] * s->deltaz[k])) + (fabs(sur) - sur) * 0.5;
     double na = s->cosAlpha[e], nb = s->cosBeta[e], ng = s->cosGamma[e];
@@ -894,7 +974,6 @@ int orc_pressure_forcing(orc_state *s) {
     const int64_t nx = s->nx, ny = s->ny, nz = s->nz, nb = s->ibCount;
     comp_t cp = {s->xp, s->yp, s->zp, 0, 0, 0};
     int64_t idx[3] = {-1, -1, -1};
-    const double dpdn = 0.0;
     for (int64_t n = 1; n <= nb; n++) {
         int64_t i = LST(s->ibList, nb, n, 1), j = LST(s->ibList, nb, n, 2), k = LST(s->ibList, nb, n, 3);
         double sur = s->pNormDis[n];
@@ -909,6 +988,7 @@ int orc_pressure_forcing(orc_state *s) {
         if (idx[0] < 1 || idx[1] < 1 || idx[2] < 1) return fail(s, "image point outside the grid lines");
         double fpos, dn;
         image_interp(s, s->p, &cp, px, py, pz, idx[0], idx[1], idx[2], na, nb_, ng, &fpos, &dn);
+        const double dpdn = wall_dpdn(s, na, nb_, ng);
         double n1 = pt1 + sur;
         double bval = dpdn;
         double aval = (dn - dpdn) / (2.0 * n1);
@@ -1065,7 +1145,8 @@ int orc_stress(orc_state *s, int first, double out[4]) {
         double normdis = diagdis;
         double na = s->cosAlpha[e], nb = s->cosBeta[e], ng = s->cosGamma[e];
         double px = xs + normdis * na, py = ys + normdis * nb, pz = zs + normdis * ng;
-        const double usurf = 0.0, vsurf = 0.0, wsurf = 0.0, dpdn = 0.0;
+        const double usurf = s->wall_vel[0], vsurf = s->wall_vel[1], wsurf = s->wall_vel[2];
+        const double dpdn = wall_dpdn(s, na, nb, ng);
         double fpos, dn, dudn_s, dvdn_s, dwdn_s;
         const comp_t *cs[3] = {&cu, &cv, &cw};
         int64_t *cache[3] = {s->ucell, s->vcell, s->wcell};

@@ -36,6 +36,10 @@ int orc_shift_nodes_initial(orc_state *s, double xs, double ys, double zs);
 int orc_compute_surface_norm(orc_state *s);
 int orc_tagging(orc_state *s, const int64_t window[6]);
 int orc_cell_count(orc_state *s);
+/* moving body: prescribed displacement, selective retagging, wall motion */
+int orc_shift_nodes_moving(orc_state *s, double xs, double ys, double zs, double dx, double dy, double dz);
+int orc_selective_retagging(orc_state *s);
+int orc_set_wall_motion(orc_state *s, const double vel[3], const double acc[3]);
 int orc_compute_norm_distance(orc_state *s);
 int orc_initial_conditions(orc_state *s);
 int orc_coefficient_matrix(orc_state *s);

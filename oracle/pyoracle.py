@@ -56,6 +56,9 @@ def lib():
         L.orc_sor_l2.argtypes = [P, D, D, I64, I64, PI, PD]
         L.orc_array.argtypes = [P, C.c_char_p, C.POINTER(C.c_void_p), PI, C.POINTER(C.c_int)]
         L.orc_counts.argtypes = [P, PI]
+        L.orc_shift_nodes_moving.argtypes = [P, D, D, D, D, D, D]
+        L.orc_selective_retagging.argtypes = [P]
+        L.orc_set_wall_motion.argtypes = [P, PD, PD]
         _lib = L
     return _lib
 
@@ -122,6 +125,22 @@ class Oracle:
         return o
 
     # ---------------------------------------------------------------- access
+    def move_body(self, disp, vel=(0.0, 0.0, 0.0), acc=(0.0, 0.0, 0.0)):
+        """Prescribed rigid motion: nodes at (initial placement + disp), then
+        computeSurfaceNorm, selectiveRetagging, cellCount and
+        computeNormDistance; the wall moves with velocity `vel` and
+        acceleration `acc` in the forcing, solid cells and surface forces."""
+        inp = self.inp
+        v = np.array(vel, dtype=np.float64)
+        a = np.array(acc, dtype=np.float64)
+        self._ok(self.L.orc_set_wall_motion(self.h, _dp(v), _dp(a)))
+        self._ok(self.L.orc_shift_nodes_moving(self.h, inp["xShift"], inp["yShift"], inp["zShift"],
+                                               float(disp[0]), float(disp[1]), float(disp[2])))
+        self._ok(self.L.orc_compute_surface_norm(self.h))
+        self._ok(self.L.orc_selective_retagging(self.h))
+        self._ok(self.L.orc_cell_count(self.h))
+        self._ok(self.L.orc_compute_norm_distance(self.h))
+
     def array(self, name):
         ptr = C.c_void_p()
         cnt = C.c_int64()

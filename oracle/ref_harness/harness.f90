@@ -9,6 +9,12 @@
 !   ref_harness run    NSTEPS DUMPDIR [STEP ...]   full steps, stage dumps at STEPs
 !   ref_harness sample NSWEEPS                     time the stages of step 1 and
 !                                                  NSWEEPS single SOR iterations
+!   ref_harness move   NSTEPS DUMPDIR DY           rigid body shifted by s*DY in y
+!                                                  at step s, then the reference's
+!                                                  computeSurfaceNorm,
+!                                                  selectiveRetagging, cellCount and
+!                                                  computeNormDistance; geometry
+!                                                  dumps per step (no flow solve)
 !
 ! This is test infrastructure: only the tests, the smoke check and the CPU
 ! baseline leg of the benchmark run it.
@@ -45,7 +51,8 @@ program ref_harness
   character(len=512) :: arg, mode
   integer(8) :: nsteps, nsw, s, k
   integer :: nargs, ia, ndump
-  integer(8) :: dsteps(64)
+  integer(8) :: dsteps(64), inode
+  real(8) :: move_dy, yc
   integer(8) :: c0, c1, crate
   real(8) :: t_setup, t_mom, t_bc, t_sol, t_frc, t_div, t_sor, t_corr, t_std, t_prf, t_str
   real(8) :: derr1, derr2, huge_eps
@@ -61,6 +68,11 @@ program ref_harness
   call get_command_argument(2, arg)
   read(arg, *) nsteps
   ndump = 0
+  if (trim(mode) == 'move') then
+    call get_command_argument(3, dumpdir)
+    call get_command_argument(4, arg)
+    read(arg, *) move_dy
+  end if
   if (trim(mode) == 'run') then
     call get_command_argument(3, dumpdir)
     do ia = 4, nargs
@@ -91,6 +103,33 @@ program ref_harness
   ita1 = 1
   totime = 0.
   totime = totime + deltat
+
+  if (trim(mode) == 'move') then
+    open(newunit=ustep, file=trim(dumpdir)//'/move_steps.txt', status='replace')
+    do s = 1, nsteps
+      yc = real(s, 8) * move_dy
+      do inode = 1, ibNodes
+        xnode1(inode) = (xnode(inode) + xShift) + 0.0d0
+        ynode1(inode) = (ynode(inode) + yShift) + yc
+        znode1(inode) = (znode(inode) + zShift) + 0.0d0
+      end do
+      DEALLOCATE(xcent, ycent, zcent, cosAlpha, cosBeta, cosGamma, alpha3, beta3, gamma3)
+      call computeSurfaceNorm
+      call selectiveRetagging
+      call dump_move(s, 'rt', .false.)
+      DEALLOCATE(interceptedIndexPtr, fluidIndexPtr, solidIndexPtr, redCellIndexPtr, blackCellIndexPtr)
+      call cellCount
+      DEALLOCATE(pNormDis, nelp, nelu1, nelu2, nelv1, nelv2, nelw1, nelw2, &
+                 u1NormDis, u2NormDis, v1NormDis, v2NormDis, w1NormDis, w2NormDis)
+      call computeNormDistance
+      call dump_move(s, 'nd', .true.)
+      write(ustep, '(I8,ES25.17,5I12)') s, yc, ibCellCount, fluidCellCount, solidCellCount, &
+                                        redCellCount, blackCellCount
+      flush(ustep)
+    end do
+    close(ustep)
+    stop
+  end if
 
   if (trim(mode) == 'sample') then
     nsw = nsteps
@@ -272,6 +311,34 @@ contains
     call dump_i8('vcell', vcell, ibElems*3)
     call dump_i8('wcell', wcell, ibElems*3)
     call dump_i8('pcell', pcell, ibElems*3)
+  end subroutine
+
+  subroutine dump_move(s, tag, lists)
+    integer(8), intent(in) :: s
+    character(len=*), intent(in) :: tag
+    logical, intent(in) :: lists
+    character(len=32) :: pre
+    write(pre, '(A,I4.4,A,A,A)') 'm', s, '_', tag, '_'
+    if (.not. lists) then
+      call dump_i8(trim(pre)//'cell', cell, (nx+2)*(ny+2)*(nz+2))
+      call dump_i8(trim(pre)//'minElemcell', minElemcell, (nx+2)*(ny+2)*(nz+2))
+      call dump_r8(trim(pre)//'ycent', ycent, ibElems)
+      call dump_r8(trim(pre)//'cosBeta', cosBeta, ibElems)
+      return
+    end if
+    call dump_i8(trim(pre)//'interceptedIndexPtr', interceptedIndexPtr, int(ibCellCount, 8)*3)
+    call dump_i8(trim(pre)//'fluidIndexPtr', fluidIndexPtr, int(fluidCellCount, 8)*3)
+    call dump_i8(trim(pre)//'solidIndexPtr', solidIndexPtr, int(solidCellCount, 8)*3)
+    call dump_i8(trim(pre)//'redCellIndexPtr', redCellIndexPtr, int(redCellCount, 8)*3)
+    call dump_i8(trim(pre)//'blackCellIndexPtr', blackCellIndexPtr, int(blackCellCount, 8)*3)
+    call dump_i8(trim(pre)//'nelp', nelp, int(ibCellCount, 8))
+    call dump_r8(trim(pre)//'pNormDis', pNormDis, int(ibCellCount, 8))
+    call dump_r8(trim(pre)//'u1NormDis', u1NormDis, int(ibCellCount, 8))
+    call dump_r8(trim(pre)//'u2NormDis', u2NormDis, int(ibCellCount, 8))
+    call dump_r8(trim(pre)//'v1NormDis', v1NormDis, int(ibCellCount, 8))
+    call dump_r8(trim(pre)//'v2NormDis', v2NormDis, int(ibCellCount, 8))
+    call dump_r8(trim(pre)//'w1NormDis', w1NormDis, int(ibCellCount, 8))
+    call dump_r8(trim(pre)//'w2NormDis', w2NormDis, int(ibCellCount, 8))
   end subroutine
 
   subroutine dump_stage(s, tag)

@@ -16,6 +16,10 @@ Usage:
   python tests/golden/make_golden.py tiny            # ~1 s
   python tests/golden/make_golden.py cfg1 [RUN_DIR]  # 200 steps, ~25 min
       (RUN_DIR: an existing harness run of the same case to reuse)
+  python tests/golden/make_golden.py move            # moving-body geometry, ~2 s
+      (tiny case, body shifted by s*0.03 in y at step s = 1..8; the
+       reference's computeSurfaceNorm, selectiveRetagging, cellCount and
+       computeNormDistance after every shift)
 """
 import hashlib
 import json
@@ -80,10 +84,42 @@ def run_harness(case, nsteps, dump_steps):
     return dump
 
 
+MOVE_DY = 0.03
+MOVE_STEPS = 8
+
+
+def make_move():
+    tmp = tempfile.mkdtemp(prefix="golden_move_")
+    try:
+        case = tmp
+        cases.write_case(cases.CONFIGS["tiny"], case)
+        dump = os.path.join(case, "dump")
+        os.makedirs(dump)
+        subprocess.run([HARNESS, "move", str(MOVE_STEPS), dump, repr(MOVE_DY)], cwd=case, check=True,
+                       stdout=subprocess.DEVNULL)
+        g = {"inputs": input_digests(case), "dy": MOVE_DY, "nsteps": MOVE_STEPS, "arrays": {}, "steps": []}
+        for fn in sorted(os.listdir(dump)):
+            if fn.endswith((".f64", ".i64")):
+                g["arrays"][fn] = digest(os.path.join(dump, fn))
+        with open(os.path.join(dump, "move_steps.txt")) as fh:
+            for line in fh:
+                p = line.split()
+                g["steps"].append({"step": int(p[0]), "yc": float(p[1]),
+                                   "counts": dict(zip(["ib", "fluid", "solid", "red", "black"],
+                                                      [int(x) for x in p[2:7]]))})
+        with open(os.path.join(HERE, "move_reference.json"), "w") as fh:
+            json.dump(g, fh, indent=0, sort_keys=True)
+        print(f"wrote move fixtures: {len(g['arrays'])} array digests, {len(g['steps'])} steps")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     which = sys.argv[1] if len(sys.argv) > 1 else "tiny"
     if not os.path.exists(HARNESS):
         raise SystemExit(f"{HARNESS} missing: build it with make -C oracle -f Makefile.ref")
+    if which == "move":
+        return make_move()
     cfg = cases.CONFIGS[which]
     if which == "tiny":
         nsteps, dump_steps = 3, [1, 2, 3]

@@ -133,3 +133,33 @@ def test_cfg1_reference_trajectory_is_physical():
     assert len(cd) == 200
     assert 0.9 < cd[-1] < 1.3
     assert all(s["nIterPcor"] > 0 for s in g["steps"])
+
+
+MOVE_RT = {"cell": "cell.i64", "minElemcell": "minElemcell.i64", "ycent": "ycent.f64", "cosBeta": "cosBeta.f64"}
+MOVE_ND = {"interceptedIndexPtr": "interceptedIndexPtr.i64", "fluidIndexPtr": "fluidIndexPtr.i64",
+           "solidIndexPtr": "solidIndexPtr.i64", "redCellIndexPtr": "redCellIndexPtr.i64",
+           "blackCellIndexPtr": "blackCellIndexPtr.i64", "nelp": "nelp.i64", "pNormDis": "pNormDis.f64",
+           "u1NormDis": "u1NormDis.f64", "u2NormDis": "u2NormDis.f64", "v1NormDis": "v1NormDis.f64",
+           "v2NormDis": "v2NormDis.f64", "w1NormDis": "w1NormDis.f64", "w2NormDis": "w2NormDis.f64"}
+
+
+def test_moving_body_geometry_matches_reference(case_dir):
+    """Rigid shift of the tiny case's sphere by s*0.03 in y at steps 1..8
+    (about a quarter cell per step): the reference's computeSurfaceNorm,
+    selectiveRetagging (6x6x6 blocks around the previous intercepted cells),
+    cellCount and computeNormDistance, bit for bit."""
+    import pyoracle
+    g = load("move_reference.json")
+    d = case_dir("tiny")
+    assert input_digests(d) == g["inputs"]
+    o = pyoracle.Oracle.from_case(d)
+    for st in g["steps"]:
+        s = st["step"]
+        o.move_body((0.0, st["yc"], 0.0))
+        c = o.counts()
+        for k in ["ib", "fluid", "solid", "red", "black"]:
+            assert c[k] == st["counts"][k], (s, k)
+        for name, key in MOVE_RT.items():
+            assert dig(o.array(name)) == g["arrays"][f"m{s:04d}_rt_{key}"], (s, name)
+        for name, key in MOVE_ND.items():
+            assert dig(o.array(name)) == g["arrays"][f"m{s:04d}_nd_{key}"], (s, name)

@@ -46,6 +46,19 @@ MODULE ib3d_cabi
       IMPORT :: c_int, c_double
       REAL(c_double), VALUE :: xs, ys, zs
     END FUNCTION
+    INTEGER(c_int) FUNCTION ib3d_shiftSurfaceNodesMovingbody(xs, ys, zs, dx, dy, dz) &
+        BIND(C, name='ib3d_shiftSurfaceNodesMovingbody')
+      IMPORT :: c_int, c_double
+      REAL(c_double), VALUE :: xs, ys, zs, dx, dy, dz
+    END FUNCTION
+    INTEGER(c_int) FUNCTION ib3d_selectiveRetagging(counts) BIND(C, name='ib3d_selectiveRetagging')
+      IMPORT :: c_int, c_int32_t
+      INTEGER(c_int32_t), DIMENSION(3) :: counts
+    END FUNCTION
+    INTEGER(c_int) FUNCTION ib3d_set_wall_motion(vel, acc) BIND(C, name='ib3d_set_wall_motion')
+      IMPORT :: c_int, c_double
+      REAL(c_double), DIMENSION(3) :: vel, acc
+    END FUNCTION
     INTEGER(c_int) FUNCTION ib3d_computeSurfaceNorm(inor) BIND(C, name='ib3d_computeSurfaceNorm')
       IMPORT :: c_int, c_int64_t
       INTEGER(c_int64_t), VALUE :: inor

@@ -209,11 +209,44 @@ SUBROUTINE shiftSurfaceNodesInitial          ! subSearch.f90:2-39
   CALL ib3d_update_host('xnode1'); CALL ib3d_update_host('ynode1'); CALL ib3d_update_host('znode1')
 END SUBROUTINE shiftSurfaceNodesInitial
 
+! subSearch.f90:41-76 with the free motion replaced by the prescribed
+! transverse oscillation y_c(t) = a0 sin(2 pi freq t) (a0, freq from inputdata);
+! the body's velocity and acceleration (finite difference of the velocity,
+! the form of subForcing.f90:20) move the wall in the forcing and stress
+SUBROUTINE shiftSurfaceNodesMovingbody
+  USE global
+  USE ib3d_cabi
+  IMPLICIT NONE
+  REAL(c_double) :: vel(3), acc(3)
+  v_init = v_final
+  xmove = 0.0d0
+  ymove = a0 * SIN(2.0d0 * pi * freq * totime)
+  zmove = 0.0d0
+  v_final = a0 * (2.0d0 * pi * freq) * COS(2.0d0 * pi * freq * totime)
+  vel = [0.0d0, v_final, 0.0d0]
+  acc = [0.0d0, (v_final - v_init) / deltat, 0.0d0]
+  CALL ib3d_check(ib3d_set_wall_motion(vel, acc), 'set_wall_motion')
+  CALL ib3d_check(ib3d_shiftSurfaceNodesMovingbody(xShift, yShift, zShift, xmove, ymove, zmove), &
+                  'shiftSurfaceNodesMovingbody')
+END SUBROUTINE shiftSurfaceNodesMovingbody
+
+SUBROUTINE selectiveRetagging                ! subSearch.f90:327-512
+  USE global
+  USE ib3d_cabi
+  IMPLICIT NONE
+  INTEGER(c_int32_t) :: counts(3)
+  CALL ib3d_check(ib3d_selectiveRetagging(counts), 'selectiveRetagging')
+  ibCellCount = counts(1)
+  fluidCellCount = counts(2)
+  solidCellCount = counts(3)
+END SUBROUTINE selectiveRetagging
+
 SUBROUTINE computeSurfaceNorm                ! subSearch.f90:80-131
   USE global
   USE ib3d_cabi
   IMPLICIT NONE
-  ALLOCATE(xcent(ibElems), ycent(ibElems), zcent(ibElems), cosAlpha(ibElems), cosBeta(ibElems), cosGamma(ibElems))
+  IF (.NOT. ALLOCATED(xcent)) &
+    ALLOCATE(xcent(ibElems), ycent(ibElems), zcent(ibElems), cosAlpha(ibElems), cosBeta(ibElems), cosGamma(ibElems))
   CALL ib3d_check(ib3d_computeSurfaceNorm(inor), 'computeSurfaceNorm')
   CALL ib3d_update_host('xcent'); CALL ib3d_update_host('ycent'); CALL ib3d_update_host('zcent')
   CALL ib3d_update_host('cosAlpha'); CALL ib3d_update_host('cosBeta'); CALL ib3d_update_host('cosGamma')

@@ -18,6 +18,9 @@ MODULE global
                      k_startSearch, k_endSearch, surGeoPoints, ibElems, ibNodes
   REAL(KIND=8) :: re, rev, eps1, dt_order, deltat, totime, divmax, dstart, dfinish, pi, &
                   u0, v0, w0, a0, freq, aoa, piv_pt, xShift, yShift, zShift
+  ! body motion (modGlobal.f90:23-24)
+  REAL(KIND=8) :: u_init = 0.0d0, u_final = 0.0d0, v_init = 0.0d0, v_final = 0.0d0, &
+                  w_init = 0.0d0, w_final = 0.0d0, xmove = 0.0d0, ymove = 0.0d0, zmove = 0.0d0
   ! grid (modGlobal.f90:29-30; shapes of subReadInput.f90:48-53)
   REAL(KIND=8), ALLOCATABLE, TARGET :: x1(:), y1(:), z1(:), deltax(:), deltay(:), deltaz(:), &
                                        xu(:), yu(:), zu(:), xv(:), yv(:), zv(:), &

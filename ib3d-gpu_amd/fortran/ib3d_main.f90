@@ -3,7 +3,11 @@
 ! Same call sequence, same files in and out (inputdata, gridx/y/z.dat,
 ! geometries/sphere.msh; iter.dat, dragcoff.dat, liftcoff.dat, result,
 ! tag/, out/).  An optional first command-line argument caps the number of
-! steps below itamax, for short checks.
+! steps below itamax, for short checks.  With a nonzero amplitude a0 in
+! inputdata the body oscillates (y_c = a0 sin(2 pi freq t)) and the geometry
+! is rebuilt every step (shiftSurfaceNodesMovingbody, computeSurfaceNorm,
+! selectiveRetagging, cellCount, computeNormDistance; the sequence of the
+! reference's unwired moving-body routines, subSearch.f90:41-512).
 PROGRAM ib3d_main
   USE global
   USE ib3d_cabi
@@ -33,6 +37,13 @@ PROGRAM ib3d_main
   ita1 = 1
   totime = 0.
   totime = totime + deltat
+  IF (a0 /= 0.0d0) THEN
+    CALL shiftSurfaceNodesMovingbody
+    CALL computeSurfaceNorm
+    CALL selectiveRetagging
+    CALL cellCount
+    CALL computeNormDistance
+  END IF
   CALL nsMomentum
   CALL velocityBC
   CALL solidCellBC
@@ -46,6 +57,13 @@ PROGRAM ib3d_main
     ita = ita + 1
     ita1 = ita1 + 1
     totime = totime + deltat
+    IF (a0 /= 0.0d0) THEN
+      CALL shiftSurfaceNodesMovingbody
+      CALL computeSurfaceNorm
+      CALL selectiveRetagging
+      CALL cellCount
+      CALL computeNormDistance
+    END IF
     CALL nsMomentum
     CALL velocityBC
     CALL solidCellBC
@@ -54,7 +72,11 @@ PROGRAM ib3d_main
     CALL poissonSolver
     CALL pressureForcing1
     CALL writeOutput
-    CALL stressCal2
+    IF (a0 /= 0.0d0) THEN
+      CALL stressCal1
+    ELSE
+      CALL stressCal2
+    END IF
     CALL writeResult
     IF (ita >= itamax) EXIT
   END DO

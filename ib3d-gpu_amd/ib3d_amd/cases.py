@@ -223,7 +223,7 @@ def search_window(nodes_1d: np.ndarray, centre: float, radius: float, h: float, 
     return int(sel[0]) + 2, int(sel[-1]) + 2
 
 
-def write_case(cfg: CaseConfig, directory: str, istart: int = 0) -> dict:
+def write_case(cfg: CaseConfig, directory: str, istart: int = 0, a0: float = None, freq: float = None) -> dict:
     """Write inputdata, grid?.dat and geometries/sphere.msh; return metadata.
     ``istart`` = 1 makes the driver restart from a ``result`` file
     (main.f90:15-16)."""
@@ -246,7 +246,12 @@ def write_case(cfg: CaseConfig, directory: str, istart: int = 0) -> dict:
         fh.write(f"      {cfg.re!r},\n")
         fh.write(f"      {cfg.steps}, {cfg.eps1!r}, 35000,\n")
         fh.write("      0., 0., 0.,\n")
-        fh.write(f"      {cfg.n_points}, 0, 0., 0, 0.,\n")
+        amp = cfg.extra.get("a0", 0.0) if a0 is None else a0
+        fr = cfg.extra.get("freq", 0.0) if freq is None else freq
+        if amp == 0.0 and fr == 0.0:
+            fh.write(f"      {cfg.n_points}, 0, 0., 0, 0.,\n")
+        else:  # oscillating body: y_c = a0 sin(2 pi freq t)
+            fh.write(f"      {cfg.n_points}, {float(amp)!r}, {float(fr)!r}, 0, 0.,\n")
         fh.write(f"      {cfg.shift[0]!r}, {cfg.shift[1]!r}, {cfg.shift[2]!r},\n")
         fh.write(f"      {int(istart)}, {cfg.dt_order!r}, {cfg.inor},\n")
         fh.write(f"      {win[0][0]}, {win[0][1]},\n")

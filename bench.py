@@ -34,6 +34,7 @@ per step.
 """
 import argparse
 import json
+import math
 import os
 import shutil
 import subprocess
@@ -53,8 +54,28 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None):
+class Motion:
+    """Prescribed transverse oscillation y_c = a0 sin(2 pi f t): per step the
+    body is moved, its geometry rebuilt (selective retagging) and the wall
+    velocity and acceleration (finite difference, as the reference's
+    disabled form) passed to the forcing and the surface forces."""
+
+    def __init__(self, a0, freq, dt):
+        self.a0, self.w, self.dt = a0, 2.0 * math.pi * freq, dt
+        self.t, self.v_prev = 0.0, 0.0
+
+    def advance(self, s):
+        self.t = self.t + self.dt
+        y = self.a0 * math.sin(self.w * self.t)
+        v = self.a0 * self.w * math.cos(self.w * self.t)
+        s.move_body((0.0, y, 0.0), (0.0, v, 0.0), (0.0, (v - self.v_prev) / self.dt, 0.0))
+        self.v_prev = v
+        s.first_step = True
+
+
+def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None, moving=None):
     s = solver_cls.from_case(case)
+    motion = Motion(moving[0], moving[1], s.deltat) if moving else None
     if net is not None:
         net.config(record=False, guess=mode == "mixed+learned")
     if mode == "fp64":
@@ -68,6 +89,8 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None)
     else:
         raise SystemExit(f"unknown mode {mode}")
     for q in range(warmup):
+        if motion:
+            motion.advance(s)
         r = s.step()
         st = s.poisson_stats()
         log(f"[bench] {mode} warmup {q + 1}/{warmup}: sweeps={st['sweeps']} relres={st['relres']:.2e} "
@@ -75,7 +98,13 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None)
     s.sync()
     sweeps, sor_ms, poi_ms, cds = [], [], [], []
     t0 = time.perf_counter()
+    geo_s = 0.0
     for q in range(steps):
+        if motion:
+            tg = time.perf_counter()
+            motion.advance(s)
+            s.sync()
+            geo_s += time.perf_counter() - tg
         r = s.step()
         st = s.poisson_stats()
         sweeps.append(st["sweeps"])
@@ -98,6 +127,8 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None)
         "poisson_ms_per_step": sum(poi_ms) / steps,
         "cd_last": cds[-1],
     }
+    if motion:
+        out["geometry_ms_per_step"] = 1e3 * geo_s / steps
     if mode == "mixed+learned":
         st = net.stats()
         out["learned_guess"] = {"solves": st["solves"], "kept": st["kept"], "guess_ms": round(st["guess_ms"], 3)}
@@ -228,7 +259,8 @@ def main():
         if "mixed+learned" in modes:
             net, net_info = train_guess(a.train_config, a.rec_steps, a.epochs, 4, 1e-3, a.omega, a.tol, a.inner)
         t0 = time.perf_counter()
-        s, main_res = run_mode(Solver, case, a.mode, a.steps, a.warmup, a.omega, a.tol, a.inner, net)
+        moving = (cfg.extra["a0"], cfg.extra["freq"]) if cfg.extra.get("a0") else None
+        s, main_res = run_mode(Solver, case, a.mode, a.steps, a.warmup, a.omega, a.tol, a.inner, net, moving)
         main_fields = {f: s.download(f) for f in ("u", "p")}
         log(f"[bench] {a.mode}: {main_res}")
         prec = 32 if a.mode.startswith("mixed") else 64
@@ -238,7 +270,7 @@ def main():
         cmp_all = []
         errors = {}
         for m in modes[1:]:
-            s2, r2 = run_mode(Solver, case, m, a.steps, a.warmup, a.omega, a.tol, a.inner, net)
+            s2, r2 = run_mode(Solver, case, m, a.steps, a.warmup, a.omega, a.tol, a.inner, net, moving)
             if m.startswith("fp64"):
                 # field difference of the headline mode against this FP64 run after the same steps
                 errors["vs_" + m] = {f: float(np.max(np.abs(main_fields[f] - s2.download(f))) /
@@ -273,7 +305,9 @@ def main():
             "vs_baseline": None,
             "dtype": "fp32+fp64" if a.mode.startswith("mixed") else "fp64",
             "data": "synthetic (generated grid and icosphere, reference initial condition)",
-            "config": {"workload": f"fixed sphere Re=100, {nx}x{ny}x{nz} stretched grid, "
+            "config": {"workload": (f"sphere oscillating as y_c = {moving[0]} sin(2 pi {moving[1]} t) "
+                                    f"(geometry rebuilt every step)" if moving else "fixed sphere")
+                                   + f" Re=100, {nx}x{ny}x{nz} stretched grid, "
                                    f"{cfg.level and 20 * 4 ** cfg.level} triangles, dt={cfg.dt}",
                        "poisson": f"{a.mode}: RB-SOR omega={a.omega}, ||r||/||b|| <= {a.tol}, "
                                   f"{a.inner} sweeps between residual checks, "

@@ -452,6 +452,23 @@ k_resid_pz(const double *__restrict__ p0, const double *__restrict__ p1, const f
     block_sum_to(rr, partial, (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
+// Cells the solve does not update (b/dt is NaN there) must hold pc = 0 and
+// e = 0: they are the Dirichlet values of their fluid neighbours.  A warm
+// start keeps the previous solution, so cells that stopped being fluid
+// (a moving body) are cleared before every solve.
+__global__ void __launch_bounds__(kSorBlock)
+k_clear_dead(double *p0, double *p1, float *e0, float *e1, const double *b0, const double *b1, int64_t NC) {
+    const int64_t f2 = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
+    if (f2 >= 2 * NC) return;
+    const int colour = f2 >= NC;
+    const int64_t f = colour ? f2 - NC : f2;
+    const double bd = (colour ? b1 : b0)[f];
+    if (bd != bd) {
+        (colour ? p1 : p0)[f] = 0.0;
+        (colour ? e1 : e0)[f] = 0.0f;
+    }
+}
+
 // p <- p + e on both colours (fluid cells only)
 __global__ void __launch_bounds__(kSorBlock)
 k_add_e(double *p0, double *p1, const float *e0, const float *e1, const double *b0, const double *b1, int64_t NC) {
@@ -726,6 +743,8 @@ static int sor_l2(Ctx &c, bool mixed, double bn, int64_t *sweeps, int64_t *outer
     const int64_t max_outer = std::max<int64_t>(1, c.max_sweeps / c.check);
     if (ensure_slots(2 * (max_outer + 2))) return -1;
     double *relv = c.slots, *conv = c.slots + c.n_slots / 2;
+    hipLaunchKernelGGL(k_clear_dead, dim3(nblocks(2 * c.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream, c.pc[0],
+                       c.pc[1], c.e32[0], c.e32[1], c.bdt[0], c.bdt[1], c.NC);
     // learned initial guess / training-pair recording: residual of p_prev
     // (FP32 copy in r32) into scalars[16], the guess's into scalars[18]
     const bool cnn = cnn_active();

@@ -211,6 +211,11 @@ CONFIGS = {
     "cfg2": CaseConfig("cfg2", (256, 192, 192), (-5.0, -5.0, -5.0), (15.0, 5.0, 5.0),
                        (-1.0, -1.0, -1.0), (2.0, 1.0, 1.0), 1.0 / 40.0, 5, 0.002,
                        steps=500),
+    # config 3: the sphere oscillating transversely, y_c = 0.25 sin(2 pi 0.2 t),
+    # 384x256x256 with the uniform h = D/40 band covering the motion envelope
+    "cfg3": CaseConfig("cfg3", (384, 256, 256), (-5.0, -5.0, -5.0), (15.0, 5.0, 5.0),
+                       (-1.0, -1.25, -1.0), (2.0, 1.25, 1.0), 1.0 / 40.0, 5, 0.002,
+                       steps=1000, extra={"a0": 0.25, "freq": 0.2}),
 }
 
 

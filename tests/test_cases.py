@@ -9,7 +9,7 @@ from ib3d_amd import cases
 REFERENCE = "/root/reference"
 
 
-@pytest.mark.parametrize("name", ["tiny", "small", "cfg1", "cfg2"])
+@pytest.mark.parametrize("name", ["tiny", "small", "cfg1", "cfg2", "cfg3"])
 def test_stretched_axes(name):
     cfg = cases.CONFIGS[name]
     for d in range(3):

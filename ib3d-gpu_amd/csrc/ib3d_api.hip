@@ -107,7 +107,7 @@ static void free_all(Ctx &c) {
     dfree(c.e32[0]); dfree(c.e32[1]); dfree(c.r32[0]); dfree(c.r32[1]);
     dfree(c.tag); dfree(c.minElem); dfree(c.elp);
     dfree(c.ibList); dfree(c.fluidList); dfree(c.solidList); dfree(c.nel); dfree(c.scell); dfree(c.scontrib);
-    dfree(c.rt_flag); dfree(c.rt_list); dfree(c.rt_count);
+    dfree(c.rt_flag); dfree(c.rt_list); dfree(c.rt_count); dfree(c.body);
     release_plans(c);
     if (c.hscal) (void)hipHostFree(c.hscal);
     c.hscal = nullptr;
@@ -645,6 +645,8 @@ int ib3d_set_mesh(int64_t n_nodes, const double *xn, const double *yn, const dou
     for (auto p : ed) dfree(*p);
     dfree(c.elp);
     dfree(c.scell); dfree(c.scontrib);
+    dfree(c.body);
+    c.nbody = 0;
     for (auto p : nd) if (dalloc(p, n_nodes)) return -1;
     for (auto p : ed) if (dalloc(p, n_elems)) return -1;
     if (dalloc(&c.elp, 3 * n_elems)) return -1;
@@ -676,6 +678,26 @@ int ib3d_selectiveRetagging(int32_t counts[3]) {
     Ctx &c = ctx();
     IB3D_REQUIRE(c.ready, "selectiveRetagging: not initialised");
     return selective_retagging(c, counts);
+}
+
+int ib3d_set_element_bodies(const int32_t *body, int64_t n_elems, int32_t n_bodies) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(c.ready, "set_element_bodies: not initialised");
+    IB3D_REQUIRE(n_elems == c.ibElems && c.ibElems > 0, "set_element_bodies: one body id per surface element");
+    IB3D_REQUIRE(n_bodies >= 1 && n_bodies <= 65535, "set_element_bodies: 1..65535 bodies");
+    for (int64_t e = 0; e < n_elems; e++)
+        IB3D_REQUIRE(body[e] >= 1 && body[e] <= n_bodies, "set_element_bodies: body ids must lie in 1..n_bodies");
+    dfree(c.body);
+    if (dalloc(&c.body, n_elems)) return -1;
+    IB3D_CHECK(h2d(c.body, body, n_elems * sizeof(int32_t)));
+    c.nbody = n_bodies;
+    return 0;
+}
+
+int ib3d_stress_bodies(double *coef) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(c.ready, "stress_bodies: not initialised");
+    return stress_bodies(c, coef);
 }
 
 int ib3d_set_wall_motion(const double vel[3], const double acc[3]) {

@@ -657,6 +657,23 @@ __global__ void k_sum_rows(const double *v, int n, double *out) {
     if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
 }
 
+// per-body sums: block (row, body) adds the elements of that body, fixed tree
+__global__ void k_sum_rows_bodies(const double *v, int n, const int *body, double *out) {
+    __shared__ double sm[1024];
+    const double *row = v + (int64_t)blockIdx.x * n;
+    const int k = (int)blockIdx.y + 1;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x)
+        if (body[q] == k) s += row[q];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = sm[0];
+}
+
 // ------------------------------------------------------------------ host
 static Coords coords(const Ctx &c) {
     Coords C;
@@ -1065,6 +1082,35 @@ int stress(Ctx &c, int first, double re, double coef[4]) {
     coef[1] = 2.0 * (s[1] / area_Sy);
     coef[2] = 2.0 * (s[2] / area_Sy);
     coef[3] = 2.0 * (s[3] / area_Sx);
+    return 0;
+}
+
+}  // namespace ib3d
+
+namespace ib3d {
+
+// Per-body coefficients from the element contributions of the last stress
+// call: the sums of subStressCal1.f90:327-331 and the normalisation of
+// 441-450 restricted to each body.
+int stress_bodies(Ctx &c, double *coef) {
+    IB3D_REQUIRE(c.body && c.nbody > 0, "stress_bodies: element bodies not set");
+    IB3D_REQUIRE(c.stress_ready, "stress_bodies: no stressCal has run");
+    const int ne = (int)c.ibElems, nb = c.nbody;
+    double *dsum = nullptr;
+    IB3D_CHECK(hipMalloc(&dsum, 7 * (size_t)nb * sizeof(double)));
+    hipLaunchKernelGGL(k_sum_rows_bodies, dim3(7, nb), dim3(1024), 0, c.stream, c.scontrib, ne, c.body, dsum);
+    IB3D_CHECK(hipGetLastError());
+    std::vector<double> h(7 * (size_t)nb);
+    IB3D_CHECK(d2h(h.data(), dsum, h.size() * sizeof(double)));
+    IB3D_CHECK(hipFree(dsum));
+    for (int k = 0; k < nb; k++) {
+        const double *sk = h.data() + 7 * (size_t)k;
+        const double area_Sx = 0.5 * sk[5], area_Sy = 0.5 * sk[6];
+        coef[4 * k] = 2.0 * (sk[0] / area_Sx);
+        coef[4 * k + 1] = 2.0 * (sk[1] / area_Sy);
+        coef[4 * k + 2] = 2.0 * (sk[2] / area_Sy);
+        coef[4 * k + 3] = 2.0 * (sk[3] / area_Sx);
+    }
     return 0;
 }
 

@@ -129,6 +129,8 @@ struct Ctx {
     // stress stencil cache [5][3][ibElems]: Elemcell, ucell, vcell, wcell, pcell
     int *scell = nullptr;
     double *scontrib = nullptr;   // per-element stress contributions (7 x ibElems)
+    int *body = nullptr;          // body id (1..nbody) of every element
+    int nbody = 0;
     bool stress_ready = false;
 
     // rigid-body motion (zero for a fixed body): wall velocity used by the
@@ -192,6 +194,7 @@ int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);
 int build_force_plans(Ctx &c);
 int shift_nodes_moving(Ctx &c, const double base[3], const double disp[3]);
 int selective_retagging(Ctx &c, int32_t counts[3]);
+int stress_bodies(Ctx &c, double *coef);
 int velocity_bc(Ctx &c);
 
 // learned initial guess (ib3d_cnn.hip)

@@ -24,7 +24,8 @@ import numpy as np
 
 __all__ = [
     "CaseConfig", "CONFIGS", "stretched_axis", "icosphere", "write_case",
-    "read_inputdata", "read_grid", "read_gmsh", "search_window",
+    "read_inputdata", "read_grid", "read_gmsh", "read_gmsh_bodies", "search_window", "ellipsoid",
+    "place_ellipsoids", "write_result", "read_result", "read_iter_dat", "read_coeff_dat",
 ]
 
 
@@ -146,9 +147,77 @@ def icosphere(level: int, radius: float = 0.5):
     return v, f
 
 
-def write_gmsh(path: str, nodes: np.ndarray, tris: np.ndarray, n_points: int = 12):
+def ellipsoid(center, semi, rot, n_lon: int = 64, n_lat: int = 33):
+    """Closed latitude-longitude triangulation of the ellipsoid
+    x = center + rot @ (semi * unit sphere point), 2*n_lon*(n_lat-1)
+    triangles (64 x 33 gives 4096), wound outward."""
+    verts = [np.array([0.0, 0.0, 1.0])]
+    for a in range(1, n_lat):
+        th = math.pi * a / n_lat
+        for b in range(n_lon):
+            ph = 2.0 * math.pi * b / n_lon
+            verts.append(np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)]))
+    verts.append(np.array([0.0, 0.0, -1.0]))
+    south = len(verts) - 1
+    ring = lambda a, b: 1 + (a - 1) * n_lon + (b % n_lon)   # noqa: E731
+    faces = []
+    for b in range(n_lon):
+        faces.append((0, ring(1, b), ring(1, b + 1)))
+    for a in range(1, n_lat - 1):
+        for b in range(n_lon):
+            p, q, r, t = ring(a, b), ring(a, b + 1), ring(a + 1, b), ring(a + 1, b + 1)
+            faces.append((p, r, t))
+            faces.append((p, t, q))
+    for b in range(n_lon):
+        faces.append((south, ring(n_lat - 1, b + 1), ring(n_lat - 1, b)))
+    unit = np.array(verts)
+    v = (unit * np.asarray(semi, dtype=np.float64)) @ np.asarray(rot, dtype=np.float64).T + np.asarray(center)
+    f = np.array(faces, dtype=np.int64)
+    # outward winding: the normal of each face points away from the centre
+    p1, p2, p3 = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    nrm = np.cross(p2 - p1, p3 - p1)
+    flip = np.einsum("ij,ij->i", nrm, (p1 + p2 + p3) / 3.0 - np.asarray(center)) < 0
+    f[flip] = f[flip][:, [0, 2, 1]]
+    return v, f
+
+
+def random_rotation(rng):
+    """Uniformly distributed rotation (unit quaternion from four normals)."""
+    q = rng.standard_normal(4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def place_ellipsoids(seed: int, count: int, lo, hi, semi_range, gap: float, max_tries: int = 100000):
+    """`count` ellipsoids inside the box [lo, hi] (the whole body, bounded by
+    its largest semi-axis), semi-axes uniform in `semi_range`, uniform random
+    orientations, pairwise clearance >= `gap` between bounding spheres (a
+    conservative stand-in for the surface gap)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+    bodies = []
+    tries = 0
+    while len(bodies) < count:
+        tries += 1
+        if tries > max_tries:
+            raise ValueError("could not place the bodies without overlap")
+        semi = rng.uniform(semi_range[0], semi_range[1], 3)
+        r = float(semi.max())
+        if np.any(hi - lo <= 2 * r):
+            continue
+        c = rng.uniform(lo + r, hi - r)
+        if all(np.linalg.norm(c - b["center"]) >= r + b["rmax"] + gap for b in bodies):
+            bodies.append({"center": c, "semi": semi, "rot": random_rotation(rng), "rmax": r})
+    return bodies
+
+
+def write_gmsh(path: str, nodes: np.ndarray, tris: np.ndarray, n_points: int = 12, body=None):
     """Gmsh 2.2 ASCII with ``n_points`` point elements (type 15) before the
-    triangles (type 2); the reference skips ``surGeoPoints`` element lines."""
+    triangles (type 2); the reference skips ``surGeoPoints`` element lines.
+    ``body`` (1-based, per triangle) goes into the physical tag."""
     with open(path, "w") as fh:
         fh.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n")
         fh.write(f"{len(nodes)}\n")
@@ -160,8 +229,9 @@ def write_gmsh(path: str, nodes: np.ndarray, tris: np.ndarray, n_points: int = 1
         for p in range(n_points):
             fh.write(f"{eid} 15 2 0 {p + 1} {p + 1}\n")
             eid += 1
-        for a, b, c in tris:
-            fh.write(f"{eid} 2 2 0 1 {a + 1} {b + 1} {c + 1}\n")
+        for q, (a, b, c) in enumerate(tris):
+            tag = 0 if body is None else int(body[q])
+            fh.write(f"{eid} 2 2 {tag} 1 {a + 1} {b + 1} {c + 1}\n")
             eid += 1
         fh.write("$EndElements\n")
 
@@ -211,6 +281,17 @@ CONFIGS = {
     "cfg2": CaseConfig("cfg2", (256, 192, 192), (-5.0, -5.0, -5.0), (15.0, 5.0, 5.0),
                        (-1.0, -1.0, -1.0), (2.0, 1.0, 1.0), 1.0 / 40.0, 5, 0.002,
                        steps=500),
+    # several ellipsoids (small parity case): 2 bodies, 256 triangles each
+    "multi": CaseConfig("multi", (48, 32, 32), (-3.0, -2.0, -2.0), (6.0, 2.0, 2.0),
+                        (-0.4, -1.0, -1.0), (2.4, 1.0, 1.0), 0.1, 0, 0.01, steps=5,
+                        extra={"bodies": 2, "seed": 7, "box": ((0.0, -0.9, -0.9), (2.3, 0.9, 0.9)),
+                               "semi": (0.3, 0.5), "gap": 0.2, "n_lon": 16, "n_lat": 9, "re_d": 100.0}),
+    # config 5: 8 ellipsoids (seed 2026) in x in [0,6], y,z in [-2,2], 4096
+    # triangles each, Re = 300 on the largest diameter, h = 0.02 in the box
+    "cfg5": CaseConfig("cfg5", (512, 320, 320), (-4.0, -5.5, -5.5), (20.0, 5.5, 5.5),
+                       (0.0, -2.0, -2.0), (6.0, 2.0, 2.0), 0.02, 0, 0.001, steps=500,
+                       extra={"bodies": 8, "seed": 2026, "box": ((0.0, -2.0, -2.0), (6.0, 2.0, 2.0)),
+                              "semi": (0.3, 0.8), "gap": 0.3, "n_lon": 64, "n_lat": 33, "re_d": 300.0}),
     # config 3: the sphere oscillating transversely, y_c = 0.25 sin(2 pi 0.2 t),
     # 384x256x256 with the uniform h = D/40 band covering the motion envelope
     "cfg3": CaseConfig("cfg3", (384, 256, 256), (-5.0, -5.0, -5.0), (15.0, 5.0, 5.0),
@@ -242,13 +323,36 @@ def write_case(cfg: CaseConfig, directory: str, istart: int = 0, a0: float = Non
         with open(os.path.join(directory, "grid" + "xyz"[d] + ".dat"), "w") as fh:
             for val in nodes:
                 fh.write(f"{float(val)!r}\n")
-    v, f = icosphere(cfg.level, cfg.radius)
-    write_gmsh(os.path.join(directory, "geometries", "sphere.msh"), v, f, cfg.n_points)
-    win = [search_window(axes[d], cfg.shift[d], cfg.radius, cfg.h, cfg.margin) for d in range(3)]
+    re = cfg.re
+    if cfg.extra.get("bodies"):
+        ex = cfg.extra
+        bodies = place_ellipsoids(ex["seed"], ex["bodies"], ex["box"][0], ex["box"][1], ex["semi"], ex["gap"])
+        vs, fs, bid, off = [], [], [], 0
+        for k, b in enumerate(bodies):
+            v, f = ellipsoid(b["center"], b["semi"], b["rot"], ex["n_lon"], ex["n_lat"])
+            vs.append(v)
+            fs.append(f + off)
+            bid.append(np.full(len(f), k + 1, dtype=np.int32))
+            off += len(v)
+        v, f = np.concatenate(vs), np.concatenate(fs)
+        write_gmsh(os.path.join(directory, "geometries", "sphere.msh"), v, f, cfg.n_points, np.concatenate(bid))
+        # Reynolds number on the largest body diameter: Re_D = U D / nu with U = 1
+        re = ex["re_d"] / (2.0 * max(float(b["semi"].max()) for b in bodies))
+        lo_b = np.min(v, axis=0)
+        hi_b = np.max(v, axis=0)
+        win = []
+        for d in range(3):
+            xc = 0.5 * (axes[d][:-1] + axes[d][1:])
+            sel = np.nonzero((xc >= lo_b[d] - cfg.margin * cfg.h) & (xc <= hi_b[d] + cfg.margin * cfg.h))[0]
+            win.append((int(sel[0]) + 2, int(sel[-1]) + 2))
+    else:
+        v, f = icosphere(cfg.level, cfg.radius)
+        write_gmsh(os.path.join(directory, "geometries", "sphere.msh"), v, f, cfg.n_points)
+        win = [search_window(axes[d], cfg.shift[d], cfg.radius, cfg.h, cfg.margin) for d in range(3)]
     nx, ny, nz = cfg.n
     with open(os.path.join(directory, "inputdata"), "w") as fh:
         fh.write(f"      {nx}, {ny}, {nz},\n")
-        fh.write(f"      {cfg.re!r},\n")
+        fh.write(f"      {re!r},\n")
         fh.write(f"      {cfg.steps}, {cfg.eps1!r}, 35000,\n")
         fh.write("      0., 0., 0.,\n")
         amp = cfg.extra.get("a0", 0.0) if a0 is None else a0
@@ -346,6 +450,19 @@ def read_gmsh(path: str, sur_geo_points: int):
         p = next(it).split()
         conn[n] = [int(p[5]), int(p[6]), int(p[7])]
     return nodes, conn
+
+
+def read_gmsh_bodies(path: str, sur_geo_points: int) -> np.ndarray:
+    """Physical tag of every triangle after the skipped elements (the body id
+    of multi-body meshes; the reference reads the connectivity only)."""
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    q = 4
+    nn = int(lines[q].split()[0])
+    q += 1 + nn + 2
+    ne = int(lines[q].split()[0])
+    q += 1 + sur_geo_points
+    return np.array([int(lines[q + n].split()[3]) for n in range(ne - sur_geo_points)], dtype=np.int32)
 
 
 # --------------------------------------------------------------------------

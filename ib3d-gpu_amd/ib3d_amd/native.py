@@ -71,6 +71,8 @@ def lib():
         "ib3d_shiftSurfaceNodesMovingbody": [D, D, D, D, D, D],
         "ib3d_selectiveRetagging": [PI32],
         "ib3d_set_wall_motion": [PD, PD],
+        "ib3d_set_element_bodies": [PI32, I64, C.c_int32],
+        "ib3d_stress_bodies": [PD],
         "ib3d_cnn_init": [C.c_uint64],
         "ib3d_cnn_release": [],
         "ib3d_cnn_set_params": [PF, I64],
@@ -187,6 +189,18 @@ class Solver:
         _ok(s.L.ib3d_coefficientMatrix())
         _ok(s.L.ib3d_non_uni_coeff())
         return s
+
+    def set_bodies(self, body):
+        """Body id (1..n) of every surface element, in mesh order."""
+        b = np.ascontiguousarray(body, dtype=np.int32)
+        self.nbody = int(b.max())
+        _ok(self.L.ib3d_set_element_bodies(b.ctypes.data_as(C.POINTER(C.c_int32)), b.size, self.nbody))
+
+    def stress_bodies(self):
+        """Per-body {Cd_visc, Cd_press, Cl_visc, Cl_press} of the last step."""
+        out = np.zeros(4 * self.nbody)
+        _ok(self.L.ib3d_stress_bodies(_dp(out)))
+        return out.reshape(self.nbody, 4)
 
     def move_body(self, disp, vel=(0.0, 0.0, 0.0), acc=(0.0, 0.0, 0.0)):
         """Prescribed rigid motion: nodes at (initial placement + disp), then

@@ -78,6 +78,15 @@ int ib3d_selectiveRetagging(int32_t counts[3]);
  * 0, subForcing.f90:11 and 20).  Zero keeps the reference's fixed body bit for
  * bit. */
 int ib3d_set_wall_motion(const double vel[3], const double acc[3]);
+/* Several bodies in one surface mesh: body id (1..n_bodies) of every
+ * element, in mesh order.  No counterpart in the reference, whose stressCal
+ * sums over all elements. */
+int ib3d_set_element_bodies(const int32_t *body, int64_t n_elems, int32_t n_bodies);
+/* Per-body coefficients of the last stressCal1/stressCal2 call: for body k,
+ * coef[4k..4k+3] = {viscous Cd, pressure Cd, viscous Cl, pressure Cl} with the
+ * sums of subStressCal1.f90:327-331 and the normalisation of 441-450
+ * restricted to the body's elements and projected areas. */
+int ib3d_stress_bodies(double *coef);
 /* subSearch.f90:80-131; inor is the normal sign read from inputdata */
 int ib3d_computeSurfaceNorm(int64_t inor);
 /* subSearch.f90:134-321; window = {i_start,i_end,j_start,j_end,k_start,k_end}

@@ -58,6 +58,12 @@ struct orc_state {
      * for dp/dn at the wall */
     double wall_vel[3], wall_acc[3];
     int64_t *cell_old;
+    /* per-element surface-force contributions of the last stress call
+     * (7 x ibElems: viscous drag, pressure drag, viscous lift, pressure lift,
+     * area, area_xz, area_yz) and the body id of every element */
+    double *contrib;
+    int32_t *body;
+    int32_t nbody;
     char err[256];
 };
 
@@ -121,7 +127,11 @@ static void free_mesh(orc_state *s) {
 }
 
 void orc_free(orc_state *s) {
-    if (s) free(s->cell_old);
+    if (s) {
+        free(s->cell_old);
+        free(s->contrib);
+        free(s->body);
+    }
     if (!s) return;
     free_lists(s);
     free_mesh(s);
@@ -1188,6 +1198,16 @@ int orc_stress(orc_state *s, int first, double out[4]) {
         double f_surf = cval * area;
         double f_surf_x = -f_surf * na;
         double f_surf_y = -f_surf * nb;
+        if (s->contrib) {
+            double *o = s->contrib;
+            o[(e - 1)] = shear_x;
+            o[ne + (e - 1)] = f_surf_x;
+            o[2 * ne + (e - 1)] = shear_y;
+            o[3 * ne + (e - 1)] = f_surf_y;
+            o[4 * ne + (e - 1)] = area;
+            o[5 * ne + (e - 1)] = area_xz;
+            o[6 * ne + (e - 1)] = area_yz;
+        }
         viscousDrag = viscousDrag + shear_x;
         pressureDrag = pressureDrag + f_surf_x;
         viscousLift = viscousLift + shear_y;
@@ -1203,6 +1223,45 @@ int orc_stress(orc_state *s, int first, double out[4]) {
     out[1] = 2.0 * (pressureDrag / area_Sy);
     out[2] = 2.0 * (viscousLift / area_Sy);
     out[3] = 2.0 * (pressureLift / area_Sx);
+    return 0;
+}
+
+/* Per-body coefficients: the sums of subStressCal1.f90:327-331 and the
+ * normalisation of 441-450 restricted to the elements of each body (body ids
+ * 1..nbody, one per element in mesh order); no counterpart in the reference,
+ * which sums over all elements. */
+int orc_set_bodies(orc_state *s, const int32_t *body, int32_t nbody) {
+    free(s->body);
+    free(s->contrib);
+    s->body = zalloc(s->ibElems, sizeof(int32_t));
+    memcpy(s->body, body, (size_t)s->ibElems * sizeof(int32_t));
+    s->nbody = nbody;
+    s->contrib = zalloc((size_t)7 * s->ibElems, 8);
+    return 0;
+}
+
+int orc_stress_bodies(orc_state *s, double *out) {
+    if (!s->contrib) return fail(s, "orc_set_bodies has not run");
+    const int64_t ne = s->ibElems;
+    for (int32_t k = 1; k <= s->nbody; k++) {
+        double vd = 0.0, pd = 0.0, vl = 0.0, pl = 0.0, ax = 0.0, ay = 0.0;
+        for (int64_t e = 0; e < ne; e++) {
+            if (s->body[e] != k) continue;
+            vd = vd + s->contrib[e];
+            pd = pd + s->contrib[ne + e];
+            vl = vl + s->contrib[2 * ne + e];
+            pl = pl + s->contrib[3 * ne + e];
+            ax = ax + s->contrib[5 * ne + e];
+            ay = ay + s->contrib[6 * ne + e];
+        }
+        ax = 0.5 * ax;
+        ay = 0.5 * ay;
+        double *o = out + 4 * (k - 1);
+        o[0] = 2.0 * (vd / ax);
+        o[1] = 2.0 * (pd / ay);
+        o[2] = 2.0 * (vl / ay);
+        o[3] = 2.0 * (pl / ax);
+    }
     return 0;
 }
 

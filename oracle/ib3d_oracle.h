@@ -40,6 +40,9 @@ int orc_cell_count(orc_state *s);
 int orc_shift_nodes_moving(orc_state *s, double xs, double ys, double zs, double dx, double dy, double dz);
 int orc_selective_retagging(orc_state *s);
 int orc_set_wall_motion(orc_state *s, const double vel[3], const double acc[3]);
+/* per-body force coefficients of the last stress call (4 per body) */
+int orc_set_bodies(orc_state *s, const int32_t *body, int32_t nbody);
+int orc_stress_bodies(orc_state *s, double *out);
 int orc_compute_norm_distance(orc_state *s);
 int orc_initial_conditions(orc_state *s);
 int orc_coefficient_matrix(orc_state *s);

@@ -59,6 +59,8 @@ def lib():
         L.orc_shift_nodes_moving.argtypes = [P, D, D, D, D, D, D]
         L.orc_selective_retagging.argtypes = [P]
         L.orc_set_wall_motion.argtypes = [P, PD, PD]
+        L.orc_set_bodies.argtypes = [P, C.POINTER(C.c_int32), C.c_int32]
+        L.orc_stress_bodies.argtypes = [P, PD]
         _lib = L
     return _lib
 
@@ -125,6 +127,16 @@ class Oracle:
         return o
 
     # ---------------------------------------------------------------- access
+    def set_bodies(self, body):
+        self.body = np.ascontiguousarray(body, dtype=np.int32)
+        self.nbody = int(self.body.max())
+        self._ok(self.L.orc_set_bodies(self.h, self.body.ctypes.data_as(C.POINTER(C.c_int32)), self.nbody))
+
+    def stress_bodies(self):
+        out = np.zeros(4 * self.nbody)
+        self._ok(self.L.orc_stress_bodies(self.h, _dp(out)))
+        return out.reshape(self.nbody, 4)
+
     def move_body(self, disp, vel=(0.0, 0.0, 0.0), acc=(0.0, 0.0, 0.0)):
         """Prescribed rigid motion: nodes at (initial placement + disp), then
         computeSurfaceNorm, selectiveRetagging, cellCount and

@@ -16,6 +16,7 @@ Usage:
   python tests/golden/make_golden.py tiny            # ~1 s
   python tests/golden/make_golden.py cfg1 [RUN_DIR]  # 200 steps, ~25 min
       (RUN_DIR: an existing harness run of the same case to reuse)
+  python tests/golden/make_golden.py multi           # two ellipsoids, 3 steps, ~2 s
   python tests/golden/make_golden.py move            # moving-body geometry, ~2 s
       (tiny case, body shifted by s*0.03 in y at step s = 1..8; the
        reference's computeSurfaceNorm, selectiveRetagging, cellCount and
@@ -121,12 +122,12 @@ def main():
     if which == "move":
         return make_move()
     cfg = cases.CONFIGS[which]
-    if which == "tiny":
+    if which in ("tiny", "multi"):
         nsteps, dump_steps = 3, [1, 2, 3]
     elif which == "cfg1":
         nsteps, dump_steps = 200, [1, 200]
     else:
-        raise SystemExit("tiny or cfg1")
+        raise SystemExit("tiny, multi, cfg1 or move")
     reuse = sys.argv[2] if len(sys.argv) > 2 else None
     tmp = tempfile.mkdtemp(prefix="golden_")
     try:

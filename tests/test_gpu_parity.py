@@ -55,6 +55,7 @@ def pair(case_dir):
         _say(f"[gpu] setup case={name} dir={d}")
         o = pyoracle.Oracle.from_case(d)
         s = Solver.from_case(d)
+        s.case_dir = d
         made.append(s)
         return s, o
     yield make
@@ -62,7 +63,7 @@ def pair(case_dir):
         s.close()
 
 
-@pytest.mark.parametrize("name", ["tiny", "small"])
+@pytest.mark.parametrize("name", ["tiny", "small", "multi"])
 def test_setup_bitwise(pair, name):
     s, o = pair(name)
     assert_bits("minElemcell", s.download("minElemcell"), o.minElemcell)
@@ -76,7 +77,7 @@ def test_setup_bitwise(pair, name):
             assert_bits(nm, s.download(nm)[1:n], t[q, 2:n + 1])
 
 
-@pytest.mark.parametrize("name,nsteps", [("tiny", 3), ("small", 2)])
+@pytest.mark.parametrize("name,nsteps", [("tiny", 3), ("small", 2), ("multi", 2)])
 def test_steps_stagewise_bitwise(pair, name, nsteps):
     s, o = pair(name)
     for st in range(1, nsteps + 1):
@@ -123,3 +124,19 @@ def test_whole_step_entry_matches_stagewise(pair):
         np.testing.assert_allclose(r["coef"], ro["coef"], rtol=1e-12, atol=1e-14)
     for f in ["u", "v", "w", "p"]:
         assert_bits(f, s.download(f), o.array(f))
+
+
+def test_per_body_forces_match_oracle(pair):
+    """Two ellipsoids: the per-body coefficients of the GPU (tree sums)
+    agree with the oracle's serial sums to 1e-12 relative."""
+    import os
+    from ib3d_amd import cases
+    s, o = pair("multi")
+    d = s.case_dir
+    body = cases.read_gmsh_bodies(os.path.join(d, "geometries", "sphere.msh"), s.inp["surGeoPoints"])
+    s.set_bodies(body)
+    o.set_bodies(body)
+    for k in range(2):
+        s.step()
+        o.step(k == 0)
+        np.testing.assert_allclose(s.stress_bodies(), o.stress_bodies(), rtol=1e-12, atol=1e-14)

@@ -76,6 +76,11 @@ class Motion:
 def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None, moving=None):
     s = solver_cls.from_case(case)
     motion = Motion(moving[0], moving[1], s.deltat) if moving else None
+    from ib3d_amd import cases as _cases
+    body = _cases.read_gmsh_bodies(os.path.join(case, "geometries", "sphere.msh"), s.inp["surGeoPoints"])
+    multi = body.max() > 1
+    if multi:
+        s.set_bodies(body)
     if net is not None:
         net.config(record=False, guess=mode == "mixed+learned")
     if mode == "fp64":
@@ -129,6 +134,8 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None,
     }
     if motion:
         out["geometry_ms_per_step"] = 1e3 * geo_s / steps
+    if multi:
+        out["per_body_coef"] = [[round(float(x), 6) for x in row] for row in s.stress_bodies()]
     if mode == "mixed+learned":
         st = net.stats()
         out["learned_guess"] = {"solves": st["solves"], "kept": st["kept"], "guess_ms": round(st["guess_ms"], 3)}
@@ -251,7 +258,8 @@ def main():
             shutil.rmtree(case, ignore_errors=True)
         return
     try:
-        cases.write_case(cfg, case)
+        meta = cases.write_case(cfg, case)
+        n_tris = meta["n_tris"]
         nx, ny, nz = cfg.n
         n_cells = nx * ny * nz
         modes = [a.mode] + [x for x in a.compare.split(",") if x and x != "none" and x != a.mode]
@@ -306,9 +314,11 @@ def main():
             "dtype": "fp32+fp64" if a.mode.startswith("mixed") else "fp64",
             "data": "synthetic (generated grid and icosphere, reference initial condition)",
             "config": {"workload": (f"sphere oscillating as y_c = {moving[0]} sin(2 pi {moving[1]} t) "
-                                    f"(geometry rebuilt every step)" if moving else "fixed sphere")
-                                   + f" Re=100, {nx}x{ny}x{nz} stretched grid, "
-                                   f"{cfg.level and 20 * 4 ** cfg.level} triangles, dt={cfg.dt}",
+                                    f"(geometry rebuilt every step)" if moving else
+                                    f"{cfg.extra['bodies']} ellipsoids (seed {cfg.extra['seed']})"
+                                    if cfg.extra.get("bodies") else "fixed sphere")
+                                   + f" Re={cfg.extra.get('re_d', 100.0)}, {nx}x{ny}x{nz} stretched grid, "
+                                   f"{n_tris} triangles, dt={cfg.dt}",
                        "poisson": f"{a.mode}: RB-SOR omega={a.omega}, ||r||/||b|| <= {a.tol}, "
                                   f"{a.inner} sweeps between residual checks, "
                                   + ("CNN guess with previous-step fallback" if a.mode == "mixed+learned"
@@ -324,6 +334,9 @@ def main():
             "gpu_wall_s": round(gpu_wall, 1),
             "field_error": errors,
         }
+        for k in ("per_body_coef", "geometry_ms_per_step"):
+            if k in main_res:
+                line[k] = main_res[k]
         if net_info:
             net_info.update(main_res.get("learned_guess", {}))
             line["learned_guess"] = net_info

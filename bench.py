@@ -8,17 +8,25 @@ full time step of the hot path (momentum predictor, boundary conditions,
 IB velocity forcing, Poisson solve + projection, IB pressure forcing,
 surface forces), all on the GPU with inputs resident in HBM.
 
-Poisson modes (same stopping rule: ||b/dt - A p||_2 <= 1e-6 ||b/dt||_2,
-omega = 1.99 as in the reference's REDBLACKSOR; omega = 1.7 needs more than
-20000 sweeps per solve on this grid):
-  fp64-cold -- plain FP64 red-black SOR from a zero guess (the reference's
-               method, with the L2 stopping rule)
-  fp64      -- FP64 red-black SOR from the previous step's correction
-  mixed     -- FP32 red-black SOR sweeps inside an FP64 defect-correction
-               loop, previous-step guess
-The headline `value` is steps/s of --mode (default mixed); the --compare
-modes are measured in the same run and reported beside it, with the Poisson
-speedup of --mode over each.
+Poisson modes (same stopping rule: ||b/dt - A p||_2 <= 1e-6 ||b/dt||_2):
+  fp64-cold     -- plain FP64 red-black SOR (omega 1.99, the reference's)
+                   from a zero guess: the reference's method with the L2
+                   stopping rule (omega 1.7 needs more than 20000 sweeps per
+                   solve on this grid)
+  fp64          -- the same from the previous step's correction
+  mixed-dc      -- FP32 red-black SOR sweeps (omega 1.99) on the defect
+                   equation inside an FP64 defect-correction loop,
+                   previous-step guess
+  mixed         -- FP64 refinement whose defect equation is solved by FP32
+                   flexible GMRES(12), every Arnoldi step preconditioned by
+                   20 FP32 red-black SOR sweeps at omega 1.9 (GMRES-IR),
+                   previous-step guess
+  mixed+learned -- mixed, starting from the CNN's guess when it has the
+                   smaller residual (headline)
+The headline `value` is steps/s of --mode; the --compare modes are measured
+in the same run and reported beside it, with the Poisson speedup of --mode
+over each.  The network is trained first on 1000 recorded solves of the
+128x64x64 case (not timed).
 
 Roofline: the dominant kernel is the SOR half-sweep.  Its average launch
 duration is measured with HIP events on the library's stream
@@ -54,6 +62,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# FP32 GMRES inside the FP64 refinement: Arnoldi steps per refinement step,
+# FP32 SOR sweeps per preconditioner application and their omega
+GMRES = {"m": 12, "k": 20, "omega": 1.9}
+
+
 class Motion:
     """Prescribed transverse oscillation y_c = a0 sin(2 pi f t): per step the
     body is moved, its geometry rebuilt (selective retagging) and the wall
@@ -87,8 +100,11 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None,
         s.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=True)
     elif mode == "fp64-cold":
         s.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=False)
-    elif mode in ("mixed", "mixed+learned"):
+    elif mode == "mixed-dc":
         s.set_poisson_mode(2, omega=omega, tol=tol, check=inner, warm=True)
+    elif mode in ("mixed", "mixed+learned"):
+        s.set_poisson_mode(3, omega=GMRES["omega"], tol=tol, check=GMRES["k"], warm=True)
+        s.set_krylov(GMRES["m"])
     elif mode == "reference":
         s.set_poisson_mode(0)
     else:
@@ -216,9 +232,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--mode", default="mixed+learned", choices=["mixed+learned", "mixed", "fp64", "fp64-cold",
-                                                                 "reference"])
-    ap.add_argument("--compare", default="fp64-cold,fp64,mixed",
+    ap.add_argument("--mode", default="mixed+learned",
+                    choices=["mixed+learned", "mixed", "mixed-dc", "fp64", "fp64-cold", "reference"])
+    ap.add_argument("--compare", default="fp64-cold,fp64,mixed-dc,mixed",
                     help="comma-separated modes measured in the same run for comparison ('none' to skip)")
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--omega", type=float, default=1.99)
@@ -229,9 +245,13 @@ def main():
     ap.add_argument("--train-config", default="cfg1", help="case whose solves are recorded for training")
     ap.add_argument("--rec-steps", type=int, default=1000, help="recorded solves (two 64^3 tiles each)")
     ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--gmres-m", type=int, default=GMRES["m"])
+    ap.add_argument("--gmres-k", type=int, default=GMRES["k"])
+    ap.add_argument("--gmres-omega", type=float, default=GMRES["omega"])
     ap.add_argument("--profile-sor", nargs=2, type=int, metavar=("PRECISION", "SWEEPS"),
                     help="profiling aid: set up the case, run SWEEPS RB-SOR sweeps and one copy probe, exit")
     a = ap.parse_args()
+    GMRES.update(m=a.gmres_m, k=a.gmres_k, omega=a.gmres_omega)
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
               f"(the hot path's multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
@@ -319,10 +339,14 @@ def main():
                                     if cfg.extra.get("bodies") else "fixed sphere")
                                    + f" Re={cfg.extra.get('re_d', 100.0)}, {nx}x{ny}x{nz} stretched grid, "
                                    f"{n_tris} triangles, dt={cfg.dt}",
-                       "poisson": f"{a.mode}: RB-SOR omega={a.omega}, ||r||/||b|| <= {a.tol}, "
-                                  f"{a.inner} sweeps between residual checks, "
+                       "poisson": (f"{a.mode}: FP64 refinement, FP32 flexible GMRES({GMRES['m']}) preconditioned by "
+                                   f"{GMRES['k']} FP32 RB-SOR sweeps at omega={GMRES['omega']}"
+                                   if a.mode in ("mixed", "mixed+learned") else
+                                   f"{a.mode}: RB-SOR omega={a.omega}, {a.inner} sweeps between residual checks")
+                                  + f", ||r||/||b|| <= {a.tol}, "
                                   + ("CNN guess with previous-step fallback" if a.mode == "mixed+learned"
-                                     else "previous-step guess" if a.mode in ("mixed", "fp64") else "zero guess"),
+                                     else "previous-step guess" if a.mode in ("mixed", "mixed-dc", "fp64")
+                                     else "zero guess"),
                        "parallelism": "1 GPU"},
             "poisson_ms_per_step": round(main_res["poisson_ms_per_step"], 3),
             "sor_ms_per_step": round(main_res["sor_ms_per_step"], 3),

@@ -104,7 +104,9 @@ static void free_all(Ctx &c) {
                      &c.cosG, &c.alpha3, &c.beta3, &c.gamma3, &c.normdis};
     for (auto pp : d1) dfree(*pp);
     for (int t = 0; t < 6; t++) dfree(c.tab[t]);
-    dfree(c.e32[0]); dfree(c.e32[1]); dfree(c.r32[0]); dfree(c.r32[1]);
+    dfree(c.e32[0]); dfree(c.r32[0]);   // colour 1 lives in the same blocks
+    c.e32[1] = nullptr;
+    c.r32[1] = nullptr;
     dfree(c.tag); dfree(c.minElem); dfree(c.elp);
     dfree(c.ibList); dfree(c.fluidList); dfree(c.solidList); dfree(c.nel); dfree(c.scell); dfree(c.scontrib);
     dfree(c.rt_flag); dfree(c.rt_list); dfree(c.rt_count); dfree(c.body);
@@ -444,12 +446,15 @@ int ib3d_init(int64_t nx, int64_t ny, int64_t nz) {
     double **f[] = {&c.u, &c.v, &c.w, &c.ut, &c.vt, &c.wt, &c.p, &c.resi_u, &c.resi_v, &c.resi_w};
     for (auto p : f) if (dalloc(p, c.P)) return -1;
     if (dalloc(&c.tag, c.P) || dalloc(&c.minElem, c.P)) return -1;
+    // FP32 colour pairs are one block each (colour 1 follows colour 0), so the
+    // Krylov vector kernels can treat them as one vector of 2*NC values
+    if (dalloc(&c.e32[0], 2 * c.NC) || dalloc(&c.r32[0], 2 * c.NC)) return -1;
+    c.e32[1] = c.e32[0] + c.NC;
+    c.r32[1] = c.r32[0] + c.NC;
+    hipLaunchKernelGGL(k_fill_nan32, dim3(nblocks(2 * c.NC, 256)), dim3(256), 0, c.stream, c.r32[0], 2 * c.NC);
     for (int col = 0; col < 2; col++) {
-        if (dalloc(&c.pc[col], c.NC) || dalloc(&c.bdt[col], c.NC) || dalloc(&c.e32[col], c.NC) ||
-            dalloc(&c.r32[col], c.NC))
-            return -1;
+        if (dalloc(&c.pc[col], c.NC) || dalloc(&c.bdt[col], c.NC)) return -1;
         hipLaunchKernelGGL(k_fill_nan64, dim3(nblocks(c.NC, 256)), dim3(256), 0, c.stream, c.bdt[col], c.NC);
-        hipLaunchKernelGGL(k_fill_nan32, dim3(nblocks(c.NC, 256)), dim3(256), 0, c.stream, c.r32[col], c.NC);
     }
     if (dalloc(&c.scalars, 64)) return -1;
     IB3D_CHECK(hipHostMalloc((void **)&c.hscal, 64 * sizeof(double), hipHostMallocDefault));
@@ -862,7 +867,7 @@ int ib3d_stressCal2(double re, double coef[4]) {
 
 int ib3d_set_poisson_mode(int mode, double omega, double tol, int check, int warm, int64_t max_sweeps) {
     Ctx &c = ctx();
-    IB3D_REQUIRE(mode >= 0 && mode <= 2, "set_poisson_mode: mode must be 0, 1 or 2");
+    IB3D_REQUIRE(mode >= 0 && mode <= 3, "set_poisson_mode: mode must be 0, 1, 2 or 3");
     IB3D_REQUIRE(omega > 0.0 && omega < 2.0, "set_poisson_mode: omega must lie in (0, 2)");
     IB3D_REQUIRE(check >= 1, "set_poisson_mode: check must be >= 1");
     IB3D_REQUIRE(tol > 0.0, "set_poisson_mode: tol must be positive");
@@ -872,6 +877,13 @@ int ib3d_set_poisson_mode(int mode, double omega, double tol, int check, int war
     c.check = check;
     c.warm = warm;
     c.max_sweeps = max_sweeps > 0 ? max_sweeps : 2000000;
+    return 0;
+}
+
+int ib3d_set_krylov(int restart) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(restart >= 1 && restart <= 23, "set_krylov: restart length must be 1..23");
+    c.krylov_m = restart;
     return 0;
 }
 

@@ -143,7 +143,8 @@ struct Ctx {
     int64_t rt_cap = 0;
 
     // Poisson configuration
-    int mode = 0;                  // 0 reference rule, 1 FP64 L2, 2 mixed FP32/FP64
+    int mode = 0;                  // 0 reference rule, 1 FP64 L2, 2 mixed FP32/FP64, 3 FP32 GMRES in FP64 refinement
+    int krylov_m = 20;             // mode 3: Arnoldi steps per refinement step
     double omega = 1.99;
     double tol = 1e-6;
     int check = 10;                // sweeps between L2 checks (mode 1) / inner sweeps (mode 2)

@@ -520,6 +520,245 @@ __global__ void __launch_bounds__(kSorBlock) k_copy_probe(const double4 *__restr
     if (q < n) b[q] = a[q];
 }
 
+// ------------------------------------------------------------ GMRES-IR
+// FP32 flexible GMRES(m) inside the FP64 refinement loop: every outer
+// iteration solves A d = r (r the FP64 residual, rounded to FP32) with m
+// Arnoldi steps whose preconditioner is k FP32 red-black SOR sweeps from
+// zero, then adds d to the FP64 pc.  Vectors are two colour arrays back to
+// back (2*NC floats, NaN on cells the solve does not update); H, beta and y
+// live on the device, so a whole cycle runs without the host.
+
+// y = A z (FP32) on cell pairs marching in z, both colours per thread;
+// NaN where the mask array is NaN
+__global__ void __launch_bounds__(kSorBlock)
+k_matvec32_pz(const float *__restrict__ z0, const float *__restrict__ z1, const float *__restrict__ m0,
+              const float *__restrict__ m1, float *__restrict__ y0, float *__restrict__ y1,
+              const float *__restrict__ axc, const float *__restrict__ ayc, const float *__restrict__ azc, SorGeom g,
+              const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    const int p = 2 * (int)(blockIdx.x * kSorBlock + threadIdx.x);
+    const int jj = (int)fdiv((uint32_t)p, g.dH);
+    const int h = p - jj * g.H;
+    if (!(p < g.plane && jj >= 1 && jj <= g.sy - 2)) return;
+    const int k0 = 1 + (int)blockIdx.y * kZch;
+    const int k1 = min(k0 + kZch, g.sz - 1);
+    const f4v c1 = *reinterpret_cast<const f4v *>(axc + 2 * h);
+    const f4v c2 = *reinterpret_cast<const f4v *>(axc + g.sxp + 2 * h);
+    const f4v c3 = *reinterpret_cast<const f4v *>(axc + 2 * g.sxp + 2 * h);
+    const float a1y = ayc[jj], a2y = ayc[g.sy + jj], a3y = ayc[2 * g.sy + jj];
+    const float *zc_[2] = {z0, z1};
+    const float *mc_[2] = {m0, m1};
+    float *yc_[2] = {y0, y1};
+    int f = p + k0 * g.plane;
+    f2v below[2] = {*reinterpret_cast<const f2v *>(z0 + f - g.plane), *reinterpret_cast<const f2v *>(z1 + f - g.plane)};
+    f2v cur[2] = {*reinterpret_cast<const f2v *>(z0 + f), *reinterpret_cast<const f2v *>(z1 + f)};
+    for (int kk = k0; kk < k1; kk++, f += g.plane) {
+        f2v above[2] = {*reinterpret_cast<const f2v *>(z0 + f + g.plane), *reinterpret_cast<const f2v *>(z1 + f + g.plane)};
+        const float a1z = azc[kk], a2z = azc[g.sz + kk], a3z = azc[2 * g.sz + kk];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int o = 1 - c;
+            const f2v mk = *reinterpret_cast<const f2v *>(mc_[c] + f);
+            const bool ok0 = mk.x == mk.x, ok1 = mk.y == mk.y;
+            f2v out = {__builtin_nanf(""), __builtin_nanf("")};
+            if (ok0 || ok1) {
+                const int q = (c + jj + kk) & 1;
+                const float side = q ? zc_[o][f + 2] : zc_[o][f - 1];
+                const f2v ps = *reinterpret_cast<const f2v *>(zc_[o] + f - g.H);
+                const f2v pn = *reinterpret_cast<const f2v *>(zc_[o] + f + g.H);
+                const f2v co = cur[o], own = cur[c];
+                const float a1x0 = q ? c1.y : c1.x, a2x0 = q ? c2.y : c2.x, a3x0 = q ? c3.y : c3.x;
+                const float a1x1 = q ? c1.w : c1.z, a2x1 = q ? c2.w : c2.z, a3x1 = q ? c3.w : c3.z;
+                if (ok0) {
+                    const float pw = q ? co.x : side, pe = q ? co.y : co.x;
+                    out.x = (((((a1x0 * pw + a3x0 * pe) + a1y * ps.x) + a3y * pn.x) + a1z * below[o].x) +
+                             a3z * above[o].x) + ((a2x0 + a2y) + a2z) * own.x;
+                }
+                if (ok1) {
+                    const float pw = q ? co.y : co.x, pe = q ? side : co.y;
+                    out.y = (((((a1x1 * pw + a3x1 * pe) + a1y * ps.y) + a3y * pn.y) + a1z * below[o].y) +
+                             a3z * above[o].y) + ((a2x1 + a2y) + a2z) * own.y;
+                }
+            }
+            *reinterpret_cast<f2v *>(yc_[c] + f) = out;
+        }
+        below[0] = cur[0];
+        below[1] = cur[1];
+        cur[0] = above[0];
+        cur[1] = above[1];
+    }
+}
+
+static constexpr int kMaxKrylov = 24;
+static constexpr int kVecBlocks = 2048;
+
+// dot products of w with v_0..v_{nv-1} (vectors of n2 float2 pairs, NaN
+// entries skipped), per-block partials [i][block]
+__global__ void __launch_bounds__(kSorBlock)
+k_dots(const float *__restrict__ w, const float *__restrict__ V, int64_t stride, int nv, int64_t n2,
+       double *__restrict__ partial, const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    __shared__ double sm[kMaxKrylov][kSorBlock / 64];
+    double s[kMaxKrylov];
+#pragma unroll
+    for (int i = 0; i < kMaxKrylov; i++) s[i] = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < n2; q += (int64_t)gridDim.x * kSorBlock) {
+        const f2v a = reinterpret_cast<const f2v *>(w)[q];
+        const bool ok0 = a.x == a.x, ok1 = a.y == a.y;
+#pragma unroll
+        for (int i = 0; i < kMaxKrylov; i++) {
+            if (i < nv) {
+                const f2v b = reinterpret_cast<const f2v *>(V + i * stride)[q];
+                if (ok0) s[i] += (double)a.x * (double)b.x;
+                if (ok1) s[i] += (double)a.y * (double)b.y;
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < kMaxKrylov; i++) {
+        if (i < nv) {
+            double v = wave_sum(s[i]);
+            if (lane == 0) sm[i][wid] = v;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nv)
+        partial[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] =
+            ((sm[threadIdx.x][0] + sm[threadIdx.x][1]) + sm[threadIdx.x][2]) + sm[threadIdx.x][3];
+}
+
+// block i: out[i] (+)= sum of partial[i][0..nblk)
+__global__ void k_rows_sum(const double *__restrict__ partial, int nblk, double *out, int accumulate,
+                           const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    __shared__ double sm[256];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nblk; q += 256) s += partial[(int64_t)blockIdx.x * nblk + q];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = accumulate ? out[blockIdx.x] + sm[0] : sm[0];
+}
+
+// w -= sum_i h[i] v_i, then per-block partial sums of w^2 (NaN skipped)
+__global__ void __launch_bounds__(kSorBlock)
+k_orth(float *__restrict__ w, const float *__restrict__ V, int64_t stride, int nv, const double *__restrict__ h,
+       int64_t n2, double *__restrict__ partial, const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    double s = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < n2; q += (int64_t)gridDim.x * kSorBlock) {
+        f2v a = reinterpret_cast<f2v *>(w)[q];
+        double ax = a.x, ay = a.y;
+        for (int i = 0; i < nv; i++) {
+            const f2v b = reinterpret_cast<const f2v *>(V + i * stride)[q];
+            ax -= h[i] * (double)b.x;
+            ay -= h[i] * (double)b.y;
+        }
+        a.x = (float)ax;
+        a.y = (float)ay;
+        reinterpret_cast<f2v *>(w)[q] = a;
+        if (a.x == a.x) s += (double)a.x * (double)a.x;
+        if (a.y == a.y) s += (double)a.y * (double)a.y;
+    }
+    block_sum_to(s, partial, blockIdx.x);
+}
+
+// partial sums of x^2 (NaN skipped)
+__global__ void __launch_bounds__(kSorBlock)
+k_norm2(const float *__restrict__ x, int64_t n2, double *__restrict__ partial, const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    double s = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < n2; q += (int64_t)gridDim.x * kSorBlock) {
+        const f2v a = reinterpret_cast<const f2v *>(x)[q];
+        if (a.x == a.x) s += (double)a.x * (double)a.x;
+        if (a.y == a.y) s += (double)a.y * (double)a.y;
+    }
+    block_sum_to(s, partial, blockIdx.x);
+}
+
+// *out = sqrt(sum of partials)
+__global__ void k_sqrt_sum(const double *__restrict__ partial, int nblk, double *out, const double *gate,
+                           double thr) {
+    if (gated(gate, thr)) return;
+    __shared__ double sm[256];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nblk; q += 256) s += partial[q];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sqrt(sm[0]);
+}
+
+// dst = src / *den (NaN kept); a zero or non-finite norm leaves zeros
+__global__ void __launch_bounds__(kSorBlock)
+k_scale_to(float *__restrict__ dst, const float *__restrict__ src, const double *den, int64_t n2,
+           const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    const double d = *den;
+    const float inv = (d > 0.0 && d < 1e300) ? (float)(1.0 / d) : 0.0f;
+    for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < n2; q += (int64_t)gridDim.x * kSorBlock) {
+        f2v a = reinterpret_cast<const f2v *>(src)[q];
+        a.x = a.x == a.x ? a.x * inv : a.x;
+        a.y = a.y == a.y ? a.y * inv : a.y;
+        reinterpret_cast<f2v *>(dst)[q] = a;
+    }
+}
+
+// least squares min || beta e1 - H y || for the (m+1) x m Hessenberg H
+// (column-major, leading dimension m+1) by Givens rotations; one thread
+__global__ void k_lsq(const double *__restrict__ H, int m, const double *beta, double *y, const double *gate,
+                      double thr) {
+    if (gated(gate, thr)) return;
+    if (threadIdx.x != 0) return;
+    double R[kMaxKrylov + 1][kMaxKrylov];
+    double gv[kMaxKrylov + 1];
+    for (int j = 0; j < m; j++)
+        for (int i = 0; i <= m; i++) R[i][j] = H[i + (m + 1) * j];
+    for (int i = 0; i <= m; i++) gv[i] = 0.0;
+    gv[0] = *beta;
+    for (int j = 0; j < m; j++) {
+        const double a = R[j][j], b = R[j + 1][j];
+        const double r = sqrt(a * a + b * b);
+        const double cs = r > 0.0 ? a / r : 1.0, sn = r > 0.0 ? b / r : 0.0;
+        for (int k = j; k < m; k++) {
+            const double t1 = R[j][k], t2 = R[j + 1][k];
+            R[j][k] = cs * t1 + sn * t2;
+            R[j + 1][k] = -sn * t1 + cs * t2;
+        }
+        const double g1 = gv[j], g2 = gv[j + 1];
+        gv[j] = cs * g1 + sn * g2;
+        gv[j + 1] = -sn * g1 + cs * g2;
+    }
+    for (int j = m - 1; j >= 0; j--) {
+        double s = gv[j];
+        for (int k = j + 1; k < m; k++) s -= R[j][k] * y[k];
+        y[j] = R[j][j] != 0.0 ? s / R[j][j] : 0.0;
+    }
+}
+
+// pc += sum_j y_j z_j on cells the solve updates (mask not NaN)
+__global__ void __launch_bounds__(kSorBlock)
+k_update_pc(double *__restrict__ p0, double *__restrict__ p1, const float *__restrict__ Z, int64_t stride, int m,
+            const double *__restrict__ y, const float *__restrict__ mask, int64_t NC, const double *gate,
+            double thr) {
+    if (gated(gate, thr)) return;
+    for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < 2 * NC;
+         q += (int64_t)gridDim.x * kSorBlock) {
+        if (!(mask[q] == mask[q])) continue;
+        double d = 0.0;
+        for (int j = 0; j < m; j++) d += y[j] * (double)Z[j * stride + q];
+        double *p = q < NC ? p0 + q : p1 + (q - NC);
+        *p += d;
+    }
+}
+
 // ---------------------------------------------------------------- host side
 static SorGeom geom(const Ctx &c) {
     SorGeom g;
@@ -599,7 +838,9 @@ int sor_float_tables(Ctx &c) {
     return 0;
 }
 
+void gmres_release();
 void sor_release_tables() {
+    gmres_release();
     if (g_axf) { (void)hipFree(g_axf); (void)hipFree(g_ayf); (void)hipFree(g_azf); }
     g_axf = g_ayf = g_azf = nullptr;
     g_f_sx = g_f_sy = g_f_sz = 0;
@@ -780,6 +1021,10 @@ static int sor_l2(Ctx &c, bool mixed, double bn, int64_t *sweeps, int64_t *outer
         IB3D_CHECK(hipStreamSynchronize(c.stream));
         if (seen == 0) *rel0 = hr[0];
         for (int64_t q = 0; q < n; q++) {
+            if (!(hr[q] == hr[q]) || !std::isfinite(bn)) {
+                set_error("poissonSolver: the residual is not finite");
+                return -1;
+            }
             if (hc[q] < 0.5) {
                 const int64_t oc = seen + q;  // converged after oc outer iterations
                 if (mixed && oc > 0) add_increment(c);
@@ -805,6 +1050,142 @@ static int sor_l2(Ctx &c, bool mixed, double bn, int64_t *sweeps, int64_t *outer
     }
 }
 
+// GMRES-IR buffers: V (m+1 vectors) and Z (m vectors) of 2*NC floats, H,
+// beta, y and reduction partials
+static float *g_kv = nullptr, *g_kz = nullptr;
+static double *g_kh = nullptr, *g_kpart = nullptr, *g_kpart2 = nullptr;
+static int64_t g_k_nc = 0;
+static int g_k_m = 0;
+
+__global__ void k_fill_nan_f32(float *a, int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
+    if (q < n) a[q] = __builtin_nanf("");
+}
+
+static int gmres_buffers(Ctx &c, int m) {
+    if (g_kv && g_k_nc == c.NC && g_k_m == m) return 0;
+    IB3D_CHECK(hipStreamSynchronize(c.stream));
+    if (g_kv) { (void)hipFree(g_kv); (void)hipFree(g_kz); (void)hipFree(g_kh); (void)hipFree(g_kpart); (void)hipFree(g_kpart2); }
+    const int64_t stride = 2 * c.NC;
+    IB3D_CHECK(hipMalloc(&g_kv, (int64_t)(m + 1) * stride * sizeof(float)));
+    // the matrix-vector kernel writes interior rows only: every other entry
+    // of a basis vector must read as "not a cell" (NaN) in the vector kernels
+    hipLaunchKernelGGL(k_fill_nan_f32, dim3(nblocks((int64_t)(m + 1) * stride, kSorBlock)), dim3(kSorBlock), 0,
+                       c.stream, g_kv, (int64_t)(m + 1) * stride);
+    IB3D_CHECK(hipMalloc(&g_kz, (int64_t)m * stride * sizeof(float)));
+    IB3D_CHECK(dzero(g_kz, (int64_t)m * stride * sizeof(float)));   // cells the sweeps never write stay 0
+    IB3D_CHECK(hipMalloc(&g_kh, ((m + 1) * m + 1 + m + 8) * sizeof(double)));
+    IB3D_CHECK(dzero(g_kh, ((m + 1) * m + 1 + m + 8) * sizeof(double)));
+    IB3D_CHECK(hipMalloc(&g_kpart, (int64_t)kMaxKrylov * kVecBlocks * sizeof(double)));
+    IB3D_CHECK(hipMalloc(&g_kpart2, (int64_t)kVecBlocks * sizeof(double)));
+    g_k_nc = c.NC;
+    g_k_m = m;
+    return 0;
+}
+
+void gmres_release() {
+    if (g_kv) { (void)hipFree(g_kv); (void)hipFree(g_kz); (void)hipFree(g_kh); (void)hipFree(g_kpart); (void)hipFree(g_kpart2); }
+    g_kv = g_kz = nullptr;
+    g_kh = g_kpart = g_kpart2 = nullptr;
+    g_k_nc = 0;
+    g_k_m = 0;
+}
+
+// One refinement step (outer iteration): FP64 residual r (as FP32 in r32),
+// then m flexible-GMRES steps in FP32 with k SOR sweeps as preconditioner,
+// then pc += Z y in FP64.  All gated on the previous step's convergence flag.
+static void gmres_cycle(Ctx &c, int m, int k, const double *gate) {
+    const SorGeom g = geom(c);
+    const int64_t NC = c.NC, stride = 2 * NC, n2 = NC;  // n2: float pairs per vector
+    double *H = g_kh, *beta = g_kh + (m + 1) * m, *y = beta + 1;
+    const dim3 vb(kVecBlocks), tb(kSorBlock);
+    const float om = (float)c.omega, omm = (float)(1.0 - c.omega);
+    float *V0 = g_kv;
+    hipLaunchKernelGGL(k_norm2, vb, tb, 0, c.stream, c.r32[0], n2, g_kpart2, gate, 0.5);
+    hipLaunchKernelGGL(k_sqrt_sum, dim3(1), dim3(256), 0, c.stream, g_kpart2, kVecBlocks, beta, gate, 0.5);
+    hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, V0, c.r32[0], beta, n2, gate, 0.5);
+    for (int j = 0; j < m; j++) {
+        float *Z = g_kz + j * stride, *Vj = g_kv + j * stride, *Vn = g_kv + (j + 1) * stride;
+        for (int sw = 0; sw < k; sw++) {
+            for (int col = 0; col < 2; col++) {
+                if (sw == 0)
+                    hipLaunchKernelGGL((k_sor_pz<float, false, true, false>), pz_grid(c), tb, 0, c.stream,
+                                       Z + col * NC, Z + (1 - col) * NC, Vj + col * NC, g_axf, g_ayf, g_azf, g, col,
+                                       om, omm, nullptr, gate, 0.5, nullptr);
+                else
+                    hipLaunchKernelGGL((k_sor_pz<float, false, false, false>), pz_grid(c), tb, 0, c.stream,
+                                       Z + col * NC, Z + (1 - col) * NC, Vj + col * NC, g_axf, g_ayf, g_azf, g, col,
+                                       om, omm, nullptr, gate, 0.5, nullptr);
+            }
+        }
+        hipLaunchKernelGGL(k_matvec32_pz, pz_grid(c), tb, 0, c.stream, Z, Z + NC, V0, V0 + NC, Vn, Vn + NC, g_axf,
+                           g_ayf, g_azf, g, gate, 0.5);
+        double *hj = H + (int64_t)(m + 1) * j;
+        hipLaunchKernelGGL(k_dots, vb, tb, 0, c.stream, Vn, g_kv, stride, j + 1, n2, g_kpart, gate, 0.5);
+        hipLaunchKernelGGL(k_rows_sum, dim3(j + 1), dim3(256), 0, c.stream, g_kpart, kVecBlocks, hj, 0, gate, 0.5);
+        hipLaunchKernelGGL(k_orth, vb, tb, 0, c.stream, Vn, g_kv, stride, j + 1, hj, n2, g_kpart2, gate, 0.5);
+        hipLaunchKernelGGL(k_sqrt_sum, dim3(1), dim3(256), 0, c.stream, g_kpart2, kVecBlocks, hj + j + 1, gate, 0.5);
+        hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, Vn, Vn, hj + j + 1, n2, gate, 0.5);
+    }
+    hipLaunchKernelGGL(k_lsq, dim3(1), dim3(64), 0, c.stream, H, m, beta, y, gate, 0.5);
+    hipLaunchKernelGGL(k_update_pc, vb, tb, 0, c.stream, c.pc[0], c.pc[1], g_kz, stride, m, y, V0, NC, gate, 0.5);
+}
+
+static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double *rel, double *rel0) {
+    const int m = c.krylov_m, k = c.check;
+    if (sor_float_tables(c)) return -1;
+    if (gmres_buffers(c, m)) return -1;
+    if (ensure_partial(nblocks(2 * c.NC, kSorBlock))) return -1;
+    const int64_t max_outer = std::max<int64_t>(1, c.max_sweeps / ((int64_t)m * k));
+    if (ensure_slots(2 * (max_outer + 2))) return -1;
+    double *relv = c.slots, *conv = c.slots + c.n_slots / 2;
+    hipLaunchKernelGGL(k_clear_dead, dim3(nblocks(2 * c.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream, c.pc[0],
+                       c.pc[1], c.e32[0], c.e32[1], c.bdt[0], c.bdt[1], c.NC);
+    const bool cnn = cnn_active();
+    if (cnn) {
+        g_bn = bn;
+        residual_async(c, false, true, bn, c.scalars + 16, c.scalars + 17, nullptr);
+        IB3D_CHECK(hipGetLastError());
+        if (cnn_before_solve(c, c.scalars + 16, c.scalars + 18, resid_rel)) return -1;
+    }
+    residual_async(c, false, true, bn, relv, conv, nullptr);
+    int64_t o = 0, seen = 0;
+    int batch = 1;
+    std::vector<double> hr, hc;
+    for (;;) {
+        for (int q = 0; q < batch && o < max_outer; q++) {
+            o++;
+            const double *gate = conv + (o - 1);
+            gmres_cycle(c, m, k, gate);
+            residual_async(c, false, true, bn, relv + o, conv + o, gate);
+        }
+        IB3D_CHECK(hipGetLastError());
+        const int64_t n = o - seen + 1;
+        hr.resize(n);
+        hc.resize(n);
+        IB3D_CHECK(hipMemcpyAsync(hr.data(), relv + seen, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        IB3D_CHECK(hipMemcpyAsync(hc.data(), conv + seen, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        IB3D_CHECK(hipStreamSynchronize(c.stream));
+        if (seen == 0) *rel0 = hr[0];
+        for (int64_t q = 0; q < n; q++) {
+            if (!(hr[q] == hr[q]) || !std::isfinite(bn)) {
+                set_error("poissonSolver: the residual is not finite");
+                return -1;
+            }
+            if (hc[q] < 0.5 || seen + q >= max_outer) {
+                const int64_t oc = seen + q;
+                if (cnn && cnn_after_solve(c, c.scalars + 16, c.scalars + 18)) return -1;
+                *sweeps = oc * m * k;
+                *outer = oc;
+                *rel = hr[q];
+                return 0;
+            }
+        }
+        seen = o + 1;
+        batch = std::min(batch * 2, 8);
+    }
+}
+
 int sor_solve(Ctx &c, double bnorm2, double eps1) {
     const double bn = std::sqrt(bnorm2);
     if (c.mode == 0) {
@@ -818,7 +1199,11 @@ int sor_solve(Ctx &c, double bnorm2, double eps1) {
     }
     int64_t sw = 0, out = 0;
     double rel = 0, rel0 = 0;
-    if (sor_l2(c, c.mode == 2, bn, &sw, &out, &rel, &rel0)) return -1;
+    if (c.mode == 3) {
+        if (sor_gmres(c, bn, &sw, &out, &rel, &rel0)) return -1;
+    } else if (sor_l2(c, c.mode == 2, bn, &sw, &out, &rel, &rel0)) {
+        return -1;
+    }
     c.last_sweeps = sw;
     c.last_outer = out;
     c.last_relres = rel;

@@ -63,6 +63,7 @@ def lib():
         "ib3d_stressCal1": [D, PD],
         "ib3d_stressCal2": [D, PD],
         "ib3d_set_poisson_mode": [I, D, D, I, I, I64],
+        "ib3d_set_krylov": [I],
         "ib3d_poisson_stats": [PI64, PI64, PD, PD, PD, PD],
         "ib3d_residual_norm": [PD, PD],
         "ib3d_step": [I, D, D, D, D, PD],
@@ -263,6 +264,9 @@ class Solver:
     def set_poisson_mode(self, mode, omega=1.7, tol=1e-6, check=10, warm=False, max_sweeps=0):
         _ok(self.L.ib3d_set_poisson_mode(int(mode), float(omega), float(tol), int(check), int(bool(warm)),
                                          int(max_sweeps)))
+
+    def set_krylov(self, restart):
+        _ok(self.L.ib3d_set_krylov(int(restart)))
 
     def poisson_stats(self):
         sw, out = C.c_int64(), C.c_int64()

@@ -141,10 +141,16 @@ int ib3d_stressCal2(double re, double coef[4]);
  * mode 2: FP32 RB-SOR sweeps (`check` per outer iteration) on the defect
  *         equation inside an FP64 defect-correction loop with the same
  *         FP64 stopping rule.
+ * mode 3: FP64 refinement (GMRES-IR): each outer iteration solves the
+ *         defect equation with FP32 flexible GMRES, every Arnoldi step
+ *         preconditioned by `check` FP32 RB-SOR sweeps at `omega`; same FP64
+ *         stopping rule.
  * warm != 0 starts each solve from the previous step's pc instead of 0. */
 int ib3d_set_poisson_mode(int mode, double omega, double tol, int check, int warm,
                           int64_t max_sweeps);
-/* Statistics of the last solve: sweeps (FP64-equivalent SOR sweeps),
+/* mode 3: Arnoldi steps per refinement step, 1..23 (default 20) */
+int ib3d_set_krylov(int restart);
+/* Statistics of the last solve: sweeps (SOR sweeps of either precision),
  * outer iterations, final and initial relative residual, SOR ms, whole
  * poissonSolver ms. */
 int ib3d_poisson_stats(int64_t *sweeps, int64_t *outer, double *relres, double *relres0,

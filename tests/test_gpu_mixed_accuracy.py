@@ -5,7 +5,9 @@ The fixed sphere at 128x64x64 runs 40 steps three times on the GPU:
 - reference rule (FP64, max|dpc| < eps1, zero guess), itself checked bit for
   bit against the reference's own run (test_gpu_reference_trajectory);
 - FP64 RB-SOR to ||r||/||b|| <= 1e-6 from the previous step's correction;
-- FP32 RB-SOR inside FP64 defect correction to the same FP64 residual.
+- FP32 RB-SOR inside FP64 defect correction to the same FP64 residual;
+- FP32 flexible GMRES preconditioned by FP32 RB-SOR inside FP64 refinement
+  (GMRES-IR) to the same FP64 residual.
 Both L2 modes must keep Cd within 0.5% of the reference's Cd series, and the
 field differences are printed (they are the per-configuration errors stated
 in the documentation).  The differences come from the stopping rule, not
@@ -36,6 +38,9 @@ def run(case, mode):
         s.set_poisson_mode(1, omega=1.99, tol=1e-6, check=20, warm=True)
     elif mode == "mixed":
         s.set_poisson_mode(2, omega=1.99, tol=1e-6, check=20, warm=True)
+    elif mode == "gmres":
+        s.set_poisson_mode(3, omega=1.9, tol=1e-6, check=20, warm=True)
+        s.set_krylov(12)
     cds, relres = [], []
     for _ in range(NSTEPS):
         r = s.step()
@@ -60,7 +65,7 @@ def test_l2_modes_against_the_reference_rule(case_dir):
     ref, cd_ref, _ = run(case, "reference")
     np.testing.assert_allclose(cd_ref, cd_ref_golden, rtol=1e-10)
     res = {}
-    for mode in ("fp64", "mixed"):
+    for mode in ("fp64", "mixed", "gmres"):
         _say(f"[gpu] accuracy: {mode}")
         f, cd, rr = run(case, mode)
         assert max(rr) <= 1e-6
@@ -74,3 +79,7 @@ def test_l2_modes_against_the_reference_rule(case_dir):
     d = {k: relmax(res["mixed"][0][k], res["fp64"][0][k]) for k in ("u", "v", "w", "p")}
     _say("[gpu] mixed vs FP64 at the same 1e-6 residual: " + ", ".join(f"{k}={v:.3e}" for k, v in d.items()))
     assert d["u"] < 1e-9 and d["p"] < 1e-8
+    # GMRES-IR stops at a different iterate with the same residual bound
+    d = {k: relmax(res["gmres"][0][k], res["fp64"][0][k]) for k in ("u", "v", "w", "p")}
+    _say("[gpu] GMRES-IR vs FP64 at the same 1e-6 residual: " + ", ".join(f"{k}={v:.3e}" for k, v in d.items()))
+    assert d["u"] < 1e-4 and d["p"] < 1e-3

@@ -46,6 +46,19 @@ __device__ __forceinline__ int xcd_block() {
     return (int)((b & 7u) * (nb >> 3) + (b >> 3));
 }
 
+// The same for the 2-D (row block, z chunk) grids of the z-marching kernels:
+// the hardware deals linear block ids (x fastest) round-robin to the XCDs, so
+// consecutive ids would put neighbouring row blocks, whose y-neighbour rows
+// and chunk-boundary planes overlap, in different L2 caches.  Each XCD gets a
+// contiguous range of logical ids (row blocks of a few consecutive z chunks).
+__device__ __forceinline__ void xcd_block2(int &bx, int &by) {
+    const unsigned gx = gridDim.x, nb = gx * gridDim.y, b = blockIdx.y * gx + blockIdx.x;
+    const unsigned nfull = nb & ~7u;
+    const unsigned l = b >= nfull ? b : (b & 7u) * (nb >> 3) + (b >> 3);
+    by = (int)(l / gx);
+    bx = (int)(l - (unsigned)by * gx);
+}
+
 __device__ __forceinline__ double wave_max(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     return v;
@@ -248,12 +261,14 @@ k_sor_pz(T *__restrict__ mine, const T *__restrict__ other, const T *__restrict_
     typedef typename Vec<T>::v2 V2;
     typedef typename Vec<T>::v4 V4;
     if (gated(gate, thr)) return;
-    const int p = 2 * (int)(blockIdx.x * kSorBlock + threadIdx.x);
+    int bx, by;
+    xcd_block2(bx, by);
+    const int p = 2 * (int)(bx * kSorBlock + threadIdx.x);
     double d = 0.0;
     const int jj = (int)fdiv((uint32_t)p, g.dH);
     const int h = p - jj * g.H;
     if (p < g.plane && jj >= 1 && jj <= g.sy - 2) {
-        const int k0 = 1 + (int)blockIdx.y * kZch;
+        const int k0 = 1 + by * kZch;
         const int k1 = min(k0 + kZch, g.sz - 1);
         const V4 c1 = *reinterpret_cast<const V4 *>(axc + 2 * h);
         const V4 c2 = *reinterpret_cast<const V4 *>(axc + g.sxp + 2 * h);
@@ -374,12 +389,14 @@ k_resid_pz(const double *__restrict__ p0, const double *__restrict__ p1, const f
            const double *__restrict__ ayc, const double *__restrict__ azc, SorGeom g, double *partial,
            const double *gate, double thr) {
     if (gated(gate, thr)) return;
-    const int p = 2 * (int)(blockIdx.x * kSorBlock + threadIdx.x);
+    int bx, by;
+    xcd_block2(bx, by);
+    const int p = 2 * (int)(bx * kSorBlock + threadIdx.x);
     const int jj = (int)fdiv((uint32_t)p, g.dH);
     const int h = p - jj * g.H;
     double rr = 0.0;
     if (p < g.plane && jj >= 1 && jj <= g.sy - 2) {
-        const int k0 = 1 + (int)blockIdx.y * kZch;
+        const int k0 = 1 + by * kZch;
         const int k1 = min(k0 + kZch, g.sz - 1);
         const d4v c1 = *reinterpret_cast<const d4v *>(axc + 2 * h);
         const d4v c2 = *reinterpret_cast<const d4v *>(axc + g.sxp + 2 * h);
@@ -449,7 +466,7 @@ k_resid_pz(const double *__restrict__ p0, const double *__restrict__ p1, const f
             cur[1] = above[1];
         }
     }
-    block_sum_to(rr, partial, (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
+    block_sum_to(rr, partial, (int64_t)by * gridDim.x + bx);
 }
 
 // Cells the solve does not update (b/dt is NaN there) must hold pc = 0 and
@@ -536,11 +553,13 @@ k_matvec32_pz(const float *__restrict__ z0, const float *__restrict__ z1, const 
               const float *__restrict__ axc, const float *__restrict__ ayc, const float *__restrict__ azc, SorGeom g,
               const double *gate, double thr) {
     if (gated(gate, thr)) return;
-    const int p = 2 * (int)(blockIdx.x * kSorBlock + threadIdx.x);
+    int bx, by;
+    xcd_block2(bx, by);
+    const int p = 2 * (int)(bx * kSorBlock + threadIdx.x);
     const int jj = (int)fdiv((uint32_t)p, g.dH);
     const int h = p - jj * g.H;
     if (!(p < g.plane && jj >= 1 && jj <= g.sy - 2)) return;
-    const int k0 = 1 + (int)blockIdx.y * kZch;
+    const int k0 = 1 + by * kZch;
     const int k1 = min(k0 + kZch, g.sz - 1);
     const f4v c1 = *reinterpret_cast<const f4v *>(axc + 2 * h);
     const f4v c2 = *reinterpret_cast<const f4v *>(axc + g.sxp + 2 * h);
@@ -998,7 +1017,9 @@ static int sor_l2(Ctx &c, bool mixed, double bn, int64_t *sweeps, int64_t *outer
     residual_async(c, false, mixed, bn, relv, conv, nullptr);
     int64_t o = 0;     // last queued outer iteration
     int64_t seen = 0;  // first index not yet inspected on the host
-    int batch = 4;
+    // the previous solve's count predicts this one's (warm starts, slowly
+    // varying flow): queue a little less than that first, then grow
+    int batch = (int)std::min<int64_t>(64, std::max<int64_t>(4, c.last_outer - 2));
     std::vector<double> hr, hc;
     for (;;) {
         for (int q = 0; q < batch && o < max_outer; q++) {
@@ -1046,7 +1067,7 @@ static int sor_l2(Ctx &c, bool mixed, double bn, int64_t *sweeps, int64_t *outer
             return 0;
         }
         seen = o + 1;
-        batch = std::min(batch * 2, 64);
+        batch = std::min(std::max(batch, 4) * 2, 64);
     }
 }
 
@@ -1150,7 +1171,11 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
     }
     residual_async(c, false, true, bn, relv, conv, nullptr);
     int64_t o = 0, seen = 0;
-    int batch = 1;
+    // A cycle is hundreds of launches; one queued after convergence still
+    // costs their launch overhead.  Queue one cycle less than the previous
+    // solve needed, then one at a time (a host check costs far less than a
+    // cycle).
+    int batch = (int)std::min<int64_t>(16, std::max<int64_t>(1, c.last_outer - 1));
     std::vector<double> hr, hc;
     for (;;) {
         for (int q = 0; q < batch && o < max_outer; q++) {
@@ -1182,7 +1207,7 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
             }
         }
         seen = o + 1;
-        batch = std::min(batch * 2, 8);
+        batch = 1;
     }
 }
 

@@ -17,10 +17,14 @@ Poisson modes (same stopping rule: ||b/dt - A p||_2 <= 1e-6 ||b/dt||_2):
   mixed-dc      -- FP32 red-black SOR sweeps (omega 1.99) on the defect
                    equation inside an FP64 defect-correction loop,
                    previous-step guess
-  mixed         -- FP64 refinement whose defect equation is solved by FP32
+  gmres-sor     -- FP64 refinement whose defect equation is solved by FP32
                    flexible GMRES(12), every Arnoldi step preconditioned by
                    20 FP32 red-black SOR sweeps at omega 1.9 (GMRES-IR),
                    previous-step guess
+  mixed         -- the same FP64 refinement and FP32 flexible GMRES(12),
+                   every Arnoldi step preconditioned by one FP32 geometric
+                   multigrid V-cycle (red-black SOR smoothing, 2+2 sweeps per
+                   level, trilinear prolongation), previous-step guess
   mixed+learned -- mixed, starting from the CNN's guess when it has the
                    smaller residual (headline)
 The headline `value` is steps/s of --mode; the --compare modes are measured
@@ -65,6 +69,10 @@ def log(msg):
 # FP32 GMRES inside the FP64 refinement: Arnoldi steps per refinement step,
 # FP32 SOR sweeps per preconditioner application and their omega
 GMRES = {"m": 12, "k": 20, "omega": 1.9}
+# the multigrid preconditioner: Arnoldi steps per refinement step, pre- and
+# post-smoothing sweeps per level and their omega, sweeps and omega on the
+# coarsest level
+MG = {"m": 12, "nu1": 2, "nu2": 2, "omega": 1.0, "coarse": 16, "coarse_omega": 1.5}
 
 
 class Motion:
@@ -102,9 +110,13 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None,
         s.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=False)
     elif mode == "mixed-dc":
         s.set_poisson_mode(2, omega=omega, tol=tol, check=inner, warm=True)
-    elif mode in ("mixed", "mixed+learned"):
+    elif mode == "gmres-sor":
         s.set_poisson_mode(3, omega=GMRES["omega"], tol=tol, check=GMRES["k"], warm=True)
         s.set_krylov(GMRES["m"])
+    elif mode in ("mixed", "mixed+learned"):
+        s.set_poisson_mode(4, tol=tol, warm=True)
+        s.set_krylov(MG["m"])
+        s.set_multigrid(MG["nu1"], MG["nu2"], MG["omega"], MG["coarse"], MG["coarse_omega"], 0)
     elif mode == "reference":
         s.set_poisson_mode(0)
     else:
@@ -233,8 +245,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", default="mixed+learned",
-                    choices=["mixed+learned", "mixed", "mixed-dc", "fp64", "fp64-cold", "reference"])
-    ap.add_argument("--compare", default="fp64-cold,fp64,mixed-dc,mixed",
+                    choices=["mixed+learned", "mixed", "gmres-sor", "mixed-dc", "fp64", "fp64-cold", "reference"])
+    ap.add_argument("--compare", default="fp64-cold,fp64,mixed-dc,gmres-sor,mixed",
                     help="comma-separated modes measured in the same run for comparison ('none' to skip)")
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--omega", type=float, default=1.99)
@@ -291,7 +303,7 @@ def main():
         s, main_res = run_mode(Solver, case, a.mode, a.steps, a.warmup, a.omega, a.tol, a.inner, net, moving)
         main_fields = {f: s.download(f) for f in ("u", "p")}
         log(f"[bench] {a.mode}: {main_res}")
-        prec = 32 if a.mode.startswith("mixed") else 64
+        prec = 32 if a.mode.startswith("mixed") or a.mode == "gmres-sor" else 64
         roof = kernel_roofline(s, prec, a.omega, n_cells)
         roof64 = kernel_roofline(s, 64, a.omega, n_cells) if prec == 32 else roof
         s.close()
@@ -331,7 +343,7 @@ def main():
             "higher_is_better": True,
             "scaling": "weak",
             "vs_baseline": None,
-            "dtype": "fp32+fp64" if a.mode.startswith("mixed") else "fp64",
+            "dtype": "fp32+fp64" if prec == 32 else "fp64",
             "data": "synthetic (generated grid and icosphere, reference initial condition)",
             "config": {"workload": (f"sphere oscillating as y_c = {moving[0]} sin(2 pi {moving[1]} t) "
                                     f"(geometry rebuilt every step)" if moving else
@@ -339,13 +351,17 @@ def main():
                                     if cfg.extra.get("bodies") else "fixed sphere")
                                    + f" Re={cfg.extra.get('re_d', 100.0)}, {nx}x{ny}x{nz} stretched grid, "
                                    f"{n_tris} triangles, dt={cfg.dt}",
-                       "poisson": (f"{a.mode}: FP64 refinement, FP32 flexible GMRES({GMRES['m']}) preconditioned by "
-                                   f"{GMRES['k']} FP32 RB-SOR sweeps at omega={GMRES['omega']}"
+                       "poisson": (f"{a.mode}: FP64 refinement, FP32 flexible GMRES({MG['m']}) preconditioned by "
+                                   f"one FP32 multigrid V-cycle (RB-SOR {MG['nu1']}+{MG['nu2']} sweeps per level at "
+                                   f"omega={MG['omega']}, {MG['coarse']} on the coarsest, trilinear prolongation)"
                                    if a.mode in ("mixed", "mixed+learned") else
+                                   f"{a.mode}: FP64 refinement, FP32 flexible GMRES({GMRES['m']}) preconditioned by "
+                                   f"{GMRES['k']} FP32 RB-SOR sweeps at omega={GMRES['omega']}"
+                                   if a.mode == "gmres-sor" else
                                    f"{a.mode}: RB-SOR omega={a.omega}, {a.inner} sweeps between residual checks")
                                   + f", ||r||/||b|| <= {a.tol}, "
                                   + ("CNN guess with previous-step fallback" if a.mode == "mixed+learned"
-                                     else "previous-step guess" if a.mode in ("mixed", "mixed-dc", "fp64")
+                                     else "previous-step guess" if a.mode in ("mixed", "gmres-sor", "mixed-dc", "fp64")
                                      else "zero guess"),
                        "parallelism": "1 GPU"},
             "poisson_ms_per_step": round(main_res["poisson_ms_per_step"], 3),

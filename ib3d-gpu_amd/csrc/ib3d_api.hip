@@ -867,7 +867,7 @@ int ib3d_stressCal2(double re, double coef[4]) {
 
 int ib3d_set_poisson_mode(int mode, double omega, double tol, int check, int warm, int64_t max_sweeps) {
     Ctx &c = ctx();
-    IB3D_REQUIRE(mode >= 0 && mode <= 3, "set_poisson_mode: mode must be 0, 1, 2 or 3");
+    IB3D_REQUIRE(mode >= 0 && mode <= 4, "set_poisson_mode: mode must be 0, 1, 2, 3 or 4");
     IB3D_REQUIRE(omega > 0.0 && omega < 2.0, "set_poisson_mode: omega must lie in (0, 2)");
     IB3D_REQUIRE(check >= 1, "set_poisson_mode: check must be >= 1");
     IB3D_REQUIRE(tol > 0.0, "set_poisson_mode: tol must be positive");
@@ -884,6 +884,23 @@ int ib3d_set_krylov(int restart) {
     Ctx &c = ctx();
     IB3D_REQUIRE(restart >= 1 && restart <= 23, "set_krylov: restart length must be 1..23");
     c.krylov_m = restart;
+    return 0;
+}
+
+int ib3d_set_multigrid(int nu1, int nu2, double omega, int coarse_sweeps, double coarse_omega, int max_levels) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(nu1 >= 1 && nu1 <= 64, "set_multigrid: pre-smoothing sweeps must be 1..64");
+    IB3D_REQUIRE(nu2 >= 0 && nu2 <= 64, "set_multigrid: post-smoothing sweeps must be 0..64");
+    IB3D_REQUIRE(coarse_sweeps >= 1 && coarse_sweeps <= 1024, "set_multigrid: coarse sweeps must be 1..1024");
+    IB3D_REQUIRE(omega > 0.0 && omega < 2.0 && coarse_omega > 0.0 && coarse_omega < 2.0,
+                 "set_multigrid: relaxation factors must lie in (0, 2)");
+    IB3D_REQUIRE(max_levels == 0 || max_levels >= 2, "set_multigrid: max_levels must be 0 (no cap) or >= 2");
+    c.mg_nu1 = nu1;
+    c.mg_nu2 = nu2;
+    c.mg_omega = omega;
+    c.mg_coarse = coarse_sweeps;
+    c.mg_coarse_omega = coarse_omega;
+    c.mg_levels = max_levels;
     return 0;
 }
 

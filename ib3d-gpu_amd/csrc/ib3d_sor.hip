@@ -546,7 +546,9 @@ __global__ void __launch_bounds__(kSorBlock) k_copy_probe(const double4 *__restr
 // live on the device, so a whole cycle runs without the host.
 
 // y = A z (FP32) on cell pairs marching in z, both colours per thread;
-// NaN where the mask array is NaN
+// NaN where the mask array is NaN.  RESID: y = m - A z instead (the mask
+// array is then the right-hand side).
+template <bool RESID>
 __global__ void __launch_bounds__(kSorBlock)
 k_matvec32_pz(const float *__restrict__ z0, const float *__restrict__ z1, const float *__restrict__ m0,
               const float *__restrict__ m1, float *__restrict__ y0, float *__restrict__ y1,
@@ -592,11 +594,13 @@ k_matvec32_pz(const float *__restrict__ z0, const float *__restrict__ z1, const 
                     const float pw = q ? co.x : side, pe = q ? co.y : co.x;
                     out.x = (((((a1x0 * pw + a3x0 * pe) + a1y * ps.x) + a3y * pn.x) + a1z * below[o].x) +
                              a3z * above[o].x) + ((a2x0 + a2y) + a2z) * own.x;
+                    if (RESID) out.x = mk.x - out.x;
                 }
                 if (ok1) {
                     const float pw = q ? co.y : co.x, pe = q ? side : co.y;
                     out.y = (((((a1x1 * pw + a3x1 * pe) + a1y * ps.y) + a3y * pn.y) + a1z * below[o].y) +
                              a3z * above[o].y) + ((a2x1 + a2y) + a2z) * own.y;
+                    if (RESID) out.y = mk.y - out.y;
                 }
             }
             *reinterpret_cast<f2v *>(yc_[c] + f) = out;
@@ -730,35 +734,54 @@ k_scale_to(float *__restrict__ dst, const float *__restrict__ src, const double 
     }
 }
 
-// least squares min || beta e1 - H y || for the (m+1) x m Hessenberg H
-// (column-major, leading dimension m+1) by Givens rotations; one thread
-__global__ void k_lsq(const double *__restrict__ H, int m, const double *beta, double *y, const double *gate,
-                      double thr) {
-    if (gated(gate, thr)) return;
-    if (threadIdx.x != 0) return;
-    double R[kMaxKrylov + 1][kMaxKrylov];
-    double gv[kMaxKrylov + 1];
-    for (int j = 0; j < m; j++)
-        for (int i = 0; i <= m; i++) R[i][j] = H[i + (m + 1) * j];
-    for (int i = 0; i <= m; i++) gv[i] = 0.0;
-    gv[0] = *beta;
-    for (int j = 0; j < m; j++) {
-        const double a = R[j][j], b = R[j + 1][j];
-        const double r = sqrt(a * a + b * b);
-        const double cs = r > 0.0 ? a / r : 1.0, sn = r > 0.0 ? b / r : 0.0;
-        for (int k = j; k < m; k++) {
-            const double t1 = R[j][k], t2 = R[j + 1][k];
-            R[j][k] = cs * t1 + sn * t2;
-            R[j + 1][k] = -sn * t1 + cs * t2;
-        }
-        const double g1 = gv[j], g2 = gv[j + 1];
-        gv[j] = cs * g1 + sn * g2;
-        gv[j + 1] = -sn * g1 + cs * g2;
+// Arnoldi bookkeeping on the device.  act[j] >= 0.5 while Arnoldi step j
+// is to run: act[0] follows the outer gate, and after step j k_givens turns
+// the new column of H into a column of R (the rotations so far plus a new
+// one, Saad's GMRES with Givens rotations), rotates the right-hand side g and
+// clears act[j+1] once |g[j+1]| (the residual norm of the least-squares
+// solution after j+1 steps) is at most `target`; the remaining steps of the
+// cycle then exit at once.
+__global__ void k_act_init(double *act, int m, const double *gate, double thr) {
+    const int t = threadIdx.x;
+    if (t <= m) act[t] = (t == 0 && !gated(gate, thr)) ? 1.0 : 0.0;
+}
+
+__global__ void k_givens(double *__restrict__ H, int m, int j, double *cs, double *sn, double *gv, const double *beta,
+                         double *act, double target) {
+    if (act[j] < 0.5 || threadIdx.x != 0) return;
+    double *h = H + (int64_t)(m + 1) * j;
+    if (j == 0) gv[0] = *beta;
+    for (int i = 0; i < j; i++) {
+        const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+        h[i] = t;
     }
-    for (int j = m - 1; j >= 0; j--) {
+    const double a = h[j], b = h[j + 1];
+    const double r = sqrt(a * a + b * b);
+    const double c = r > 0.0 ? a / r : 1.0, s = r > 0.0 ? b / r : 0.0;
+    cs[j] = c;
+    sn[j] = s;
+    h[j] = r;
+    h[j + 1] = 0.0;
+    gv[j + 1] = -s * gv[j];
+    gv[j] = c * gv[j];
+    act[j + 1] = (r > 0.0 && fabs(gv[j + 1]) > target) ? 1.0 : 0.0;
+}
+
+// y from R y = g over the steps that ran (act[j] >= 0.5); y = 0 beyond;
+// *steps counts the steps run
+__global__ void k_lsq(const double *__restrict__ H, int m, const double *gv, const double *act, double *y,
+                      double *steps) {
+    if (act[0] < 0.5 || threadIdx.x != 0) return;
+    int k = 0;
+    while (k < m && act[k] >= 0.5) k++;
+    *steps += (double)k;
+    for (int j = m - 1; j >= k; j--) y[j] = 0.0;
+    for (int j = k - 1; j >= 0; j--) {
         double s = gv[j];
-        for (int k = j + 1; k < m; k++) s -= R[j][k] * y[k];
-        y[j] = R[j][j] != 0.0 ? s / R[j][j] : 0.0;
+        for (int i = j + 1; i < k; i++) s -= H[j + (int64_t)(m + 1) * i] * y[i];
+        const double d = H[j + (int64_t)(m + 1) * j];
+        y[j] = d != 0.0 ? s / d : 0.0;
     }
 }
 
@@ -772,7 +795,8 @@ k_update_pc(double *__restrict__ p0, double *__restrict__ p1, const float *__res
          q += (int64_t)gridDim.x * kSorBlock) {
         if (!(mask[q] == mask[q])) continue;
         double d = 0.0;
-        for (int j = 0; j < m; j++) d += y[j] * (double)Z[j * stride + q];
+        for (int j = 0; j < m; j++)
+            if (y[j] != 0.0) d += y[j] * (double)Z[j * stride + q];
         double *p = q < NC ? p0 + q : p1 + (q - NC);
         *p += d;
     }
@@ -858,7 +882,9 @@ int sor_float_tables(Ctx &c) {
 }
 
 void gmres_release();
+void mg_release();
 void sor_release_tables() {
+    mg_release();
     gmres_release();
     if (g_axf) { (void)hipFree(g_axf); (void)hipFree(g_ayf); (void)hipFree(g_azf); }
     g_axf = g_ayf = g_azf = nullptr;
@@ -1083,6 +1109,539 @@ __global__ void k_fill_nan_f32(float *a, int64_t n) {
     if (q < n) a[q] = __builtin_nanf("");
 }
 
+// ------------------------------------------------------------ multigrid
+// FP32 geometric multigrid V-cycle, the FGMRES preconditioner of Poisson
+// mode 4 (in place of k SOR sweeps).  Level l+1 halves every interior
+// dimension of level l: a coarse cell is the union of 2x2x2 fine cells and
+// its faces are every other fine face, so a stretched grid is coarsened
+// along its own lines.  The coarse Laplacian rows come from the coarse cell
+// centres with the same construction and boundary folds as
+// coefficientMatrix (subCoefficientMatrix.f90:8-53).  A coarse cell takes
+// part when any of its children does (its right-hand side is then the mean
+// of the active children's residuals, else NaN); cells that do not take part
+// hold 0, as solid and intercepted cells do on the fine grid.  Smoother:
+// red-black SOR sweeps (k_sor_pz), the first from zero; prolongation:
+// every active fine cell adds the trilinear interpolant of the coarse
+// correction at its centre (coarse cells outside the solve count as 0; at the
+// domain ends the ghost value follows the boundary fold: equal to the end
+// cell's at the Neumann ends, its negative at the x outlet).
+
+// coarse right-hand side = mean of the active children's residuals; NaN on
+// coarse cells without one and on ghost / padding slots
+__global__ void __launch_bounds__(kSorBlock)
+k_mg_restrict(const float *__restrict__ rf, SorGeom gf, float *__restrict__ bc, SorGeom gc, int nxc, int nyc,
+              int nzc, const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    const int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
+    if (q >= 2 * (int64_t)gc.NC) return;
+    const int C = q >= gc.NC ? 1 : 0;
+    int I, J, K;
+    decode(gc, (int)(q - (int64_t)C * gc.NC), C, I, J, K);
+    float out = __builtin_nanf("");
+    if (I >= 1 && I <= nxc && J >= 1 && J <= nyc && K >= 1 && K <= nzc) {
+        float s = 0.0f;
+        int n = 0;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int ii = 2 * I - 1 + (t & 1), jj = 2 * J - 1 + ((t >> 1) & 1), kk = 2 * K - 1 + (t >> 2);
+            const int c = (ii + jj + kk) & 1;
+            const float v = rf[(int64_t)c * gf.NC + (jj + gf.sy * kk) * gf.H + (ii >> 1)];
+            if (v == v) {
+                s += v;
+                n++;
+            }
+        }
+        if (n) out = s / (float)n;
+    }
+    bc[q] = out;
+}
+
+// Interpolation along one axis from a fine cell's centre: coarse cells i0,
+// i1 (padded indices of the coarse level) with weights w0, w1.
+struct MgAxis {
+    int i0, i1;
+    float w0, w1;
+};
+
+// x_fine += trilinear interpolant of x_coarse, on active fine cells
+__global__ void __launch_bounds__(kSorBlock)
+k_mg_prolong(float *__restrict__ xf, const float *__restrict__ bf, SorGeom gf, const float *__restrict__ xc,
+             SorGeom gc, const MgAxis *__restrict__ tx, const MgAxis *__restrict__ ty,
+             const MgAxis *__restrict__ tz, const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    const int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
+    if (q >= 2 * (int64_t)gf.NC) return;
+    if (!(bf[q] == bf[q])) return;
+    const int c = q >= gf.NC ? 1 : 0;
+    int ii, jj, kk;
+    decode(gf, (int)(q - (int64_t)c * gf.NC), c, ii, jj, kk);
+    const MgAxis ax = tx[ii], ay = ty[jj], az = tz[kk];
+    const int I[2] = {ax.i0, ax.i1}, J[2] = {ay.i0, ay.i1}, K[2] = {az.i0, az.i1};
+    const float wx[2] = {ax.w0, ax.w1}, wy[2] = {ay.w0, ay.w1}, wz[2] = {az.w0, az.w1};
+    float s = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int a = t & 1, b = (t >> 1) & 1, d = t >> 2;
+        const int C = (I[a] + J[b] + K[d]) & 1;
+        s += (wx[a] * wy[b] * wz[d]) * xc[(int64_t)C * gc.NC + (J[b] + gc.sy * K[d]) * gc.H + (I[a] >> 1)];
+    }
+    xf[q] += s;
+}
+
+// FP32 residual r = b - A x over both colours, one cell per thread (coarse
+// levels, where a z-marching thread has too little company); NaN where b is
+__global__ void __launch_bounds__(kSorBlock)
+k_mg_resid_cell(const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ r,
+                const float *__restrict__ axc, const float *__restrict__ ayc, const float *__restrict__ azc,
+                SorGeom g, const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    const int64_t q = (int64_t)xcd_block() * kSorBlock + threadIdx.x;
+    if (q >= 2 * (int64_t)g.NC) return;
+    const float rb = b[q];
+    if (!(rb == rb)) {
+        r[q] = rb;
+        return;
+    }
+    const int c = q >= g.NC ? 1 : 0;
+    const int f = (int)(q - (int64_t)c * g.NC);
+    int ii, jj, kk;
+    decode(g, f, c, ii, jj, kk);
+    const int s = (c + jj + kk) & 1;
+    const float *o = x + (int64_t)(1 - c) * g.NC;
+    const float a1x = axc[ii], a2x = axc[g.sxp + ii], a3x = axc[2 * g.sxp + ii];
+    const float a1y = ayc[jj], a2y = ayc[g.sy + jj], a3y = ayc[2 * g.sy + jj];
+    const float a1z = azc[kk], a2z = azc[g.sz + kk], a3z = azc[2 * g.sz + kk];
+    const float ax = (((((a1x * o[f - 1 + s] + a3x * o[f + s]) + a1y * o[f - g.H]) + a3y * o[f + g.H]) +
+                       a1z * o[f - g.plane]) + a3z * o[f + g.plane]) + ((a2x + a2y) + a2z) * x[q];
+    r[q] = rb - ax;
+}
+
+// ---- the smallest levels in one workgroup
+// One level as the single-workgroup kernel sees it.
+struct MgView {
+    float *x, *b, *r;
+    const float *ax, *ay, *az;
+    const MgAxis *tx, *ty, *tz;
+    int nx, ny, nz, H, NC, sy, sz, sxp;
+};
+static constexpr int kMgSmallMax = 8;
+static constexpr int64_t kMgSmallCells = 8192;
+static constexpr int kMgSmallThreads = 1024;
+struct MgSmall {
+    MgView L[kMgSmallMax];
+    int n, nu1, nu2, ncoarse;
+    float om, omc;
+};
+
+__device__ __forceinline__ void view_decode(const MgView &L, int f, int c, int &ii, int &jj, int &kk) {
+    const int row = f / L.H;
+    const int h = f - row * L.H;
+    kk = row / L.sy;
+    jj = row - kk * L.sy;
+    ii = 2 * h + ((c + jj + kk) & 1);
+}
+
+// one colour half-sweep of red-black SOR over the whole level, then a barrier
+__device__ void blk_sweep(const MgView &L, int c, bool first, float om) {
+    float *mine = L.x + (int64_t)c * L.NC;
+    const float *o = L.x + (int64_t)(1 - c) * L.NC, *rhs = L.b + (int64_t)c * L.NC;
+    const int plane = L.H * L.sy;
+    const bool no_other = first && c == 0;
+    for (int f = threadIdx.x; f < L.NC; f += blockDim.x) {
+        const float rr = rhs[f];
+        if (!(rr == rr)) continue;
+        int ii, jj, kk;
+        view_decode(L, f, c, ii, jj, kk);
+        const int s = (c + jj + kk) & 1;
+        const float a1x = L.ax[ii], a2x = L.ax[L.sxp + ii], a3x = L.ax[2 * L.sxp + ii];
+        const float a1y = L.ay[jj], a2y = L.ay[L.sy + jj], a3y = L.ay[2 * L.sy + jj];
+        const float a1z = L.az[kk], a2z = L.az[L.sz + kk], a3z = L.az[2 * L.sz + kk];
+        float pw = 0.0f, pe = 0.0f, ps = 0.0f, pn = 0.0f, pb = 0.0f, pt = 0.0f;
+        if (!no_other) {
+            pw = o[f - 1 + s];
+            pe = o[f + s];
+            ps = o[f - L.H];
+            pn = o[f + L.H];
+            pb = o[f - plane];
+            pt = o[f + plane];
+        }
+        const float pold = first ? 0.0f : mine[f];
+        const float gs = sor_div(((((((rr - a1x * pw) - a3x * pe) - a1y * ps) - a3y * pn) - a1z * pb) - a3z * pt),
+                                 ((a2x + a2y) + a2z));
+        mine[f] = (1.0f - om) * pold + om * gs;
+    }
+    __syncthreads();
+}
+
+__device__ void blk_resid(const MgView &L) {
+    const int plane = L.H * L.sy;
+    for (int q = threadIdx.x; q < 2 * L.NC; q += blockDim.x) {
+        const float rb = L.b[q];
+        if (!(rb == rb)) {
+            L.r[q] = rb;
+            continue;
+        }
+        const int c = q >= L.NC ? 1 : 0;
+        const int f = q - c * L.NC;
+        int ii, jj, kk;
+        view_decode(L, f, c, ii, jj, kk);
+        const int s = (c + jj + kk) & 1;
+        const float *o = L.x + (int64_t)(1 - c) * L.NC;
+        const float a1x = L.ax[ii], a2x = L.ax[L.sxp + ii], a3x = L.ax[2 * L.sxp + ii];
+        const float a1y = L.ay[jj], a2y = L.ay[L.sy + jj], a3y = L.ay[2 * L.sy + jj];
+        const float a1z = L.az[kk], a2z = L.az[L.sz + kk], a3z = L.az[2 * L.sz + kk];
+        const float ax = (((((a1x * o[f - 1 + s] + a3x * o[f + s]) + a1y * o[f - L.H]) + a3y * o[f + L.H]) +
+                           a1z * o[f - plane]) + a3z * o[f + plane]) + ((a2x + a2y) + a2z) * L.x[q];
+        L.r[q] = rb - ax;
+    }
+    __syncthreads();
+}
+
+__device__ void blk_restrict(const MgView &F, const MgView &C) {
+    for (int q = threadIdx.x; q < 2 * C.NC; q += blockDim.x) {
+        const int cc = q >= C.NC ? 1 : 0;
+        int I, J, K;
+        view_decode(C, q - cc * C.NC, cc, I, J, K);
+        float out = __builtin_nanf("");
+        if (I >= 1 && I <= C.nx && J >= 1 && J <= C.ny && K >= 1 && K <= C.nz) {
+            float s = 0.0f;
+            int n = 0;
+            for (int t = 0; t < 8; t++) {
+                const int ii = 2 * I - 1 + (t & 1), jj = 2 * J - 1 + ((t >> 1) & 1), kk = 2 * K - 1 + (t >> 2);
+                const int c = (ii + jj + kk) & 1;
+                const float v = F.r[c * F.NC + (jj + F.sy * kk) * F.H + (ii >> 1)];
+                if (v == v) {
+                    s += v;
+                    n++;
+                }
+            }
+            if (n) out = s / (float)n;
+        }
+        C.b[q] = out;
+    }
+    __syncthreads();
+}
+
+__device__ void blk_prolong(const MgView &F, const MgView &C) {
+    for (int q = threadIdx.x; q < 2 * F.NC; q += blockDim.x) {
+        if (!(F.b[q] == F.b[q])) continue;
+        const int c = q >= F.NC ? 1 : 0;
+        int ii, jj, kk;
+        view_decode(F, q - c * F.NC, c, ii, jj, kk);
+        const MgAxis ax = F.tx[ii], ay = F.ty[jj], az = F.tz[kk];
+        const int I[2] = {ax.i0, ax.i1}, J[2] = {ay.i0, ay.i1}, K[2] = {az.i0, az.i1};
+        const float wx[2] = {ax.w0, ax.w1}, wy[2] = {ay.w0, ay.w1}, wz[2] = {az.w0, az.w1};
+        float s = 0.0f;
+        for (int t = 0; t < 8; t++) {
+            const int a = t & 1, b = (t >> 1) & 1, d = t >> 2;
+            const int cc = (I[a] + J[b] + K[d]) & 1;
+            s += (wx[a] * wy[b] * wz[d]) * C.x[cc * C.NC + (J[b] + C.sy * K[d]) * C.H + (I[a] >> 1)];
+        }
+        F.x[q] += s;
+    }
+    __syncthreads();
+}
+
+// V-cycle over levels S.L[0..n) in one workgroup: x of level 0 from its b
+__global__ void __launch_bounds__(kMgSmallThreads) k_mg_small(MgSmall S, const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    for (int l = 0; l + 1 < S.n; l++) {
+        for (int s = 0; s < S.nu1; s++) {
+            blk_sweep(S.L[l], 0, s == 0, S.om);
+            blk_sweep(S.L[l], 1, s == 0, S.om);
+        }
+        blk_resid(S.L[l]);
+        blk_restrict(S.L[l], S.L[l + 1]);
+    }
+    for (int s = 0; s < S.ncoarse; s++) {
+        blk_sweep(S.L[S.n - 1], 0, s == 0, S.omc);
+        blk_sweep(S.L[S.n - 1], 1, s == 0, S.omc);
+    }
+    for (int l = S.n - 2; l >= 0; l--) {
+        blk_prolong(S.L[l], S.L[l + 1]);
+        for (int s = 0; s < S.nu2; s++) {
+            blk_sweep(S.L[l], 0, false, S.om);
+            blk_sweep(S.L[l], 1, false, S.om);
+        }
+    }
+}
+
+struct MgLevel {
+    int64_t nx = 0, ny = 0, nz = 0, sx = 0, sy = 0, sz = 0, H = 0, NC = 0, sxp = 0;
+    float *ax = nullptr, *ay = nullptr, *az = nullptr;  // Laplacian rows (level 0: the FP32 fine tables)
+    float *x = nullptr, *b = nullptr, *r = nullptr;     // correction, right-hand side, residual (2*NC each)
+    MgAxis *tx = nullptr, *ty = nullptr, *tz = nullptr;   // prolongation from the next level (sx, sy, sz entries)
+    bool owns_tables = false, owns_xb = false;
+};
+static std::vector<MgLevel> g_mg;
+static int g_mg_small = 0;  // first level cycled inside the single-workgroup kernel
+static int64_t g_mg_key[4] = {0, 0, 0, 0};
+
+static SorGeom level_geom(const MgLevel &L) {
+    SorGeom g;
+    g.H = (int)L.H;
+    g.NC = (int)L.NC;
+    g.sy = (int)L.sy;
+    g.sx = (int)L.sx;
+    g.sz = (int)L.sz;
+    g.plane = (int)(L.H * L.sy);
+    g.sxp = (int)L.sxp;
+    g.dH = make_fastdiv(g.H);
+    g.dSy = make_fastdiv(g.sy);
+    return g;
+}
+
+void mg_release() {
+    for (MgLevel &L : g_mg) {
+        if (L.owns_tables) { (void)hipFree(L.ax); (void)hipFree(L.ay); (void)hipFree(L.az); }
+        if (L.owns_xb) { (void)hipFree(L.x); (void)hipFree(L.b); }
+        if (L.r) (void)hipFree(L.r);
+        if (L.tx) { (void)hipFree(L.tx); (void)hipFree(L.ty); (void)hipFree(L.tz); }
+    }
+    g_mg.clear();
+    g_mg_key[0] = g_mg_key[1] = g_mg_key[2] = g_mg_key[3] = 0;
+}
+
+// Laplacian rows (3 x stride floats, row col holds A(ii, col+1) at ii = 1..n)
+// of an axis whose interior faces are `faces` (n+1 values): ghost faces are
+// extrapolated by the end widths, centres are face midpoints, and the end
+// rows are folded as coefficientMatrix does: Neumann at the low end of every
+// axis and at the high end of y and z, the antisymmetric outlet fold at the
+// high end of x.
+static std::vector<float> mg_rows(const std::vector<double> &faces, bool x_axis, int64_t stride) {
+    const int64_t n = (int64_t)faces.size() - 1;
+    std::vector<double> x1(n + 4, 0.0), xp(n + 4, 0.0), A(3 * n, 0.0);
+    for (int64_t i = 2; i <= n + 2; i++) x1[i] = faces[i - 2];
+    x1[1] = x1[2] - (x1[3] - x1[2]);
+    x1[n + 3] = x1[n + 2] + (x1[n + 2] - x1[n + 1]);
+    for (int64_t i = 1; i <= n + 2; i++) xp[i] = 0.5 * (x1[i] + x1[i + 1]);
+    for (int64_t i = 2; i <= n + 1; i++) {
+        const double r1 = xp[i] - xp[i - 1], r2 = xp[i + 1] - xp[i], rs = r1 + r2;
+        A[(i - 2) + 0 * n] = 2.0 / (r1 * rs);
+        A[(i - 2) + 1 * n] = -2.0 / (r1 * r2);
+        A[(i - 2) + 2 * n] = 2.0 / (r2 * rs);
+    }
+    A[0 + n] += A[0];
+    A[0] = 0.0;
+    if (x_axis)
+        A[(n - 1) + n] -= A[(n - 1) + 2 * n];
+    else
+        A[(n - 1) + n] += A[(n - 1) + 2 * n];
+    A[(n - 1) + 2 * n] = 0.0;
+    std::vector<float> r(3 * stride, 0.0f);
+    for (int col = 0; col < 3; col++)
+        for (int64_t ii = 1; ii <= n; ii++) r[col * stride + ii] = (float)A[(ii - 1) + n * col];
+    return r;
+}
+
+static bool mg_can_halve(int64_t n) { return n % 2 == 0 && n / 2 >= 2; }
+
+// interior faces of an axis at level l (every 2^l-th fine face; x1 holds the
+// fine faces at 1-based positions 2..n+2)
+static std::vector<double> mg_faces(const std::vector<double> &x1, int64_t n, int l) {
+    const int64_t f = int64_t(1) << l;
+    std::vector<double> g(n / f + 1);
+    for (int64_t j = 0; j <= n / f; j++) g[j] = x1[2 + f * j];
+    return g;
+}
+
+// centres of padded cells 0..n+1 (ghost widths repeat the end widths)
+static std::vector<double> mg_centres(const std::vector<double> &faces) {
+    const int64_t n = (int64_t)faces.size() - 1;
+    std::vector<double> c(n + 2);
+    for (int64_t i = 1; i <= n; i++) c[i] = 0.5 * (faces[i - 1] + faces[i]);
+    c[0] = faces[0] - 0.5 * (faces[1] - faces[0]);
+    c[n + 1] = faces[n] + 0.5 * (faces[n] - faces[n - 1]);
+    return c;
+}
+
+// linear interpolation weights from coarse centres to the fine centres of
+// one axis; `outlet`: the high-end ghost holds minus the end cell's value
+static std::vector<MgAxis> mg_axis(const std::vector<double> &ffaces, const std::vector<double> &cfaces,
+                                   bool outlet, int64_t size) {
+    const int64_t nf = (int64_t)ffaces.size() - 1, nc = (int64_t)cfaces.size() - 1;
+    const std::vector<double> xf = mg_centres(ffaces), xc = mg_centres(cfaces);
+    std::vector<MgAxis> t(size, MgAxis{1, 1, 0.0f, 0.0f});
+    for (int64_t i = 1; i <= nf; i++) {
+        const int64_t I = (i + 1) >> 1;
+        const int64_t N = xf[i] < xc[I] ? I - 1 : I + 1;  // the other coarse centre bracketing xf[i]
+        const double w = (xf[i] - xc[N]) / (xc[I] - xc[N]); // weight of I
+        MgAxis a;
+        a.i0 = (int)I;
+        a.w0 = (float)w;
+        if (N >= 1 && N <= nc) {
+            a.i1 = (int)N;
+            a.w1 = (float)(1.0 - w);
+        } else if (N > nc && outlet) {
+            a.i1 = (int)I;
+            a.w1 = (float)(-(1.0 - w));
+        } else {
+            a.i1 = (int)I;
+            a.w1 = (float)(1.0 - w);
+        }
+        t[i] = a;
+    }
+    return t;
+}
+
+static int mg_setup(Ctx &c) {
+    const int64_t key[4] = {c.nx, c.ny, c.nz, (int64_t)c.mg_levels};
+    if (!g_mg.empty() && key[0] == g_mg_key[0] && key[1] == g_mg_key[1] && key[2] == g_mg_key[2] &&
+        key[3] == g_mg_key[3])
+        return 0;
+    IB3D_CHECK(hipStreamSynchronize(c.stream));
+    mg_release();
+    if (sor_float_tables(c)) return -1;
+    MgLevel L0;
+    L0.nx = c.nx; L0.ny = c.ny; L0.nz = c.nz;
+    L0.sx = c.sx; L0.sy = c.sy; L0.sz = c.sz;
+    L0.H = c.H; L0.NC = c.NC; L0.sxp = c.sxp;
+    L0.ax = g_axf; L0.ay = g_ayf; L0.az = g_azf;
+    g_mg.push_back(L0);
+    const int max_levels = c.mg_levels > 0 ? c.mg_levels : 32;
+    while ((int)g_mg.size() < max_levels) {
+        const MgLevel &F = g_mg.back();
+        if (!(mg_can_halve(F.nx) && mg_can_halve(F.ny) && mg_can_halve(F.nz))) break;
+        MgLevel L;
+        L.nx = F.nx / 2; L.ny = F.ny / 2; L.nz = F.nz / 2;
+        L.sx = L.nx + 2; L.sy = L.ny + 2; L.sz = L.nz + 2;
+        L.H = (((L.sx + 1) / 2) + 1) & ~int64_t(1);
+        L.NC = L.sy * L.sz * L.H;
+        L.sxp = std::max<int64_t>((L.sx + 7) & ~int64_t(7), 2 * L.H);
+        g_mg.push_back(L);
+    }
+    IB3D_REQUIRE(g_mg.size() >= 2, "poissonSolver: the grid cannot be coarsened for multigrid (each interior "
+                                   "dimension must be even and at least 4)");
+    const int nl = (int)g_mg.size();
+    // the levels below kMgSmallCells cells (and at most kMgSmallMax of them) go to one workgroup; the
+    // finest level never does
+    g_mg_small = nl - 1;
+    while (g_mg_small > 1 && nl - (g_mg_small - 1) <= kMgSmallMax && 2 * g_mg[g_mg_small - 1].NC <= kMgSmallCells)
+        g_mg_small--;
+    for (int l = 0; l < nl; l++) {
+        MgLevel &L = g_mg[l];
+        if (check_fastdiv((int)L.H, L.NC + 1) || check_fastdiv((int)L.sy, L.NC / L.H + 1)) {
+            set_error("poissonSolver: fast division self-check failed on a multigrid level");
+            return -1;
+        }
+        if (l < nl - 1) {  // residual of this level feeds the next, whose correction comes back
+            IB3D_CHECK(hipMalloc(&L.r, 2 * L.NC * sizeof(float)));
+            hipLaunchKernelGGL(k_fill_nan_f32, dim3(nblocks(2 * L.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream,
+                               L.r, 2 * L.NC);
+            const std::vector<MgAxis> tx = mg_axis(mg_faces(c.hx1, c.nx, l), mg_faces(c.hx1, c.nx, l + 1), true, L.sx);
+            const std::vector<MgAxis> ty = mg_axis(mg_faces(c.hy1, c.ny, l), mg_faces(c.hy1, c.ny, l + 1), false, L.sy);
+            const std::vector<MgAxis> tz = mg_axis(mg_faces(c.hz1, c.nz, l), mg_faces(c.hz1, c.nz, l + 1), false, L.sz);
+            IB3D_CHECK(hipMalloc(&L.tx, tx.size() * sizeof(MgAxis)));
+            IB3D_CHECK(hipMalloc(&L.ty, ty.size() * sizeof(MgAxis)));
+            IB3D_CHECK(hipMalloc(&L.tz, tz.size() * sizeof(MgAxis)));
+            IB3D_CHECK(h2d(L.tx, tx.data(), tx.size() * sizeof(MgAxis)));
+            IB3D_CHECK(h2d(L.ty, ty.data(), ty.size() * sizeof(MgAxis)));
+            IB3D_CHECK(h2d(L.tz, tz.data(), tz.size() * sizeof(MgAxis)));
+        }
+        if (l == 0) continue;
+        const std::vector<float> rx = mg_rows(mg_faces(c.hx1, c.nx, l), true, L.sxp);
+        const std::vector<float> ry = mg_rows(mg_faces(c.hy1, c.ny, l), false, L.sy);
+        const std::vector<float> rz = mg_rows(mg_faces(c.hz1, c.nz, l), false, L.sz);
+        IB3D_CHECK(hipMalloc(&L.ax, rx.size() * 4));
+        IB3D_CHECK(hipMalloc(&L.ay, ry.size() * 4));
+        IB3D_CHECK(hipMalloc(&L.az, rz.size() * 4));
+        L.owns_tables = true;
+        IB3D_CHECK(h2d(L.ax, rx.data(), rx.size() * 4));
+        IB3D_CHECK(h2d(L.ay, ry.data(), ry.size() * 4));
+        IB3D_CHECK(h2d(L.az, rz.data(), rz.size() * 4));
+        IB3D_CHECK(hipMalloc(&L.x, 2 * L.NC * sizeof(float)));
+        IB3D_CHECK(hipMalloc(&L.b, 2 * L.NC * sizeof(float)));
+        L.owns_xb = true;
+        IB3D_CHECK(dzero(L.x, 2 * L.NC * sizeof(float)));   // cells outside the solve stay 0
+        hipLaunchKernelGGL(k_fill_nan_f32, dim3(nblocks(2 * L.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream,
+                           L.b, 2 * L.NC);
+    }
+    IB3D_CHECK(hipGetLastError());
+    for (int q = 0; q < 4; q++) g_mg_key[q] = key[q];
+    return 0;
+}
+
+static dim3 level_grid(const MgLevel &L) {
+    return dim3(nblocks((L.H * L.sy) / 2, kSorBlock), (unsigned)((L.sz - 2 + kZch - 1) / kZch));
+}
+
+// n red-black SOR sweeps on level L (the first from zero when `first`);
+// z-marching cell pairs on the finest level, one cell per thread below
+static void mg_smooth(Ctx &c, const MgLevel &L, bool finest, int n, bool first, double omega, const double *gate) {
+    const SorGeom g = level_geom(L);
+    const float om = (float)omega, omm = (float)(1.0 - omega);
+    const dim3 cg(nblocks(L.NC, kSorBlock));
+    for (int s = 0; s < n; s++) {
+        for (int col = 0; col < 2; col++) {
+            float *mine = L.x + col * L.NC;
+            const float *other = L.x + (1 - col) * L.NC, *rhs = L.b + col * L.NC;
+            if (!finest) {
+                if (first && s == 0)
+                    hipLaunchKernelGGL((k_sor<float, false, true, false>), cg, dim3(kSorBlock), 0, c.stream, mine,
+                                       other, rhs, L.ax, L.ay, L.az, g, col, om, omm, nullptr, gate, 0.5, nullptr);
+                else
+                    hipLaunchKernelGGL((k_sor<float, false, false, false>), cg, dim3(kSorBlock), 0, c.stream, mine,
+                                       other, rhs, L.ax, L.ay, L.az, g, col, om, omm, nullptr, gate, 0.5, nullptr);
+            } else if (first && s == 0)
+                hipLaunchKernelGGL((k_sor_pz<float, false, true, false>), level_grid(L), dim3(kSorBlock), 0, c.stream,
+                                   mine, other, rhs, L.ax, L.ay, L.az, g, col, om, omm, nullptr, gate, 0.5, nullptr);
+            else
+                hipLaunchKernelGGL((k_sor_pz<float, false, false, false>), level_grid(L), dim3(kSorBlock), 0,
+                                   c.stream, mine, other, rhs, L.ax, L.ay, L.az, g, col, om, omm, nullptr, gate, 0.5,
+                                   nullptr);
+        }
+    }
+}
+
+// z = M^-1 v: one V-cycle from a zero guess (v and z are 2*NC fine vectors).
+// Levels from g_mg_small on are cycled by one single-workgroup launch.
+static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
+    const int nl = (int)g_mg.size(), ns = g_mg_small;
+    g_mg[0].b = const_cast<float *>(v);
+    g_mg[0].x = z;
+    const dim3 tb(kSorBlock);
+    for (int l = 0; l < ns; l++) {
+        const MgLevel &F = g_mg[l], &C = g_mg[l + 1];
+        const SorGeom gf = level_geom(F), gc = level_geom(C);
+        mg_smooth(c, F, l == 0, c.mg_nu1, true, c.mg_omega, gate);
+        if (l == 0)
+            hipLaunchKernelGGL(k_matvec32_pz<true>, level_grid(F), tb, 0, c.stream, F.x, F.x + F.NC, F.b,
+                               F.b + F.NC, F.r, F.r + F.NC, F.ax, F.ay, F.az, gf, gate, 0.5);
+        else
+            hipLaunchKernelGGL(k_mg_resid_cell, dim3(nblocks(2 * F.NC, kSorBlock)), tb, 0, c.stream, F.x, F.b, F.r,
+                               F.ax, F.ay, F.az, gf, gate, 0.5);
+        hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks(2 * C.NC, kSorBlock)), tb, 0, c.stream, F.r, gf, C.b, gc,
+                           (int)C.nx, (int)C.ny, (int)C.nz, gate, 0.5);
+    }
+    MgSmall S;
+    S.n = nl - ns;
+    S.nu1 = c.mg_nu1;
+    S.nu2 = c.mg_nu2;
+    S.ncoarse = c.mg_coarse;
+    S.om = (float)c.mg_omega;
+    S.omc = (float)c.mg_coarse_omega;
+    for (int l = ns; l < nl; l++) {
+        const MgLevel &L = g_mg[l];
+        MgView &V = S.L[l - ns];
+        V.x = L.x; V.b = L.b; V.r = L.r;
+        V.ax = L.ax; V.ay = L.ay; V.az = L.az;
+        V.tx = L.tx; V.ty = L.ty; V.tz = L.tz;
+        V.nx = (int)L.nx; V.ny = (int)L.ny; V.nz = (int)L.nz;
+        V.H = (int)L.H; V.NC = (int)L.NC; V.sy = (int)L.sy; V.sz = (int)L.sz; V.sxp = (int)L.sxp;
+    }
+    hipLaunchKernelGGL(k_mg_small, dim3(1), dim3(kMgSmallThreads), 0, c.stream, S, gate, 0.5);
+    for (int l = ns - 1; l >= 0; l--) {
+        const MgLevel &F = g_mg[l], &C = g_mg[l + 1];
+        hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(2 * F.NC, kSorBlock)), tb, 0, c.stream, F.x, F.b,
+                           level_geom(F), C.x, level_geom(C), F.tx, F.ty, F.tz, gate, 0.5);
+        mg_smooth(c, F, l == 0, c.mg_nu2, false, c.mg_omega, gate);
+    }
+}
+
+
+// g_kh: H ((m+1) x m, column-major), beta, y[m], cs[m], sn[m], g[m+1], act[m+1]
+static int64_t kh_size(int m) { return (int64_t)(m + 1) * m + 1 + 3 * m + 2 * (m + 1); }
+
 static int gmres_buffers(Ctx &c, int m) {
     if (g_kv && g_k_nc == c.NC && g_k_m == m) return 0;
     IB3D_CHECK(hipStreamSynchronize(c.stream));
@@ -1095,8 +1654,8 @@ static int gmres_buffers(Ctx &c, int m) {
                        c.stream, g_kv, (int64_t)(m + 1) * stride);
     IB3D_CHECK(hipMalloc(&g_kz, (int64_t)m * stride * sizeof(float)));
     IB3D_CHECK(dzero(g_kz, (int64_t)m * stride * sizeof(float)));   // cells the sweeps never write stay 0
-    IB3D_CHECK(hipMalloc(&g_kh, ((m + 1) * m + 1 + m + 8) * sizeof(double)));
-    IB3D_CHECK(dzero(g_kh, ((m + 1) * m + 1 + m + 8) * sizeof(double)));
+    IB3D_CHECK(hipMalloc(&g_kh, kh_size(m) * sizeof(double)));
+    IB3D_CHECK(dzero(g_kh, kh_size(m) * sizeof(double)));
     IB3D_CHECK(hipMalloc(&g_kpart, (int64_t)kMaxKrylov * kVecBlocks * sizeof(double)));
     IB3D_CHECK(hipMalloc(&g_kpart2, (int64_t)kVecBlocks * sizeof(double)));
     g_k_nc = c.NC;
@@ -1113,53 +1672,66 @@ void gmres_release() {
 }
 
 // One refinement step (outer iteration): FP64 residual r (as FP32 in r32),
-// then m flexible-GMRES steps in FP32 with k SOR sweeps as preconditioner,
-// then pc += Z y in FP64.  All gated on the previous step's convergence flag.
-static void gmres_cycle(Ctx &c, int m, int k, const double *gate) {
+// then up to m flexible-GMRES steps in FP32, each preconditioned by k SOR
+// sweeps (mode 3) or one multigrid V-cycle (mode 4), stopping early once the
+// least-squares residual is at most `target`, then pc += Z y in FP64.  All
+// gated on the previous step's convergence flag.
+static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target) {
     const SorGeom g = geom(c);
     const int64_t NC = c.NC, stride = 2 * NC, n2 = NC;  // n2: float pairs per vector
-    double *H = g_kh, *beta = g_kh + (m + 1) * m, *y = beta + 1;
+    double *H = g_kh, *beta = H + (int64_t)(m + 1) * m, *y = beta + 1, *cs = y + m, *sn = cs + m, *gv = sn + m,
+           *act = gv + (m + 1);
     const dim3 vb(kVecBlocks), tb(kSorBlock);
     const float om = (float)c.omega, omm = (float)(1.0 - c.omega);
     float *V0 = g_kv;
-    hipLaunchKernelGGL(k_norm2, vb, tb, 0, c.stream, c.r32[0], n2, g_kpart2, gate, 0.5);
-    hipLaunchKernelGGL(k_sqrt_sum, dim3(1), dim3(256), 0, c.stream, g_kpart2, kVecBlocks, beta, gate, 0.5);
-    hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, V0, c.r32[0], beta, n2, gate, 0.5);
+    hipLaunchKernelGGL(k_act_init, dim3(1), dim3(64), 0, c.stream, act, m, gate, 0.5);
+    hipLaunchKernelGGL(k_norm2, vb, tb, 0, c.stream, c.r32[0], n2, g_kpart2, act, 0.5);
+    hipLaunchKernelGGL(k_sqrt_sum, dim3(1), dim3(256), 0, c.stream, g_kpart2, kVecBlocks, beta, act, 0.5);
+    hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, V0, c.r32[0], beta, n2, act, 0.5);
     for (int j = 0; j < m; j++) {
+        const double *aj = act + j;
         float *Z = g_kz + j * stride, *Vj = g_kv + j * stride, *Vn = g_kv + (j + 1) * stride;
-        for (int sw = 0; sw < k; sw++) {
-            for (int col = 0; col < 2; col++) {
-                if (sw == 0)
-                    hipLaunchKernelGGL((k_sor_pz<float, false, true, false>), pz_grid(c), tb, 0, c.stream,
-                                       Z + col * NC, Z + (1 - col) * NC, Vj + col * NC, g_axf, g_ayf, g_azf, g, col,
-                                       om, omm, nullptr, gate, 0.5, nullptr);
-                else
-                    hipLaunchKernelGGL((k_sor_pz<float, false, false, false>), pz_grid(c), tb, 0, c.stream,
-                                       Z + col * NC, Z + (1 - col) * NC, Vj + col * NC, g_axf, g_ayf, g_azf, g, col,
-                                       om, omm, nullptr, gate, 0.5, nullptr);
+        if (c.mode == 4) {
+            mg_vcycle(c, Vj, Z, aj);
+        } else {
+            for (int sw = 0; sw < k; sw++) {
+                for (int col = 0; col < 2; col++) {
+                    if (sw == 0)
+                        hipLaunchKernelGGL((k_sor_pz<float, false, true, false>), pz_grid(c), tb, 0, c.stream,
+                                           Z + col * NC, Z + (1 - col) * NC, Vj + col * NC, g_axf, g_ayf, g_azf, g,
+                                           col, om, omm, nullptr, aj, 0.5, nullptr);
+                    else
+                        hipLaunchKernelGGL((k_sor_pz<float, false, false, false>), pz_grid(c), tb, 0, c.stream,
+                                           Z + col * NC, Z + (1 - col) * NC, Vj + col * NC, g_axf, g_ayf, g_azf, g,
+                                           col, om, omm, nullptr, aj, 0.5, nullptr);
+                }
             }
         }
-        hipLaunchKernelGGL(k_matvec32_pz, pz_grid(c), tb, 0, c.stream, Z, Z + NC, V0, V0 + NC, Vn, Vn + NC, g_axf,
-                           g_ayf, g_azf, g, gate, 0.5);
+        hipLaunchKernelGGL(k_matvec32_pz<false>, pz_grid(c), tb, 0, c.stream, Z, Z + NC, V0, V0 + NC, Vn, Vn + NC,
+                           g_axf, g_ayf, g_azf, g, aj, 0.5);
         double *hj = H + (int64_t)(m + 1) * j;
-        hipLaunchKernelGGL(k_dots, vb, tb, 0, c.stream, Vn, g_kv, stride, j + 1, n2, g_kpart, gate, 0.5);
-        hipLaunchKernelGGL(k_rows_sum, dim3(j + 1), dim3(256), 0, c.stream, g_kpart, kVecBlocks, hj, 0, gate, 0.5);
-        hipLaunchKernelGGL(k_orth, vb, tb, 0, c.stream, Vn, g_kv, stride, j + 1, hj, n2, g_kpart2, gate, 0.5);
-        hipLaunchKernelGGL(k_sqrt_sum, dim3(1), dim3(256), 0, c.stream, g_kpart2, kVecBlocks, hj + j + 1, gate, 0.5);
-        hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, Vn, Vn, hj + j + 1, n2, gate, 0.5);
+        hipLaunchKernelGGL(k_dots, vb, tb, 0, c.stream, Vn, g_kv, stride, j + 1, n2, g_kpart, aj, 0.5);
+        hipLaunchKernelGGL(k_rows_sum, dim3(j + 1), dim3(256), 0, c.stream, g_kpart, kVecBlocks, hj, 0, aj, 0.5);
+        hipLaunchKernelGGL(k_orth, vb, tb, 0, c.stream, Vn, g_kv, stride, j + 1, hj, n2, g_kpart2, aj, 0.5);
+        hipLaunchKernelGGL(k_sqrt_sum, dim3(1), dim3(256), 0, c.stream, g_kpart2, kVecBlocks, hj + j + 1, aj, 0.5);
+        if (j + 1 < m)  // (before k_givens, which rotates h[j+1] to 0)
+            hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, Vn, Vn, hj + j + 1, n2, aj, 0.5);
+        hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, c.stream, H, m, j, cs, sn, gv, beta, act, target);
     }
-    hipLaunchKernelGGL(k_lsq, dim3(1), dim3(64), 0, c.stream, H, m, beta, y, gate, 0.5);
-    hipLaunchKernelGGL(k_update_pc, vb, tb, 0, c.stream, c.pc[0], c.pc[1], g_kz, stride, m, y, V0, NC, gate, 0.5);
+    hipLaunchKernelGGL(k_lsq, dim3(1), dim3(64), 0, c.stream, H, m, gv, act, y, c.scalars + 24);
+    hipLaunchKernelGGL(k_update_pc, vb, tb, 0, c.stream, c.pc[0], c.pc[1], g_kz, stride, m, y, V0, NC, act, 0.5);
 }
 
 static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double *rel, double *rel0) {
-    const int m = c.krylov_m, k = c.check;
+    const int m = c.krylov_m, k = c.mode == 4 ? 1 : c.check;
     if (sor_float_tables(c)) return -1;
     if (gmres_buffers(c, m)) return -1;
+    if (c.mode == 4 && mg_setup(c)) return -1;
     if (ensure_partial(nblocks(2 * c.NC, kSorBlock))) return -1;
     const int64_t max_outer = std::max<int64_t>(1, c.max_sweeps / ((int64_t)m * k));
     if (ensure_slots(2 * (max_outer + 2))) return -1;
     double *relv = c.slots, *conv = c.slots + c.n_slots / 2;
+    IB3D_CHECK(dzero(c.scalars + 24, sizeof(double)));  // Arnoldi steps run in this solve
     hipLaunchKernelGGL(k_clear_dead, dim3(nblocks(2 * c.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream, c.pc[0],
                        c.pc[1], c.e32[0], c.e32[1], c.bdt[0], c.bdt[1], c.NC);
     const bool cnn = cnn_active();
@@ -1181,7 +1753,7 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
         for (int q = 0; q < batch && o < max_outer; q++) {
             o++;
             const double *gate = conv + (o - 1);
-            gmres_cycle(c, m, k, gate);
+            gmres_cycle(c, m, k, gate, c.krylov_target * c.tol * bn);
             residual_async(c, false, true, bn, relv + o, conv + o, gate);
         }
         IB3D_CHECK(hipGetLastError());
@@ -1200,7 +1772,9 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
             if (hc[q] < 0.5 || seen + q >= max_outer) {
                 const int64_t oc = seen + q;
                 if (cnn && cnn_after_solve(c, c.scalars + 16, c.scalars + 18)) return -1;
-                *sweeps = oc * m * k;
+                double steps = 0.0;
+                IB3D_CHECK(d2h(&steps, c.scalars + 24, sizeof(double)));
+                *sweeps = (int64_t)steps * k;
                 *outer = oc;
                 *rel = hr[q];
                 return 0;
@@ -1224,7 +1798,7 @@ int sor_solve(Ctx &c, double bnorm2, double eps1) {
     }
     int64_t sw = 0, out = 0;
     double rel = 0, rel0 = 0;
-    if (c.mode == 3) {
+    if (c.mode == 3 || c.mode == 4) {
         if (sor_gmres(c, bn, &sw, &out, &rel, &rel0)) return -1;
     } else if (sor_l2(c, c.mode == 2, bn, &sw, &out, &rel, &rel0)) {
         return -1;

@@ -64,6 +64,7 @@ def lib():
         "ib3d_stressCal2": [D, PD],
         "ib3d_set_poisson_mode": [I, D, D, I, I, I64],
         "ib3d_set_krylov": [I],
+        "ib3d_set_multigrid": [I, I, D, I, D, I],
         "ib3d_poisson_stats": [PI64, PI64, PD, PD, PD, PD],
         "ib3d_residual_norm": [PD, PD],
         "ib3d_step": [I, D, D, D, D, PD],
@@ -267,6 +268,10 @@ class Solver:
 
     def set_krylov(self, restart):
         _ok(self.L.ib3d_set_krylov(int(restart)))
+
+    def set_multigrid(self, nu1=2, nu2=2, omega=1.0, coarse_sweeps=16, coarse_omega=1.5, max_levels=0):
+        _ok(self.L.ib3d_set_multigrid(int(nu1), int(nu2), float(omega), int(coarse_sweeps), float(coarse_omega),
+                                      int(max_levels)))
 
     def poisson_stats(self):
         sw, out = C.c_int64(), C.c_int64()

@@ -145,12 +145,23 @@ int ib3d_stressCal2(double re, double coef[4]);
  *         defect equation with FP32 flexible GMRES, every Arnoldi step
  *         preconditioned by `check` FP32 RB-SOR sweeps at `omega`; same FP64
  *         stopping rule.
+ * mode 4: as mode 3, every Arnoldi step preconditioned by one FP32
+ *         geometric-multigrid V-cycle instead (ib3d_set_multigrid); `check`
+ *         and `omega` are unused.
  * warm != 0 starts each solve from the previous step's pc instead of 0. */
 int ib3d_set_poisson_mode(int mode, double omega, double tol, int check, int warm,
                           int64_t max_sweeps);
-/* mode 3: Arnoldi steps per refinement step, 1..23 (default 20) */
+/* modes 3 and 4: Arnoldi steps per refinement step, 1..23 (default 20) */
 int ib3d_set_krylov(int restart);
-/* Statistics of the last solve: sweeps (SOR sweeps of either precision),
+/* mode 4 V-cycle: nu1 >= 1 pre- and nu2 >= 0 post-smoothing red-black SOR
+ * sweeps at `omega` on every level but the coarsest, `coarse_sweeps` at
+ * `coarse_omega` there; max_levels caps the hierarchy (0: halve every
+ * interior dimension while each stays even and >= 4).  Defaults 2, 2, 1.0,
+ * 16, 1.5, 0. */
+int ib3d_set_multigrid(int nu1, int nu2, double omega, int coarse_sweeps, double coarse_omega,
+                       int max_levels);
+/* Statistics of the last solve: sweeps (SOR sweeps of either precision;
+ * V-cycles in mode 4),
  * outer iterations, final and initial relative residual, SOR ms, whole
  * poissonSolver ms. */
 int ib3d_poisson_stats(int64_t *sweeps, int64_t *outer, double *relres, double *relres0,

@@ -7,8 +7,9 @@ The fixed sphere at 128x64x64 runs 40 steps three times on the GPU:
 - FP64 RB-SOR to ||r||/||b|| <= 1e-6 from the previous step's correction;
 - FP32 RB-SOR inside FP64 defect correction to the same FP64 residual;
 - FP32 flexible GMRES preconditioned by FP32 RB-SOR inside FP64 refinement
-  (GMRES-IR) to the same FP64 residual.
-Both L2 modes must keep Cd within 0.5% of the reference's Cd series, and the
+  (GMRES-IR) to the same FP64 residual;
+- the same with one FP32 geometric-multigrid V-cycle as the preconditioner.
+All L2 modes must keep Cd within 0.5% of the reference's Cd series, and the
 field differences are printed (they are the per-configuration errors stated
 in the documentation).  The differences come from the stopping rule, not
 from FP32: the mixed and the FP64 L2 solutions agree far more closely with
@@ -41,6 +42,10 @@ def run(case, mode):
     elif mode == "gmres":
         s.set_poisson_mode(3, omega=1.9, tol=1e-6, check=20, warm=True)
         s.set_krylov(12)
+    elif mode == "multigrid":
+        s.set_poisson_mode(4, tol=1e-6, warm=True)
+        s.set_krylov(12)
+        s.set_multigrid(2, 2, 1.0, 16, 1.5, 0)
     cds, relres = [], []
     for _ in range(NSTEPS):
         r = s.step()
@@ -65,7 +70,7 @@ def test_l2_modes_against_the_reference_rule(case_dir):
     ref, cd_ref, _ = run(case, "reference")
     np.testing.assert_allclose(cd_ref, cd_ref_golden, rtol=1e-10)
     res = {}
-    for mode in ("fp64", "mixed", "gmres"):
+    for mode in ("fp64", "mixed", "gmres", "multigrid"):
         _say(f"[gpu] accuracy: {mode}")
         f, cd, rr = run(case, mode)
         assert max(rr) <= 1e-6
@@ -80,6 +85,7 @@ def test_l2_modes_against_the_reference_rule(case_dir):
     _say("[gpu] mixed vs FP64 at the same 1e-6 residual: " + ", ".join(f"{k}={v:.3e}" for k, v in d.items()))
     assert d["u"] < 1e-9 and d["p"] < 1e-8
     # GMRES-IR stops at a different iterate with the same residual bound
-    d = {k: relmax(res["gmres"][0][k], res["fp64"][0][k]) for k in ("u", "v", "w", "p")}
-    _say("[gpu] GMRES-IR vs FP64 at the same 1e-6 residual: " + ", ".join(f"{k}={v:.3e}" for k, v in d.items()))
-    assert d["u"] < 1e-4 and d["p"] < 1e-3
+    for mode in ("gmres", "multigrid"):
+        d = {k: relmax(res[mode][0][k], res["fp64"][0][k]) for k in ("u", "v", "w", "p")}
+        _say(f"[gpu] {mode} vs FP64 at the same 1e-6 residual: " + ", ".join(f"{k}={v:.3e}" for k, v in d.items()))
+        assert d["u"] < 1e-4 and d["p"] < 1e-3

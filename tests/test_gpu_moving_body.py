@@ -98,9 +98,14 @@ def test_warm_started_modes_agree_with_cold_fp64_on_a_moving_body(case_dir):
     from ib3d_amd.native import Solver
     d = case_dir("tiny")
     runs = {}
-    for name, mode, warm in [("cold", 1, False), ("fp64", 1, True), ("mixed", 2, True)]:
+    for name, mode, warm in [("cold", 1, False), ("fp64", 1, True), ("mixed", 2, True), ("gmres", 3, True),
+                             ("multigrid", 4, True)]:
         s = Solver.from_case(d)
-        s.set_poisson_mode(mode, omega=1.99, tol=1e-9, check=10, warm=warm)
+        if mode >= 3:
+            s.set_poisson_mode(mode, omega=1.9, tol=1e-9, check=20, warm=warm)
+            s.set_krylov(12)
+        else:
+            s.set_poisson_mode(mode, omega=1.99, tol=1e-9, check=10, warm=warm)
         dt = s.deltat
         for k in range(1, 7):
             y, v, a = motion(k * dt, 0.1, 2.0, dt)
@@ -110,7 +115,7 @@ def test_warm_started_modes_agree_with_cold_fp64_on_a_moving_body(case_dir):
             assert s.poisson_stats()["relres"] <= 1e-9
         runs[name] = {f: s.download(f) for f in ("u", "p")}
         s.close()
-    for name in ("fp64", "mixed"):
+    for name in ("fp64", "mixed", "gmres", "multigrid"):
         for f in ("u", "p"):
             e = float(np.max(np.abs(runs[name][f] - runs["cold"][f])) / np.max(np.abs(runs["cold"][f])))
             _say(f"[gpu] moving body, {name} warm vs cold FP64: {f} rel diff {e:.2e}")

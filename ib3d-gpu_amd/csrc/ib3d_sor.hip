@@ -1126,36 +1126,6 @@ __global__ void k_fill_nan_f32(float *a, int64_t n) {
 // domain ends the ghost value follows the boundary fold: equal to the end
 // cell's at the Neumann ends, its negative at the x outlet).
 
-// coarse right-hand side = mean of the active children's residuals; NaN on
-// coarse cells without one and on ghost / padding slots
-__global__ void __launch_bounds__(kSorBlock)
-k_mg_restrict(const float *__restrict__ rf, SorGeom gf, float *__restrict__ bc, SorGeom gc, int nxc, int nyc,
-              int nzc, const double *gate, double thr) {
-    if (gated(gate, thr)) return;
-    const int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
-    if (q >= 2 * (int64_t)gc.NC) return;
-    const int C = q >= gc.NC ? 1 : 0;
-    int I, J, K;
-    decode(gc, (int)(q - (int64_t)C * gc.NC), C, I, J, K);
-    float out = __builtin_nanf("");
-    if (I >= 1 && I <= nxc && J >= 1 && J <= nyc && K >= 1 && K <= nzc) {
-        float s = 0.0f;
-        int n = 0;
-#pragma unroll
-        for (int t = 0; t < 8; t++) {
-            const int ii = 2 * I - 1 + (t & 1), jj = 2 * J - 1 + ((t >> 1) & 1), kk = 2 * K - 1 + (t >> 2);
-            const int c = (ii + jj + kk) & 1;
-            const float v = rf[(int64_t)c * gf.NC + (jj + gf.sy * kk) * gf.H + (ii >> 1)];
-            if (v == v) {
-                s += v;
-                n++;
-            }
-        }
-        if (n) out = s / (float)n;
-    }
-    bc[q] = out;
-}
-
 // Interpolation along one axis from a fine cell's centre: coarse cells i0,
 // i1 (padded indices of the coarse level) with weights w0, w1.
 struct MgAxis {
@@ -1163,74 +1133,13 @@ struct MgAxis {
     float w0, w1;
 };
 
-// x_fine += trilinear interpolant of x_coarse, on active fine cells
-__global__ void __launch_bounds__(kSorBlock)
-k_mg_prolong(float *__restrict__ xf, const float *__restrict__ bf, SorGeom gf, const float *__restrict__ xc,
-             SorGeom gc, const MgAxis *__restrict__ tx, const MgAxis *__restrict__ ty,
-             const MgAxis *__restrict__ tz, const double *gate, double thr) {
-    if (gated(gate, thr)) return;
-    const int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
-    if (q >= 2 * (int64_t)gf.NC) return;
-    if (!(bf[q] == bf[q])) return;
-    const int c = q >= gf.NC ? 1 : 0;
-    int ii, jj, kk;
-    decode(gf, (int)(q - (int64_t)c * gf.NC), c, ii, jj, kk);
-    const MgAxis ax = tx[ii], ay = ty[jj], az = tz[kk];
-    const int I[2] = {ax.i0, ax.i1}, J[2] = {ay.i0, ay.i1}, K[2] = {az.i0, az.i1};
-    const float wx[2] = {ax.w0, ax.w1}, wy[2] = {ay.w0, ay.w1}, wz[2] = {az.w0, az.w1};
-    float s = 0.0f;
-#pragma unroll
-    for (int t = 0; t < 8; t++) {
-        const int a = t & 1, b = (t >> 1) & 1, d = t >> 2;
-        const int C = (I[a] + J[b] + K[d]) & 1;
-        s += (wx[a] * wy[b] * wz[d]) * xc[(int64_t)C * gc.NC + (J[b] + gc.sy * K[d]) * gc.H + (I[a] >> 1)];
-    }
-    xf[q] += s;
-}
-
-// FP32 residual r = b - A x over both colours, one cell per thread (coarse
-// levels, where a z-marching thread has too little company); NaN where b is
-__global__ void __launch_bounds__(kSorBlock)
-k_mg_resid_cell(const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ r,
-                const float *__restrict__ axc, const float *__restrict__ ayc, const float *__restrict__ azc,
-                SorGeom g, const double *gate, double thr) {
-    if (gated(gate, thr)) return;
-    const int64_t q = (int64_t)xcd_block() * kSorBlock + threadIdx.x;
-    if (q >= 2 * (int64_t)g.NC) return;
-    const float rb = b[q];
-    if (!(rb == rb)) {
-        r[q] = rb;
-        return;
-    }
-    const int c = q >= g.NC ? 1 : 0;
-    const int f = (int)(q - (int64_t)c * g.NC);
-    int ii, jj, kk;
-    decode(g, f, c, ii, jj, kk);
-    const int s = (c + jj + kk) & 1;
-    const float *o = x + (int64_t)(1 - c) * g.NC;
-    const float a1x = axc[ii], a2x = axc[g.sxp + ii], a3x = axc[2 * g.sxp + ii];
-    const float a1y = ayc[jj], a2y = ayc[g.sy + jj], a3y = ayc[2 * g.sy + jj];
-    const float a1z = azc[kk], a2z = azc[g.sz + kk], a3z = azc[2 * g.sz + kk];
-    const float ax = (((((a1x * o[f - 1 + s] + a3x * o[f + s]) + a1y * o[f - g.H]) + a3y * o[f + g.H]) +
-                       a1z * o[f - g.plane]) + a3z * o[f + g.plane]) + ((a2x + a2y) + a2z) * x[q];
-    r[q] = rb - ax;
-}
-
-// ---- the smallest levels in one workgroup
-// One level as the single-workgroup kernel sees it.
+// One level as the kernels see it: two colour arrays per vector (2*NC floats).
 struct MgView {
-    float *x, *b, *r;
-    const float *ax, *ay, *az;
-    const MgAxis *tx, *ty, *tz;
+    float *x, *b;                   // correction, right-hand side (NaN where the cell is not solved for)
+    float *r;                       // stored residual b - A x (finest level only)
+    const float *ax, *ay, *az;      // Laplacian rows
+    const MgAxis *tx, *ty, *tz;     // prolongation from the next level
     int nx, ny, nz, H, NC, sy, sz, sxp;
-};
-static constexpr int kMgSmallMax = 8;
-static constexpr int64_t kMgSmallCells = 8192;
-static constexpr int kMgSmallThreads = 1024;
-struct MgSmall {
-    MgView L[kMgSmallMax];
-    int n, nu1, nu2, ncoarse;
-    float om, omc;
 };
 
 __device__ __forceinline__ void view_decode(const MgView &L, int f, int c, int &ii, int &jj, int &kk) {
@@ -1239,6 +1148,135 @@ __device__ __forceinline__ void view_decode(const MgView &L, int f, int c, int &
     kk = row / L.sy;
     jj = row - kk * L.sy;
     ii = 2 * h + ((c + jj + kk) & 1);
+}
+
+// b - A x at an active cell (colour c, slot f, padded position ii, jj, kk);
+// the same operation order as the FP32 matrix-vector product
+__device__ __forceinline__ float cell_resid(const MgView &L, float rb, int c, int f, int ii, int jj, int kk) {
+    const float *o = L.x + (int64_t)(1 - c) * L.NC;
+    const int s = ii & 1, plane = L.H * L.sy;
+    const float a1x = L.ax[ii], a2x = L.ax[L.sxp + ii], a3x = L.ax[2 * L.sxp + ii];
+    const float a1y = L.ay[jj], a2y = L.ay[L.sy + jj], a3y = L.ay[2 * L.sy + jj];
+    const float a1z = L.az[kk], a2z = L.az[L.sz + kk], a3z = L.az[2 * L.sz + kk];
+    const float ax = (((((a1x * o[f - 1 + s] + a3x * o[f + s]) + a1y * o[f - L.H]) + a3y * o[f + L.H]) +
+                       a1z * o[f - plane]) + a3z * o[f + plane]) +
+                     ((a2x + a2y) + a2z) * L.x[(int64_t)c * L.NC + f];
+    return rb - ax;
+}
+
+// coarse right-hand side at coarse slot q (colour cc, slot fc): the mean of
+// the active children's residuals b - A x (STORED: read from F.r); NaN on
+// coarse cells without an active child and on ghost / padding slots
+template <bool STORED>
+__device__ __forceinline__ float restrict_at(const MgView &F, const MgView &C, int cc, int fc, int I, int J, int K) {
+    float out = __builtin_nanf("");
+    if (I >= 1 && I <= C.nx && J >= 1 && J <= C.ny && K >= 1 && K <= C.nz) {
+        float s = 0.0f;
+        int n = 0;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int ii = 2 * I - 1 + (t & 1), jj = 2 * J - 1 + ((t >> 1) & 1), kk = 2 * K - 1 + (t >> 2);
+            const int c = (ii + jj + kk) & 1;
+            const int f = (jj + F.sy * kk) * F.H + (ii >> 1);
+            if (STORED) {
+                const float v = F.r[(int64_t)c * F.NC + f];
+                if (v == v) {
+                    s += v;
+                    n++;
+                }
+            } else {
+                const float rb = F.b[(int64_t)c * F.NC + f];
+                if (rb == rb) {
+                    s += cell_resid(F, rb, c, f, ii, jj, kk);
+                    n++;
+                }
+            }
+        }
+        if (n) out = s / (float)n;
+    }
+    return out;
+}
+
+// trilinear interpolant of the coarse correction at an active fine cell
+__device__ __forceinline__ float prolong_at(const MgView &F, const MgView &C, int ii, int jj, int kk) {
+    const MgAxis ax = F.tx[ii], ay = F.ty[jj], az = F.tz[kk];
+    const int I[2] = {ax.i0, ax.i1}, J[2] = {ay.i0, ay.i1}, K[2] = {az.i0, az.i1};
+    const float wx[2] = {ax.w0, ax.w1}, wy[2] = {ay.w0, ay.w1}, wz[2] = {az.w0, az.w1};
+    float s = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int a = t & 1, b = (t >> 1) & 1, d = t >> 2;
+        const int cc = (I[a] + J[b] + K[d]) & 1;
+        s += (wx[a] * wy[b] * wz[d]) * C.x[(int64_t)cc * C.NC + (J[b] + C.sy * K[d]) * C.H + (I[a] >> 1)];
+    }
+    return s;
+}
+
+// residual of the fine level restricted to the coarse right-hand side, one
+// thread per coarse slot; STORED: the finest level's residual was stored by
+// the z-marching residual kernel, else it is formed here
+template <bool STORED>
+__global__ void __launch_bounds__(kSorBlock)
+k_mg_resid_restrict(MgView F, MgView C, SorGeom gc, const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    const int64_t q = (int64_t)xcd_block() * kSorBlock + threadIdx.x;
+    if (q >= 2 * (int64_t)C.NC) return;
+    const int cc = q >= C.NC ? 1 : 0;
+    const int fc = (int)(q - (int64_t)cc * C.NC);
+    int I, J, K;
+    decode(gc, fc, cc, I, J, K);
+    C.b[q] = restrict_at<STORED>(F, C, cc, fc, I, J, K);
+}
+
+// x_fine += trilinear interpolant of x_coarse, on active fine cells
+__global__ void __launch_bounds__(kSorBlock)
+k_mg_prolong(MgView F, MgView C, SorGeom gf, const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    const int64_t q = (int64_t)xcd_block() * kSorBlock + threadIdx.x;
+    if (q >= 2 * (int64_t)F.NC) return;
+    if (!(F.b[q] == F.b[q])) return;
+    const int c = q >= F.NC ? 1 : 0;
+    int ii, jj, kk;
+    decode(gf, (int)(q - (int64_t)c * F.NC), c, ii, jj, kk);
+    F.x[q] += prolong_at(F, C, ii, jj, kk);
+}
+
+// ---- the smallest levels in one workgroup, held in LDS
+static constexpr int kMgSmallMax = 8;
+static constexpr int kMgSmallThreads = 1024;
+static constexpr int64_t kMgSmallLds = 64 * 1024;
+struct MgSmall {
+    MgView L[kMgSmallMax];   // global arrays of the levels (x, b used for level 0 only)
+    int off[kMgSmallMax];    // LDS offset (floats) of each level's block: x, b (2*NC each), the Laplacian
+                             // rows (3*sxp, 3*sy, 3*sz) and, but for the coarsest, the prolongation
+                             // tables (sx + sy + sz entries, 16-byte aligned)
+    int n, nu1, nu2, ncoarse;
+    float om, omc;
+};
+
+// floats of one level's LDS block (a multiple of 4)
+__host__ __device__ inline int mg_lds_floats(int NC, int sx, int sy, int sz, int sxp, bool tables) {
+    const int rows = (3 * (sxp + sy + sz) + 3) & ~3;
+    return 4 * NC + rows + (tables ? 4 * (sx + sy + sz) : 0);
+}
+
+__device__ __forceinline__ MgView lds_view(const MgSmall &S, int l, float *lds) {
+    MgView v = S.L[l];
+    float *p = lds + S.off[l];
+    v.x = p;
+    v.b = p + 2 * v.NC;
+    v.r = nullptr;
+    float *rows = p + 4 * v.NC;
+    v.ax = rows;
+    v.ay = rows + 3 * v.sxp;
+    v.az = rows + 3 * (v.sxp + v.sy);
+    if (l + 1 < S.n) {
+        MgAxis *t = reinterpret_cast<MgAxis *>(rows + ((3 * (v.sxp + v.sy + v.sz) + 3) & ~3));
+        v.tx = t;
+        v.ty = t + (v.nx + 2);
+        v.tz = t + (v.nx + 2) + v.sy;
+    }
+    return v;
 }
 
 // one colour half-sweep of red-black SOR over the whole level, then a barrier
@@ -1252,7 +1290,7 @@ __device__ void blk_sweep(const MgView &L, int c, bool first, float om) {
         if (!(rr == rr)) continue;
         int ii, jj, kk;
         view_decode(L, f, c, ii, jj, kk);
-        const int s = (c + jj + kk) & 1;
+        const int s = ii & 1;
         const float a1x = L.ax[ii], a2x = L.ax[L.sxp + ii], a3x = L.ax[2 * L.sxp + ii];
         const float a1y = L.ay[jj], a2y = L.ay[L.sy + jj], a3y = L.ay[2 * L.sy + jj];
         const float a1z = L.az[kk], a2z = L.az[L.sz + kk], a3z = L.az[2 * L.sz + kk];
@@ -1273,51 +1311,13 @@ __device__ void blk_sweep(const MgView &L, int c, bool first, float om) {
     __syncthreads();
 }
 
-__device__ void blk_resid(const MgView &L) {
-    const int plane = L.H * L.sy;
-    for (int q = threadIdx.x; q < 2 * L.NC; q += blockDim.x) {
-        const float rb = L.b[q];
-        if (!(rb == rb)) {
-            L.r[q] = rb;
-            continue;
-        }
-        const int c = q >= L.NC ? 1 : 0;
-        const int f = q - c * L.NC;
-        int ii, jj, kk;
-        view_decode(L, f, c, ii, jj, kk);
-        const int s = (c + jj + kk) & 1;
-        const float *o = L.x + (int64_t)(1 - c) * L.NC;
-        const float a1x = L.ax[ii], a2x = L.ax[L.sxp + ii], a3x = L.ax[2 * L.sxp + ii];
-        const float a1y = L.ay[jj], a2y = L.ay[L.sy + jj], a3y = L.ay[2 * L.sy + jj];
-        const float a1z = L.az[kk], a2z = L.az[L.sz + kk], a3z = L.az[2 * L.sz + kk];
-        const float ax = (((((a1x * o[f - 1 + s] + a3x * o[f + s]) + a1y * o[f - L.H]) + a3y * o[f + L.H]) +
-                           a1z * o[f - plane]) + a3z * o[f + plane]) + ((a2x + a2y) + a2z) * L.x[q];
-        L.r[q] = rb - ax;
-    }
-    __syncthreads();
-}
-
-__device__ void blk_restrict(const MgView &F, const MgView &C) {
+__device__ void blk_resid_restrict(const MgView &F, const MgView &C) {
     for (int q = threadIdx.x; q < 2 * C.NC; q += blockDim.x) {
         const int cc = q >= C.NC ? 1 : 0;
+        const int fc = q - cc * C.NC;
         int I, J, K;
-        view_decode(C, q - cc * C.NC, cc, I, J, K);
-        float out = __builtin_nanf("");
-        if (I >= 1 && I <= C.nx && J >= 1 && J <= C.ny && K >= 1 && K <= C.nz) {
-            float s = 0.0f;
-            int n = 0;
-            for (int t = 0; t < 8; t++) {
-                const int ii = 2 * I - 1 + (t & 1), jj = 2 * J - 1 + ((t >> 1) & 1), kk = 2 * K - 1 + (t >> 2);
-                const int c = (ii + jj + kk) & 1;
-                const float v = F.r[c * F.NC + (jj + F.sy * kk) * F.H + (ii >> 1)];
-                if (v == v) {
-                    s += v;
-                    n++;
-                }
-            }
-            if (n) out = s / (float)n;
-        }
-        C.b[q] = out;
+        view_decode(C, fc, cc, I, J, K);
+        C.b[q] = restrict_at<false>(F, C, cc, fc, I, J, K);
     }
     __syncthreads();
 }
@@ -1328,53 +1328,73 @@ __device__ void blk_prolong(const MgView &F, const MgView &C) {
         const int c = q >= F.NC ? 1 : 0;
         int ii, jj, kk;
         view_decode(F, q - c * F.NC, c, ii, jj, kk);
-        const MgAxis ax = F.tx[ii], ay = F.ty[jj], az = F.tz[kk];
-        const int I[2] = {ax.i0, ax.i1}, J[2] = {ay.i0, ay.i1}, K[2] = {az.i0, az.i1};
-        const float wx[2] = {ax.w0, ax.w1}, wy[2] = {ay.w0, ay.w1}, wz[2] = {az.w0, az.w1};
-        float s = 0.0f;
-        for (int t = 0; t < 8; t++) {
-            const int a = t & 1, b = (t >> 1) & 1, d = t >> 2;
-            const int cc = (I[a] + J[b] + K[d]) & 1;
-            s += (wx[a] * wy[b] * wz[d]) * C.x[cc * C.NC + (J[b] + C.sy * K[d]) * C.H + (I[a] >> 1)];
-        }
-        F.x[q] += s;
+        F.x[q] += prolong_at(F, C, ii, jj, kk);
     }
     __syncthreads();
 }
 
-// V-cycle over levels S.L[0..n) in one workgroup: x of level 0 from its b
+// V-cycle over levels S.L[0..n): x of level 0 from its b.  x and b of every
+// level live in dynamic LDS for the duration (2 x 2*NC floats per level);
+// S.L[0]'s global b is read at the start and its global x written at the end.
 __global__ void __launch_bounds__(kMgSmallThreads) k_mg_small(MgSmall S, const double *gate, double thr) {
     if (gated(gate, thr)) return;
-    for (int l = 0; l + 1 < S.n; l++) {
-        for (int s = 0; s < S.nu1; s++) {
-            blk_sweep(S.L[l], 0, s == 0, S.om);
-            blk_sweep(S.L[l], 1, s == 0, S.om);
+    extern __shared__ float lds[];
+    for (int l = 0; l < S.n; l++) {
+        const MgView V = lds_view(S, l, lds), G = S.L[l];
+        for (int q = threadIdx.x; q < 2 * V.NC; q += blockDim.x) {
+            V.x[q] = 0.0f;  // cells outside the solve hold 0
+            if (l == 0) V.b[q] = G.b[q];
         }
-        blk_resid(S.L[l]);
-        blk_restrict(S.L[l], S.L[l + 1]);
+        float *ax = const_cast<float *>(V.ax), *ay = const_cast<float *>(V.ay), *az = const_cast<float *>(V.az);
+        for (int q = threadIdx.x; q < 3 * V.sxp; q += blockDim.x) ax[q] = G.ax[q];
+        for (int q = threadIdx.x; q < 3 * V.sy; q += blockDim.x) ay[q] = G.ay[q];
+        for (int q = threadIdx.x; q < 3 * V.sz; q += blockDim.x) az[q] = G.az[q];
+        if (l + 1 < S.n) {
+            MgAxis *tx = const_cast<MgAxis *>(V.tx), *ty = const_cast<MgAxis *>(V.ty), *tz = const_cast<MgAxis *>(V.tz);
+            for (int q = threadIdx.x; q < V.nx + 2; q += blockDim.x) tx[q] = G.tx[q];
+            for (int q = threadIdx.x; q < V.sy; q += blockDim.x) ty[q] = G.ty[q];
+            for (int q = threadIdx.x; q < V.sz; q += blockDim.x) tz[q] = G.tz[q];
+        }
     }
-    for (int s = 0; s < S.ncoarse; s++) {
-        blk_sweep(S.L[S.n - 1], 0, s == 0, S.omc);
-        blk_sweep(S.L[S.n - 1], 1, s == 0, S.omc);
+    __syncthreads();
+    for (int l = 0; l + 1 < S.n; l++) {
+        const MgView F = lds_view(S, l, lds);
+        for (int s = 0; s < S.nu1; s++) {
+            blk_sweep(F, 0, s == 0, S.om);
+            blk_sweep(F, 1, s == 0, S.om);
+        }
+        blk_resid_restrict(F, lds_view(S, l + 1, lds));
+    }
+    {
+        const MgView C = lds_view(S, S.n - 1, lds);
+        for (int s = 0; s < S.ncoarse; s++) {
+            blk_sweep(C, 0, s == 0, S.omc);
+            blk_sweep(C, 1, s == 0, S.omc);
+        }
     }
     for (int l = S.n - 2; l >= 0; l--) {
-        blk_prolong(S.L[l], S.L[l + 1]);
+        const MgView F = lds_view(S, l, lds);
+        blk_prolong(F, lds_view(S, l + 1, lds));
         for (int s = 0; s < S.nu2; s++) {
-            blk_sweep(S.L[l], 0, false, S.om);
-            blk_sweep(S.L[l], 1, false, S.om);
+            blk_sweep(F, 0, false, S.om);
+            blk_sweep(F, 1, false, S.om);
         }
     }
+    const MgView V0 = lds_view(S, 0, lds);
+    for (int q = threadIdx.x; q < 2 * V0.NC; q += blockDim.x) S.L[0].x[q] = V0.x[q];
 }
 
 struct MgLevel {
     int64_t nx = 0, ny = 0, nz = 0, sx = 0, sy = 0, sz = 0, H = 0, NC = 0, sxp = 0;
     float *ax = nullptr, *ay = nullptr, *az = nullptr;  // Laplacian rows (level 0: the FP32 fine tables)
-    float *x = nullptr, *b = nullptr, *r = nullptr;     // correction, right-hand side, residual (2*NC each)
+    float *x = nullptr, *b = nullptr;                    // correction, right-hand side (2*NC each)
+    float *r = nullptr;                                  // residual (finest level only)
     MgAxis *tx = nullptr, *ty = nullptr, *tz = nullptr;   // prolongation from the next level (sx, sy, sz entries)
     bool owns_tables = false, owns_xb = false;
 };
 static std::vector<MgLevel> g_mg;
-static int g_mg_small = 0;  // first level cycled inside the single-workgroup kernel
+static int g_mg_small = 0;  // first level cycled inside the single-workgroup kernel (= size: none)
+static int g_mg_lds = 0;    // its dynamic LDS bytes
 static int64_t g_mg_key[4] = {0, 0, 0, 0};
 
 static SorGeom level_geom(const MgLevel &L) {
@@ -1395,8 +1415,8 @@ void mg_release() {
     for (MgLevel &L : g_mg) {
         if (L.owns_tables) { (void)hipFree(L.ax); (void)hipFree(L.ay); (void)hipFree(L.az); }
         if (L.owns_xb) { (void)hipFree(L.x); (void)hipFree(L.b); }
-        if (L.r) (void)hipFree(L.r);
         if (L.tx) { (void)hipFree(L.tx); (void)hipFree(L.ty); (void)hipFree(L.tz); }
+        if (L.r) (void)hipFree(L.r);
     }
     g_mg.clear();
     g_mg_key[0] = g_mg_key[1] = g_mg_key[2] = g_mg_key[3] = 0;
@@ -1513,21 +1533,34 @@ static int mg_setup(Ctx &c) {
     IB3D_REQUIRE(g_mg.size() >= 2, "poissonSolver: the grid cannot be coarsened for multigrid (each interior "
                                    "dimension must be even and at least 4)");
     const int nl = (int)g_mg.size();
-    // the levels below kMgSmallCells cells (and at most kMgSmallMax of them) go to one workgroup; the
-    // finest level never does
-    g_mg_small = nl - 1;
-    while (g_mg_small > 1 && nl - (g_mg_small - 1) <= kMgSmallMax && 2 * g_mg[g_mg_small - 1].NC <= kMgSmallCells)
+    // the coarsest levels whose x and b fit the LDS budget (at most kMgSmallMax of them) are cycled by
+    // one workgroup; the finest level never is
+    int lds_max = 0;
+    IB3D_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device));
+    const int64_t budget = std::min<int64_t>(lds_max, kMgSmallLds);
+    g_mg_small = nl;
+    int64_t bytes = 0;
+    while (g_mg_small > 1 && nl - (g_mg_small - 1) <= kMgSmallMax) {
+        const MgLevel &L = g_mg[g_mg_small - 1];
+        const int64_t b = 4 * (int64_t)mg_lds_floats((int)L.NC, (int)L.sx, (int)L.sy, (int)L.sz, (int)L.sxp,
+                                                     g_mg_small - 1 < nl - 1);
+        if (bytes + b > budget) break;
         g_mg_small--;
+        bytes += b;
+    }
+    g_mg_lds = (int)bytes;
     for (int l = 0; l < nl; l++) {
         MgLevel &L = g_mg[l];
         if (check_fastdiv((int)L.H, L.NC + 1) || check_fastdiv((int)L.sy, L.NC / L.H + 1)) {
             set_error("poissonSolver: fast division self-check failed on a multigrid level");
             return -1;
         }
-        if (l < nl - 1) {  // residual of this level feeds the next, whose correction comes back
+        if (l == 0) {  // the finest residual is stored by the z-marching kernel (rows it skips stay NaN)
             IB3D_CHECK(hipMalloc(&L.r, 2 * L.NC * sizeof(float)));
             hipLaunchKernelGGL(k_fill_nan_f32, dim3(nblocks(2 * L.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream,
                                L.r, 2 * L.NC);
+        }
+        if (l < nl - 1) {  // the next level's correction comes back through these
             const std::vector<MgAxis> tx = mg_axis(mg_faces(c.hx1, c.nx, l), mg_faces(c.hx1, c.nx, l + 1), true, L.sx);
             const std::vector<MgAxis> ty = mg_axis(mg_faces(c.hy1, c.ny, l), mg_faces(c.hy1, c.ny, l + 1), false, L.sy);
             const std::vector<MgAxis> tz = mg_axis(mg_faces(c.hz1, c.nz, l), mg_faces(c.hz1, c.nz, l + 1), false, L.sz);
@@ -1593,6 +1626,16 @@ static void mg_smooth(Ctx &c, const MgLevel &L, bool finest, int n, bool first, 
     }
 }
 
+static MgView level_view(const MgLevel &L) {
+    MgView V;
+    V.x = L.x; V.b = L.b; V.r = L.r;
+    V.ax = L.ax; V.ay = L.ay; V.az = L.az;
+    V.tx = L.tx; V.ty = L.ty; V.tz = L.tz;
+    V.nx = (int)L.nx; V.ny = (int)L.ny; V.nz = (int)L.nz;
+    V.H = (int)L.H; V.NC = (int)L.NC; V.sy = (int)L.sy; V.sz = (int)L.sz; V.sxp = (int)L.sxp;
+    return V;
+}
+
 // z = M^-1 v: one V-cycle from a zero guess (v and z are 2*NC fine vectors).
 // Levels from g_mg_small on are cycled by one single-workgroup launch.
 static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
@@ -1600,40 +1643,43 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
     g_mg[0].b = const_cast<float *>(v);
     g_mg[0].x = z;
     const dim3 tb(kSorBlock);
-    for (int l = 0; l < ns; l++) {
+    const int top = std::min(ns, nl - 1);  // levels [0, top) restrict through launches
+    for (int l = 0; l < top; l++) {
         const MgLevel &F = g_mg[l], &C = g_mg[l + 1];
-        const SorGeom gf = level_geom(F), gc = level_geom(C);
         mg_smooth(c, F, l == 0, c.mg_nu1, true, c.mg_omega, gate);
-        if (l == 0)
+        if (l == 0) {
             hipLaunchKernelGGL(k_matvec32_pz<true>, level_grid(F), tb, 0, c.stream, F.x, F.x + F.NC, F.b,
-                               F.b + F.NC, F.r, F.r + F.NC, F.ax, F.ay, F.az, gf, gate, 0.5);
-        else
-            hipLaunchKernelGGL(k_mg_resid_cell, dim3(nblocks(2 * F.NC, kSorBlock)), tb, 0, c.stream, F.x, F.b, F.r,
-                               F.ax, F.ay, F.az, gf, gate, 0.5);
-        hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks(2 * C.NC, kSorBlock)), tb, 0, c.stream, F.r, gf, C.b, gc,
-                           (int)C.nx, (int)C.ny, (int)C.nz, gate, 0.5);
+                               F.b + F.NC, F.r, F.r + F.NC, F.ax, F.ay, F.az, level_geom(F), gate, 0.5);
+            hipLaunchKernelGGL(k_mg_resid_restrict<true>, dim3(nblocks(2 * C.NC, kSorBlock)), tb, 0, c.stream,
+                               level_view(F), level_view(C), level_geom(C), gate, 0.5);
+        } else {
+            hipLaunchKernelGGL(k_mg_resid_restrict<false>, dim3(nblocks(2 * C.NC, kSorBlock)), tb, 0, c.stream,
+                               level_view(F), level_view(C), level_geom(C), gate, 0.5);
+        }
     }
-    MgSmall S;
-    S.n = nl - ns;
-    S.nu1 = c.mg_nu1;
-    S.nu2 = c.mg_nu2;
-    S.ncoarse = c.mg_coarse;
-    S.om = (float)c.mg_omega;
-    S.omc = (float)c.mg_coarse_omega;
-    for (int l = ns; l < nl; l++) {
-        const MgLevel &L = g_mg[l];
-        MgView &V = S.L[l - ns];
-        V.x = L.x; V.b = L.b; V.r = L.r;
-        V.ax = L.ax; V.ay = L.ay; V.az = L.az;
-        V.tx = L.tx; V.ty = L.ty; V.tz = L.tz;
-        V.nx = (int)L.nx; V.ny = (int)L.ny; V.nz = (int)L.nz;
-        V.H = (int)L.H; V.NC = (int)L.NC; V.sy = (int)L.sy; V.sz = (int)L.sz; V.sxp = (int)L.sxp;
+    if (ns < nl) {
+        MgSmall S;
+        S.n = nl - ns;
+        S.nu1 = c.mg_nu1;
+        S.nu2 = c.mg_nu2;
+        S.ncoarse = c.mg_coarse;
+        S.om = (float)c.mg_omega;
+        S.omc = (float)c.mg_coarse_omega;
+        int off = 0;
+        for (int l = ns; l < nl; l++) {
+            const MgLevel &L = g_mg[l];
+            S.L[l - ns] = level_view(L);
+            S.off[l - ns] = off;
+            off += mg_lds_floats((int)L.NC, (int)L.sx, (int)L.sy, (int)L.sz, (int)L.sxp, l < nl - 1);
+        }
+        hipLaunchKernelGGL(k_mg_small, dim3(1), dim3(kMgSmallThreads), g_mg_lds, c.stream, S, gate, 0.5);
+    } else {
+        mg_smooth(c, g_mg[nl - 1], false, c.mg_coarse, true, c.mg_coarse_omega, gate);
     }
-    hipLaunchKernelGGL(k_mg_small, dim3(1), dim3(kMgSmallThreads), 0, c.stream, S, gate, 0.5);
-    for (int l = ns - 1; l >= 0; l--) {
+    for (int l = top - 1; l >= 0; l--) {
         const MgLevel &F = g_mg[l], &C = g_mg[l + 1];
-        hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(2 * F.NC, kSorBlock)), tb, 0, c.stream, F.x, F.b,
-                           level_geom(F), C.x, level_geom(C), F.tx, F.ty, F.tz, gate, 0.5);
+        hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(2 * F.NC, kSorBlock)), tb, 0, c.stream, level_view(F),
+                           level_view(C), level_geom(F), gate, 0.5);
         mg_smooth(c, F, l == 0, c.mg_nu2, false, c.mg_omega, gate);
     }
 }

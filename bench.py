@@ -21,10 +21,11 @@ Poisson modes (same stopping rule: ||b/dt - A p||_2 <= 1e-6 ||b/dt||_2):
                    flexible GMRES(12), every Arnoldi step preconditioned by
                    20 FP32 red-black SOR sweeps at omega 1.9 (GMRES-IR),
                    previous-step guess
-  mixed         -- the same FP64 refinement and FP32 flexible GMRES(12),
+  mixed         -- the same FP64 refinement and FP32 flexible GMRES(6),
                    every Arnoldi step preconditioned by one FP32 geometric
                    multigrid V-cycle (red-black SOR smoothing, 2+2 sweeps per
-                   level, trilinear prolongation), previous-step guess
+                   level at omega 1.55, trilinear prolongation),
+                   previous-step guess
   mixed+learned -- mixed, starting from the CNN's guess when it has the
                    smaller residual (headline)
 The headline `value` is steps/s of --mode; the --compare modes are measured
@@ -72,7 +73,7 @@ GMRES = {"m": 12, "k": 20, "omega": 1.9}
 # the multigrid preconditioner: Arnoldi steps per refinement step, pre- and
 # post-smoothing sweeps per level and their omega, sweeps and omega on the
 # coarsest level
-MG = {"m": 12, "nu1": 2, "nu2": 2, "omega": 1.0, "coarse": 16, "coarse_omega": 1.5}
+MG = {"m": 6, "nu1": 2, "nu2": 2, "omega": 1.55, "coarse": 8, "coarse_omega": 1.5}
 
 
 class Motion:

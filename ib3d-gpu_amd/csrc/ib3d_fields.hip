@@ -280,16 +280,17 @@ k_correct(double *__restrict__ p, double *__restrict__ ut, double *__restrict__ 
 __global__ void __launch_bounds__(256)
 k_steady_copy(double *__restrict__ u, double *__restrict__ v, double *__restrict__ w, const double *__restrict__ ut,
               const double *__restrict__ vt, const double *__restrict__ wt, const int8_t *__restrict__ cell, int nx,
-              int ny, int nz, double dt, double *out) {
-    const int64_t P = (int64_t)(nx + 2) * (ny + 2) * (nz + 2);
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+              int ny, int nz, double dt, double *partial) {
+    const int P = (nx + 2) * (ny + 2) * (nz + 2);  // < 2^31
+    const int c = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
     double m = 0.0;
     if (c < P) {
         const double a = ut[c], b = vt[c], d = wt[c];
         if (cell[c] == 0) {
-            const int ii = (int)(c % (nx + 2));
-            const int64_t row = c / (nx + 2);
-            const int jj = (int)(row % (ny + 2)), kk = (int)(row / (ny + 2));
+            const int row = c / (nx + 2);
+            const int ii = c - row * (nx + 2);
+            const int kk = row / (ny + 2);
+            const int jj = row - kk * (ny + 2);
             if (ii >= 1 && ii <= nx && jj >= 1 && jj <= ny && kk >= 1 && kk <= nz) {
                 const double du = fabs(a - u[c]) / dt, dv = fabs(b - v[c]) / dt, dw = fabs(d - w[c]) / dt;
                 m = fmax(fmax(du, dv), dw);
@@ -300,15 +301,7 @@ k_steady_copy(double *__restrict__ u, double *__restrict__ v, double *__restrict
         v[c] = b;
         w[c] = d;
     }
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    __shared__ double sm[4];
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
-        if (r > 0.0 || r != r)
-            atomicMax(reinterpret_cast<unsigned long long *>(out), (unsigned long long)__double_as_longlong(r));
-    }
+    block_max_partial(m, partial);
 }
 
 // ------------------------------------------------------------------ host
@@ -359,9 +352,11 @@ int correct(Ctx &c, double dt, int do_p, int do_v) {
 }
 
 int steady_copy(Ctx &c, double dt, double *derr_std) {
-    IB3D_CHECK(hipMemsetAsync(c.scalars + 3, 0, sizeof(double), c.stream));
-    hipLaunchKernelGGL(k_steady_copy, dim3(nblocks(c.P, 256)), dim3(256), 0, c.stream, c.u, c.v, c.w, c.ut, c.vt,
-                       c.wt, c.tag, (int)c.nx, (int)c.ny, (int)c.nz, dt, c.scalars + 3);
+    const unsigned nb = nblocks(c.P, 256);
+    if (ensure_partial(nb)) return -1;
+    hipLaunchKernelGGL(k_steady_copy, dim3(nb), dim3(256), 0, c.stream, c.u, c.v, c.w, c.ut, c.vt,
+                       c.wt, c.tag, (int)c.nx, (int)c.ny, (int)c.nz, dt, c.partial);
+    launch_max_partials(c.partial, (int64_t)nb, c.scalars + 3);
     IB3D_CHECK(hipGetLastError());
     IB3D_CHECK(hipMemcpyAsync(c.hscal + 3, c.scalars + 3, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     IB3D_CHECK(hipStreamSynchronize(c.stream));

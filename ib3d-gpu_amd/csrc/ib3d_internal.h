@@ -172,6 +172,22 @@ Ctx &ctx();
 // launch helpers
 inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// max that keeps a NaN operand (fmax would drop it)
+__device__ inline double nan_max(double a, double b) { return (a != a || b != b) ? __builtin_nan("") : fmax(a, b); }
+
+// max over a 256-thread workgroup into partial[blockIdx.x] (no atomics: one
+// contended address per launch serialises tens of thousands of workgroups)
+__device__ inline void block_max_partial(double v, double *partial) {
+    for (int off = 32; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off, 64));
+    __shared__ double sm_max[4];
+    if ((threadIdx.x & 63) == 0) sm_max[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = nan_max(nan_max(sm_max[0], sm_max[1]), nan_max(sm_max[2], sm_max[3]));
+}
+// *out = max of partial[0..n) (NaN if any is NaN), on the library stream
+void launch_max_partials(const double *partial, int64_t n, double *out);
+
 // Every transfer and fill goes through the library's stream, which is
 // created non-blocking and therefore not ordered with the null stream: a
 // blocking hipMemcpy/hipMemset would race with kernels queued on it.  These

@@ -102,6 +102,23 @@ __device__ __forceinline__ void block_sum_to(double v, double *partial, int64_t 
 // skip this launch when the previous iteration already converged
 __device__ __forceinline__ bool gated(const double *gate, double thr) { return gate && *gate < thr; }
 
+__global__ void k_max_partials(const double *partial, int64_t n, double *out) {
+    __shared__ double sm[1024];
+    double m = 0.0;
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) m = nan_max(m, partial[q]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] = nan_max(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sm[0];
+}
+
+void launch_max_partials(const double *partial, int64_t n, double *out) {
+    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(1024), 0, ctx().stream, partial, n, out);
+}
+
 __global__ void k_sum_partials(const double *partial, int64_t n, double *out) {
     __shared__ double sm[1024];
     double s = 0.0;
@@ -498,21 +515,23 @@ k_add_e(double *p0, double *p1, const float *e0, const float *e1, const double *
 }
 
 // subPcorVcor.f90:89-99 computeDiv, writing b/dt into the colour arrays
-// (NaN elsewhere), divmax via atomicMax and partial sums of (b/dt)^2.
+// (NaN elsewhere), per-block partial maxima of |b| (pmax) and partial sums
+// of (b/dt)^2 (psum).  P < 2^31, so the index splits in 32 bits.
 __global__ void __launch_bounds__(kSorBlock)
 k_div(const double *__restrict__ ut, const double *__restrict__ vt, const double *__restrict__ wt,
       const int8_t *__restrict__ tag, const double *__restrict__ dx, const double *__restrict__ dy,
       const double *__restrict__ dz, double *__restrict__ b0, double *__restrict__ b1, int sx, int sy,
-      int sz, int64_t H, double deltat, double *divmax, double *partial) {
-    const int64_t P = (int64_t)sx * sy * sz;
-    const int64_t idx = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
+      int sz, int H, double deltat, double *pmax, double *psum) {
+    const int P = sx * sy * sz;
+    const int idx = (int)blockIdx.x * kSorBlock + (int)threadIdx.x;
     double am = 0.0, bb = 0.0;
     if (idx < P) {
-        const int ii = (int)(idx % sx);
-        const int64_t row = idx / sx;
-        const int jj = (int)(row % sy), kk = (int)(row / sy);
+        const int row = idx / sx;
+        const int ii = idx - row * sx;
+        const int kk = row / sy;
+        const int jj = row - kk * sy;
         const int colour = (ii + jj + kk) & 1;
-        const int64_t f = row * H + (ii >> 1);
+        const int64_t f = (int64_t)row * H + (ii >> 1);
         double val = __builtin_nan("");
         const bool interior = ii >= 1 && ii <= sx - 2 && jj >= 1 && jj <= sy - 2 && kk >= 1 && kk <= sz - 2;
         if (interior && tag[idx] == 0) {
@@ -525,8 +544,8 @@ k_div(const double *__restrict__ ut, const double *__restrict__ vt, const double
         }
         (colour ? b1 : b0)[f] = val;
     }
-    block_max_to(am, divmax);
-    block_sum_to(bb, partial, blockIdx.x);
+    block_max_partial(am, pmax);
+    block_sum_to(bb, psum, blockIdx.x);
 }
 
 // plain streaming copy of known size: the reference point for converting
@@ -893,11 +912,12 @@ void sor_release_tables() {
 
 int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2) {
     const unsigned nb = nblocks(c.P, kSorBlock);
-    if (ensure_partial(std::max<int64_t>(nb, nblocks(2 * c.NC, kSorBlock)))) return -1;
+    if (ensure_partial(std::max<int64_t>(2 * (int64_t)nb, nblocks(2 * c.NC, kSorBlock)))) return -1;
     IB3D_CHECK(hipMemsetAsync(c.scalars, 0, 8 * sizeof(double), c.stream));
     hipLaunchKernelGGL(k_div, dim3(nb), dim3(kSorBlock), 0, c.stream, c.ut, c.vt, c.wt, c.tag, c.dx, c.dy,
-                       c.dz, c.bdt[0], c.bdt[1], (int)c.sx, (int)c.sy, (int)c.sz, c.H, deltat, c.scalars,
+                       c.dz, c.bdt[0], c.bdt[1], (int)c.sx, (int)c.sy, (int)c.sz, (int)c.H, deltat, c.partial + nb,
                        c.partial);
+    launch_max_partials(c.partial + nb, (int64_t)nb, c.scalars);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, c.stream, c.partial, (int64_t)nb, c.scalars + 1);
     IB3D_CHECK(hipGetLastError());
     IB3D_CHECK(hipMemcpyAsync(c.hscal, c.scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));

@@ -108,7 +108,9 @@ static void free_all(Ctx &c) {
     c.e32[1] = nullptr;
     c.r32[1] = nullptr;
     dfree(c.tag); dfree(c.minElem); dfree(c.elp);
-    dfree(c.ibList); dfree(c.fluidList); dfree(c.solidList); dfree(c.nel); dfree(c.scell); dfree(c.scontrib);
+    dfree(c.ibList); dfree(c.fluidList); dfree(c.solidList);
+    c.cap_lists[0] = c.cap_lists[1] = c.cap_lists[2] = 0;
+    c.cap_nel = 0; dfree(c.nel); dfree(c.scell); dfree(c.scontrib);
     dfree(c.rt_flag); dfree(c.rt_list); dfree(c.rt_count); dfree(c.body);
     release_plans(c);
     if (c.hscal) (void)hipHostFree(c.hscal);

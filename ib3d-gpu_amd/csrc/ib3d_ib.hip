@@ -252,29 +252,31 @@ k_retag_list(const int *list, const int *count, int sx, int sy, const double *__
 }
 
 // per interior row (j,k): counts of fluid, solid, intercepted, red-fluid
-__global__ void k_row_counts(const int8_t *tag, int nx, int ny, int nz, int *counts) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// Per-row counts of fluid (tag 0), solid (1), intercepted (2) and red
+// fluid cells; one wave per row, 64 consecutive cells per ballot.
+__global__ void __launch_bounds__(256) k_row_counts(const int8_t *tag, int nx, int ny, int nz, int *counts) {
+    const int lane = threadIdx.x & 63;
+    const int r = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     const int nrows = ny * nz;
-    if (r >= nrows) return;
+    if (r >= nrows) return;  // uniform per wave
     const int j = 2 + r % ny, k = 2 + r / ny;
     const int sx = nx + 2, sy = ny + 2;
     const int8_t *row = tag + lin(1, j, k, sx, sy);
     int f = 0, s = 0, b = 0, red = 0;
-    for (int i = 2; i <= nx + 1; i++) {
-        const int8_t t = row[i - 1];
-        if (t == 0) {
-            f++;
-            red += (i + j + k) & 1;
-        } else if (t == 1) {
-            s++;
-        } else if (t == 2) {
-            b++;
-        }
+    for (int i0 = 2; i0 <= nx + 1; i0 += 64) {
+        const int i = i0 + lane;
+        const int t = i <= nx + 1 ? (int)row[i - 1] : -1;
+        f += __popcll(__ballot(t == 0));
+        s += __popcll(__ballot(t == 1));
+        b += __popcll(__ballot(t == 2));
+        red += __popcll(__ballot(t == 0 && ((i + j + k) & 1)));
     }
-    counts[r] = f;
-    counts[nrows + r] = s;
-    counts[2 * nrows + r] = b;
-    counts[3 * nrows + r] = red;
+    if (lane == 0) {
+        counts[r] = f;
+        counts[nrows + r] = s;
+        counts[2 * nrows + r] = b;
+        counts[3 * nrows + r] = red;
+    }
 }
 
 // exclusive scan of the 4 count arrays (one block, in place), totals at the end
@@ -308,25 +310,36 @@ __global__ void k_scan_counts(int *counts, int nrows, int *totals) {
     }
 }
 
-// order-preserving scatter into (count,3) lists, (k,j,i) order
-__global__ void k_row_scatter(const int8_t *tag, int nx, int ny, int nz, const int *offs, int *fl, int nf, int *sl,
-                              int ns, int *bl, int nb) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// order-preserving scatter into (count,3) lists, (k,j,i) order; one wave
+// per row, a cell's slot = row offset + set ballot bits below its lane
+__global__ void __launch_bounds__(256) k_row_scatter(const int8_t *tag, int nx, int ny, int nz, const int *offs,
+                                                     int *fl, int nf, int *sl, int ns, int *bl, int nb) {
+    const int lane = threadIdx.x & 63;
+    const int r = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     const int nrows = ny * nz;
-    if (r >= nrows) return;
+    if (r >= nrows) return;  // uniform per wave
     const int j = 2 + r % ny, k = 2 + r / ny;
     const int sx = nx + 2, sy = ny + 2;
     const int8_t *row = tag + lin(1, j, k, sx, sy);
+    const unsigned long long below = (1ull << lane) - 1ull;
     int f = offs[r], s = offs[nrows + r], b = offs[2 * nrows + r];
-    for (int i = 2; i <= nx + 1; i++) {
-        const int8_t t = row[i - 1];
+    for (int i0 = 2; i0 <= nx + 1; i0 += 64) {
+        const int i = i0 + lane;
+        const int t = i <= nx + 1 ? (int)row[i - 1] : -1;
+        const unsigned long long mf = __ballot(t == 0), ms = __ballot(t == 1), mb = __ballot(t == 2);
         if (t == 0) {
-            fl[f] = i; fl[nf + f] = j; fl[2 * nf + f] = k; f++;
+            const int q = f + __popcll(mf & below);
+            fl[q] = i; fl[nf + q] = j; fl[2 * nf + q] = k;
         } else if (t == 1) {
-            sl[s] = i; sl[ns + s] = j; sl[2 * ns + s] = k; s++;
+            const int q = s + __popcll(ms & below);
+            sl[q] = i; sl[ns + q] = j; sl[2 * ns + q] = k;
         } else if (t == 2) {
-            bl[b] = i; bl[nb + b] = j; bl[2 * nb + b] = k; b++;
+            const int q = b + __popcll(mb & below);
+            bl[q] = i; bl[nb + q] = j; bl[2 * nb + q] = k;
         }
+        f += __popcll(mf);
+        s += __popcll(ms);
+        b += __popcll(mb);
     }
 }
 
@@ -726,9 +739,11 @@ static int dev_err(Ctx &c) {
     IB3D_CHECK(hipMemsetAsync(g_err, 0, sizeof(int), c.stream));
     return 0;
 }
+void count_tags_release();
 void ib_release() {
     if (g_err) (void)hipFree(g_err);
     g_err = nullptr;
+    count_tags_release();
 }
 
 int count_tags(Ctx &c, int32_t out[4], bool scatter);
@@ -800,12 +815,38 @@ int tagging(Ctx &c, const int64_t w[6], int32_t counts[3]) {
 }
 
 // counts of fluid, solid, intercepted, red-fluid; optionally build the lists
+static int *g_cnt = nullptr, *g_tot = nullptr;  // per-row counts (4 x rows) and totals, reused
+static int g_cnt_rows = 0;
+
+void count_tags_release() {
+    if (g_cnt) (void)hipFree(g_cnt);
+    if (g_tot) (void)hipFree(g_tot);
+    g_cnt = g_tot = nullptr;
+    g_cnt_rows = 0;
+}
+
+// grow-only device list of 3*n ints
+static int grow_list(int *&p, int64_t &cap, int64_t n) {
+    n = std::max<int64_t>(1, n);
+    if (n <= cap) return 0;
+    if (p) IB3D_CHECK(hipFree(p));
+    p = nullptr;
+    const int64_t c2 = n + n / 8;
+    IB3D_CHECK(hipMalloc(&p, 3 * c2 * sizeof(int)));
+    cap = c2;
+    return 0;
+}
+
 int count_tags(Ctx &c, int32_t out[4], bool scatter) {
     const int nrows = (int)(c.ny * c.nz);
-    int *cnt = nullptr, *tot = nullptr;
-    IB3D_CHECK(hipMalloc(&cnt, 4 * (size_t)nrows * sizeof(int)));
-    IB3D_CHECK(hipMalloc(&tot, 4 * sizeof(int)));
-    hipLaunchKernelGGL(k_row_counts, dim3(nblocks(nrows, 256)), dim3(256), 0, c.stream, c.tag, (int)c.nx, (int)c.ny,
+    if (nrows > g_cnt_rows) {
+        if (g_cnt) IB3D_CHECK(hipFree(g_cnt));
+        if (!g_tot) IB3D_CHECK(hipMalloc(&g_tot, 4 * sizeof(int)));
+        IB3D_CHECK(hipMalloc(&g_cnt, 4 * (size_t)nrows * sizeof(int)));
+        g_cnt_rows = nrows;
+    }
+    int *cnt = g_cnt, *tot = g_tot;
+    hipLaunchKernelGGL(k_row_counts, dim3(nblocks(nrows, 4)), dim3(256), 0, c.stream, c.tag, (int)c.nx, (int)c.ny,
                        (int)c.nz, cnt);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, cnt, nrows, tot);
     IB3D_CHECK(hipGetLastError());
@@ -814,26 +855,20 @@ int count_tags(Ctx &c, int32_t out[4], bool scatter) {
     IB3D_CHECK(hipStreamSynchronize(c.stream));
     for (int a = 0; a < 4; a++) out[a] = h[a];
     if (scatter) {
-        if (c.fluidList) IB3D_CHECK(hipFree(c.fluidList));
-        if (c.solidList) IB3D_CHECK(hipFree(c.solidList));
-        if (c.ibList) IB3D_CHECK(hipFree(c.ibList));
-        c.fluidList = c.solidList = c.ibList = nullptr;
         c.fluidCount = h[0];
         c.solidCount = h[1];
         c.ibCount = h[2];
         c.redCount = h[3];
         c.blackCount = h[0] - h[3];
-        IB3D_CHECK(hipMalloc(&c.fluidList, 3 * std::max<int64_t>(1, c.fluidCount) * sizeof(int)));
-        IB3D_CHECK(hipMalloc(&c.solidList, 3 * std::max<int64_t>(1, c.solidCount) * sizeof(int)));
-        IB3D_CHECK(hipMalloc(&c.ibList, 3 * std::max<int64_t>(1, c.ibCount) * sizeof(int)));
-        hipLaunchKernelGGL(k_row_scatter, dim3(nblocks(nrows, 256)), dim3(256), 0, c.stream, c.tag, (int)c.nx,
+        if (grow_list(c.fluidList, c.cap_lists[0], c.fluidCount) ||
+            grow_list(c.solidList, c.cap_lists[1], c.solidCount) || grow_list(c.ibList, c.cap_lists[2], c.ibCount))
+            return -1;
+        hipLaunchKernelGGL(k_row_scatter, dim3(nblocks(nrows, 4)), dim3(256), 0, c.stream, c.tag, (int)c.nx,
                            (int)c.ny, (int)c.nz, cnt, c.fluidList, (int)c.fluidCount, c.solidList, (int)c.solidCount,
                            c.ibList, (int)c.ibCount);
         IB3D_CHECK(hipGetLastError());
         IB3D_CHECK(hipStreamSynchronize(c.stream));
     }
-    IB3D_CHECK(hipFree(cnt));
-    IB3D_CHECK(hipFree(tot));
     return 0;
 }
 
@@ -853,11 +888,17 @@ int cell_count(Ctx &c, int32_t counts[5]) {
 
 int norm_distance(Ctx &c) {
     IB3D_REQUIRE(c.lists_ready, "computeNormDistance: cellCount has not run");
-    if (c.nel) IB3D_CHECK(hipFree(c.nel));
-    if (c.normdis) IB3D_CHECK(hipFree(c.normdis));
     const int64_t nb = std::max<int64_t>(1, c.ibCount);
-    IB3D_CHECK(hipMalloc(&c.nel, nb * sizeof(int)));
-    IB3D_CHECK(hipMalloc(&c.normdis, 7 * nb * sizeof(double)));
+    if (nb > c.cap_nel) {
+        if (c.nel) IB3D_CHECK(hipFree(c.nel));
+        if (c.normdis) IB3D_CHECK(hipFree(c.normdis));
+        c.nel = nullptr;
+        c.normdis = nullptr;
+        const int64_t cap = nb + nb / 8;
+        IB3D_CHECK(hipMalloc(&c.nel, cap * sizeof(int)));
+        IB3D_CHECK(hipMalloc(&c.normdis, 7 * cap * sizeof(double)));
+        c.cap_nel = cap;
+    }
     if (dev_err(c)) return -1;
     if (c.ibCount > 0)
         hipLaunchKernelGGL(k_norm_distance, dim3(nblocks(c.ibCount, 256)), dim3(256), 0, c.stream, c.ibList,
@@ -879,20 +920,67 @@ static void free_plan(ForcePlan &p) {
     p = ForcePlan();
 }
 
+// Host staging for the plan builder: pinned, reused across rebuilds (a
+// moving body rebuilds the plans every step).
+static int *g_doi = nullptr;        // device: stencil records of the 7 updates per IB cell
+static double *g_dod = nullptr;
+static int64_t g_dcap = 0;
+static void *g_pin = nullptr;       // pinned host staging
+static size_t g_pin_cap = 0;
+
+static int pinned(size_t bytes) {
+    if (bytes <= g_pin_cap) return 0;
+    if (g_pin) IB3D_CHECK(hipHostFree(g_pin));
+    g_pin = nullptr;
+    IB3D_CHECK(hipHostMalloc(&g_pin, bytes, hipHostMallocDefault));
+    g_pin_cap = bytes;
+    return 0;
+}
+
 void release_plans(Ctx &c) {
     free_plan(c.vplan);
     free_plan(c.pplan);
+    if (g_doi) (void)hipFree(g_doi);
+    if (g_dod) (void)hipFree(g_dod);
+    g_doi = nullptr;
+    g_dod = nullptr;
+    g_dcap = 0;
+    if (g_pin) (void)hipHostFree(g_pin);
+    g_pin = nullptr;
+    g_pin_cap = 0;
 }
 
-// final-writer table is stored as [loc | upd | comp] in final_loc (3*n ints)
+// Upload a plan: the updates in level order (8 stencil ints, 8 sources, 5
+// doubles each) and the final-writer table [loc | upd | comp] (3*n_final
+// ints).  Device buffers grow only; all arrays go through one pinned staging
+// area and are copied with a single wait.
 static int upload_plan(Ctx &c, ForcePlan &P, const std::vector<int> &order, const std::vector<int> &oi,
                        const std::vector<double> &od, const std::vector<int> &src, const std::vector<int> &level,
                        const std::vector<std::pair<int, int>> &finals, const std::vector<int> &fcomp) {
     const int n = (int)order.size();
     std::vector<int> pos_of(oi.size() / 8, -1);
     for (int q = 0; q < n; q++) pos_of[order[q]] = q;
-    std::vector<int> soi(8 * (size_t)std::max(1, n)), ssrc(8 * (size_t)std::max(1, n));
-    std::vector<double> sod(5 * (size_t)std::max(1, n));
+    const int m = std::max(1, n), nf = std::max(1, (int)finals.size());
+    if (m > P.cap_upd) {
+        if (P.upd) { (void)hipFree(P.upd); (void)hipFree(P.srcs); (void)hipFree(P.pos); (void)hipFree(P.res); }
+        const int cap = m + m / 4;
+        IB3D_CHECK(hipMalloc(&P.upd, 8 * (size_t)cap * sizeof(int)));
+        IB3D_CHECK(hipMalloc(&P.srcs, 8 * (size_t)cap * sizeof(int)));
+        IB3D_CHECK(hipMalloc(&P.pos, 5 * (size_t)cap * sizeof(double)));
+        IB3D_CHECK(hipMalloc(&P.res, (size_t)cap * sizeof(double)));
+        P.cap_upd = cap;
+    }
+    if (3 * nf > P.cap_final) {
+        if (P.final_loc) (void)hipFree(P.final_loc);
+        const int cap = 3 * nf + 3 * nf / 4;
+        IB3D_CHECK(hipMalloc(&P.final_loc, (size_t)cap * sizeof(int)));
+        P.cap_final = cap;
+    }
+    const size_t b_oi = 8 * (size_t)m * sizeof(int), b_od = 5 * (size_t)m * sizeof(double),
+                 b_fin = 3 * (size_t)nf * sizeof(int);
+    if (pinned(b_od + 2 * b_oi + b_fin)) return -1;
+    double *sod = static_cast<double *>(g_pin);
+    int *soi = reinterpret_cast<int *>(sod + 5 * (size_t)m), *ssrc = soi + 8 * (size_t)m, *fin = ssrc + 8 * (size_t)m;
     int maxl = -1;
     for (int q = 0; q < n; q++) {
         const int u = order[q];
@@ -909,31 +997,35 @@ static int upload_plan(Ctx &c, ForcePlan &P, const std::vector<int> &order, cons
     P.level_start.assign(P.n_levels + 1, 0);
     for (int q = 0; q < n; q++) P.level_start[level[order[q]] + 1]++;
     for (int l = 0; l < P.n_levels; l++) P.level_start[l + 1] += P.level_start[l];
-    const size_t m = std::max(1, n);
-    IB3D_CHECK(hipMalloc(&P.upd, 8 * m * sizeof(int)));
-    IB3D_CHECK(hipMalloc(&P.srcs, 8 * m * sizeof(int)));
-    IB3D_CHECK(hipMalloc(&P.pos, 5 * m * sizeof(double)));
-    IB3D_CHECK(hipMalloc(&P.res, m * sizeof(double)));
-    IB3D_CHECK(h2d(P.upd, soi.data(), 8 * m * sizeof(int)));
-    IB3D_CHECK(h2d(P.srcs, ssrc.data(), 8 * m * sizeof(int)));
-    IB3D_CHECK(h2d(P.pos, sod.data(), 5 * m * sizeof(double)));
     P.n_final = (int)finals.size();
-    std::vector<int> fin(3 * (size_t)std::max(1, P.n_final));
     for (int q = 0; q < P.n_final; q++) {
         fin[q] = finals[q].first;
         fin[P.n_final + q] = pos_of[finals[q].second];
         fin[2 * P.n_final + q] = fcomp[q];
     }
-    IB3D_CHECK(hipMalloc(&P.final_loc, fin.size() * sizeof(int)));
-    IB3D_CHECK(h2d(P.final_loc, fin.data(), fin.size() * sizeof(int)));
+    IB3D_CHECK(hipMemcpyAsync(P.upd, soi, b_oi, hipMemcpyHostToDevice, c.stream));
+    IB3D_CHECK(hipMemcpyAsync(P.srcs, ssrc, b_oi, hipMemcpyHostToDevice, c.stream));
+    IB3D_CHECK(hipMemcpyAsync(P.pos, sod, b_od, hipMemcpyHostToDevice, c.stream));
+    IB3D_CHECK(hipMemcpyAsync(P.final_loc, fin, b_fin, hipMemcpyHostToDevice, c.stream));
+    IB3D_CHECK(hipStreamSynchronize(c.stream));  // the staging area is reused by the next upload
     return 0;
 }
 
-// Dependency levels of the serial loop over `ids` (in serial order).
+// Dependency levels of the serial loop over `ids` (in serial order).  The
+// latest writer of each (location, component) lives in an open-addressing
+// table (linear probing, keys loc*4+comp, -1 = empty).
 static void levels(const std::vector<int> &ids, const std::vector<int> &oi, int sx, int sy, std::vector<int> &src,
                    std::vector<int> &level, std::vector<std::pair<int, int>> &finals, std::vector<int> &fcomp) {
-    std::unordered_map<int64_t, int> last;  // (comp, loc) -> latest writer
-    last.reserve(ids.size() * 2);
+    size_t cap = 16;
+    while (cap < 4 * ids.size()) cap <<= 1;
+    const size_t mask = cap - 1;
+    std::vector<int64_t> keys(cap, -1);
+    std::vector<int> vals(cap, -1);
+    auto slot = [&](int64_t key) {
+        size_t h = (size_t)((uint64_t)key * 0x9E3779B97F4A7C15ull >> 20) & mask;
+        while (keys[h] != -1 && keys[h] != key) h = (h + 1) & mask;
+        return h;
+    };
     for (int u : ids) {
         const int *o = &oi[8 * (size_t)u];
         const int comp = o[0], ix = o[2], iy = o[3], iz = o[4];
@@ -942,18 +1034,22 @@ static void levels(const std::vector<int> &ids, const std::vector<int> &oi, int 
         for (int t = 0; t < 8; t++) {
             const int64_t loc = (int64_t)(ax + (t & 1) - 1) + (int64_t)sx * ((ay + ((t >> 1) & 1) - 1) +
                                                                              (int64_t)sy * (az + (t >> 2) - 1));
-            const int64_t key = loc * 4 + comp;
-            auto it = last.find(key);
-            const int s = it == last.end() ? -1 : it->second;
+            const size_t h = slot(loc * 4 + comp);
+            const int s = keys[h] == -1 ? -1 : vals[h];
             src[8 * (size_t)u + t] = s;
             if (s >= 0) lv = std::max(lv, level[s] + 1);
         }
         level[u] = lv;
-        last[(int64_t)o[1] * 4 + comp] = u;
+        const int64_t wkey = (int64_t)o[1] * 4 + comp;
+        const size_t h = slot(wkey);
+        keys[h] = wkey;
+        vals[h] = u;
     }
     finals.clear();
     fcomp.clear();
-    std::vector<std::pair<int64_t, int>> tmp(last.begin(), last.end());
+    std::vector<std::pair<int64_t, int>> tmp;
+    for (size_t h = 0; h < cap; h++)
+        if (keys[h] != -1) tmp.push_back({keys[h], vals[h]});
     std::sort(tmp.begin(), tmp.end());
     for (auto &kv : tmp) {
         finals.push_back({(int)(kv.first / 4), kv.second});
@@ -962,40 +1058,53 @@ static void levels(const std::vector<int> &ids, const std::vector<int> &oi, int 
 }
 
 int build_force_plans(Ctx &c) {
-    release_plans(c);
     c.plans_ready = false;
+    c.vplan.n_updates = c.pplan.n_updates = 0;
+    c.vplan.n_final = c.pplan.n_final = 0;
     const int nb = (int)c.ibCount;
     if (nb == 0) {
+        c.vplan.level_start.assign(1, 0);
+        c.vplan.n_levels = 0;
+        c.pplan.level_start.assign(1, 0);
+        c.pplan.n_levels = 0;
         c.plans_ready = true;
         return 0;
     }
-    int *doi = nullptr;
-    double *dod = nullptr;
-    IB3D_CHECK(hipMalloc(&doi, 8 * 7 * (size_t)nb * sizeof(int)));
-    IB3D_CHECK(hipMalloc(&dod, 5 * 7 * (size_t)nb * sizeof(double)));
+    const int64_t nu7 = 7 * (int64_t)nb;
+    if (nu7 > g_dcap) {
+        if (g_doi) (void)hipFree(g_doi);
+        if (g_dod) (void)hipFree(g_dod);
+        const int64_t cap = nu7 + nu7 / 4;
+        IB3D_CHECK(hipMalloc(&g_doi, 8 * (size_t)cap * sizeof(int)));
+        IB3D_CHECK(hipMalloc(&g_dod, 5 * (size_t)cap * sizeof(double)));
+        g_dcap = cap;
+    }
     if (dev_err(c)) return -1;
     StencilArgs A;
     A.ib = c.ibList; A.nb = nb; A.nel = c.nel; A.nd = c.normdis;
     A.dx = c.dx; A.dy = c.dy; A.dz = c.dz; A.ca = c.cosA; A.cb = c.cosB; A.cg = c.cosG;
     A.C = coords(c);
     A.nx = (int)c.nx; A.ny = (int)c.ny; A.nz = (int)c.nz;
-    hipLaunchKernelGGL(k_force_stencil, dim3(nblocks(7 * (int64_t)nb, 256)), dim3(256), 0, c.stream, A, doi, dod,
-                       g_err);
+    hipLaunchKernelGGL(k_force_stencil, dim3(nblocks(nu7, 256)), dim3(256), 0, c.stream, A, g_doi, g_dod, g_err);
     IB3D_CHECK(hipGetLastError());
-    std::vector<int> oi(8 * 7 * (size_t)nb);
-    std::vector<double> od(5 * 7 * (size_t)nb);
-    IB3D_CHECK(hipMemcpyAsync(oi.data(), doi, oi.size() * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    IB3D_CHECK(hipMemcpyAsync(od.data(), dod, od.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    const int err = read_err(c, g_err);
-    IB3D_CHECK(hipFree(doi));
-    IB3D_CHECK(hipFree(dod));
+    const size_t b_oi = 8 * (size_t)nu7 * sizeof(int), b_od = 5 * (size_t)nu7 * sizeof(double);
+    if (pinned(b_oi + b_od)) return -1;
+    IB3D_CHECK(hipMemcpyAsync(g_pin, g_doi, b_oi, hipMemcpyDeviceToHost, c.stream));
+    IB3D_CHECK(hipMemcpyAsync(static_cast<char *>(g_pin) + b_oi, g_dod, b_od, hipMemcpyDeviceToHost, c.stream));
+    const int err = read_err(c, g_err);  // waits for the stream
     IB3D_REQUIRE(err == 0, "forcing: an image point has no bracketing grid lines");
+    const int *poi = static_cast<const int *>(g_pin);
+    const double *pod = reinterpret_cast<const double *>(static_cast<char *>(g_pin) + b_oi);
+    const std::vector<int> oi(poi, poi + 8 * nu7);
+    const std::vector<double> od(pod, pod + 5 * nu7);
     const size_t nu = 7 * (size_t)nb;
     std::vector<int> src(8 * nu, -1), level(nu, 0);
     std::vector<std::pair<int, int>> finals;
     std::vector<int> fcomp;
     // velocity: per cell u2 u1 v2 v1 w2 w1 in serial order
     std::vector<int> vids, pids;
+    vids.reserve(6 * (size_t)nb);
+    pids.reserve(nb);
     for (int n = 0; n < nb; n++) {
         for (int kind = 0; kind < 6; kind++) vids.push_back(7 * n + kind);
         pids.push_back(7 * n + 6);

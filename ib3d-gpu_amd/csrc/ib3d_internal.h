@@ -56,6 +56,7 @@ struct ForcePlan {
     int *final_loc = nullptr;          // device: write location of each final writer
     int *final_upd = nullptr;          // device: the update that writes it last
     int n_final = 0;
+    int cap_upd = 0, cap_final = 0;    // allocated sizes (the buffers are reused across rebuilds)
 };
 
 struct Ctx {
@@ -119,6 +120,8 @@ struct Ctx {
     // lists: int32 SoA [3][count], (k,j,i) order like the reference
     int64_t ibCount = 0, fluidCount = 0, solidCount = 0, redCount = 0, blackCount = 0;
     int *ibList = nullptr, *fluidList = nullptr, *solidList = nullptr;
+    int64_t cap_lists[3] = {0, 0, 0};  // allocated entries of fluid, solid, ib lists (grow only)
+    int64_t cap_nel = 0;               // allocated IB cells of nel / normdis
     bool lists_ready = false;
     int *nel = nullptr;            // one element per intercepted cell (nelp = nelu1 = ... )
     double *normdis = nullptr;     // [7][ibCount]: p, u1, u2, v1, v2, w1, w2

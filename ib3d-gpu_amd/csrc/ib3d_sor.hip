@@ -20,9 +20,13 @@
 //             pair also supplies the x neighbours); colour rows have even
 //             length so every pair is an aligned vector access, and the
 //             coefficients are hoisted out of the z loop.
-//   k_sor     one cell per thread (kept for reference; not launched).
-//   k_resid   FP64 residual b/dt - A p over both colours (optionally of
+//   k_sor     one cell per thread: the smoother on the coarse multigrid
+//             levels, where a z-marching thread would have too few peers.
+//   k_resid_pz  FP64 residual b/dt - A p over both colours (optionally of
 //             p + e), written as FP32 for the next inner solve.
+//   k_matvec32_pz, k_dots, k_orth, k_givens, k_lsq, k_update_pc
+//             FP32 flexible GMRES inside the FP64 refinement (modes 3, 4).
+//   k_mg_*    the FP32 multigrid V-cycle that preconditions it in mode 4.
 //
 // Host loops never wait on every iteration: sweeps are queued in batches and
 // each kernel first reads a device-side convergence flag of the previous

@@ -14,11 +14,11 @@
 // then iterates to the same tolerance either way.
 //
 // Everything runs in hand-written kernels:
-//   k_conv     implicit-GEMM convolution on MFMA (v_mfma_f32_16x16x32_bf16):
+//   k_conv_lds implicit-GEMM convolution on MFMA (v_mfma_f32_16x16x32_bf16):
 //              D[16 out-ch][16 voxels] = W[16 x 27*Cin] * im2col[27*Cin x 16];
-//              BF16 weights and activations, FP32 accumulation.  With FLIP
-//              and transposed weights the same kernel is the backward-data
-//              convolution.
+//              BF16 weights and activations, FP32 accumulation, the input
+//              tile staged in LDS.  With FLIP and transposed weights the same
+//              kernel is the backward-data convolution.
 //   k_conv_dw  weight gradient on MFMA: D[16 out][16 in] per tap =
 //              dZ^T[16 x 32 voxels] * X_tap[32 voxels x 16], both operands
 //              staged transposed in LDS; deterministic per-block partials.
@@ -94,18 +94,42 @@ __global__ void k_pack_bwd(const float *__restrict__ prm, int woff, int cin, int
 // MODE 0: + bias, ReLU, 16 BF16 channels out.  MODE 1: + bias, channel 0 as
 // FP32 (the output layer).  MODE 2: times ReLU'(gate) (backward data), 16
 // BF16 channels out.  FLIP: source offset -d instead of +d.
+//
+// A 256-thread workgroup computes 16 (x) x 4 (y) x 4 (z) output voxels.  It
+// loads their 18 x 6 x 6 input neighbourhood once into LDS (zero outside the
+// volume: the "same" padding), then every wave runs, for 4 of the 16 output
+// rows of 16 voxels, D[16 out-ch][16 voxels] = W[16 x 27*Cin] *
+// im2col[27*Cin x 16] as 7 (Cin 8) or 14 (Cin 16) MFMAs of K = 32, its B
+// operands read from LDS; the 27-fold reuse of every input value is served
+// by LDS instead of L1/L2.
+static constexpr int kCtX = 16, kCtY = 4, kCtZ = 4;
+static constexpr int kChX = kCtX + 2, kChY = kCtY + 2, kChZ = kCtZ + 2;
+
 template <int CIN, bool FLIP, int MODE>
 __global__ void __launch_bounds__(256)
-k_conv(const __bf16 *__restrict__ X, const bf16x8 *__restrict__ W, const float *__restrict__ bias, Vol v,
-       __bf16 *__restrict__ Y16, float *__restrict__ Y32, const __bf16 *__restrict__ gate, int runs_per_wave,
-       int64_t nruns) {
+k_conv_lds(const __bf16 *__restrict__ X, const bf16x8 *__restrict__ W, const float *__restrict__ bias, Vol v,
+           __bf16 *__restrict__ Y16, float *__restrict__ Y32, const __bf16 *__restrict__ gate, int ntz) {
     constexpr int NKC = CIN == 16 ? 14 : 7;
+    constexpr int VU = CIN / 8;  // 16-byte units per voxel
+    __shared__ __attribute__((aligned(16))) __bf16 tile[kChZ * kChY * kChX * CIN];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = lane & 15, g = lane >> 4;
+    const int x0 = (int)blockIdx.x * kCtX, y0 = (int)blockIdx.y * kCtY;
+    const int b = (int)blockIdx.z / ntz, z0 = ((int)blockIdx.z - b * ntz) * kCtZ;
+    // stage the input neighbourhood
+    for (int u = threadIdx.x; u < kChZ * kChY * kChX * VU; u += 256) {
+        const int vox = u / VU, part = u - vox * VU;
+        const int hx = vox % kChX, hy = (vox / kChX) % kChY, hz = vox / (kChX * kChY);
+        const int xs = x0 - 1 + hx, ys = y0 - 1 + hy, zs = z0 - 1 + hz;
+        bf16x8 val = {};
+        if (xs >= 0 && xs < v.vx && ys >= 0 && ys < v.vy && zs >= 0 && zs < v.vz)
+            val = *reinterpret_cast<const bf16x8 *>(X + vox_index(v, b, zs, ys, xs) * CIN + 8 * part);
+        *reinterpret_cast<bf16x8 *>(tile + vox * CIN + 8 * part) = val;
+    }
     bf16x8 wr[NKC];
 #pragma unroll
     for (int kc = 0; kc < NKC; kc++) wr[kc] = W[kc * 64 + lane];
-    int off_x[NKC], off_y[NKC], off_z[NKC], choff[NKC];
+    int toff[NKC];
     bool tap_ok[NKC];
 #pragma unroll
     for (int kc = 0; kc < NKC; kc++) {
@@ -125,33 +149,22 @@ k_conv(const __bf16 *__restrict__ X, const bf16x8 *__restrict__ W, const float *
             dy = -dy;
             dz = -dz;
         }
-        off_x[kc] = dx;
-        off_y[kc] = dy;
-        off_z[kc] = dz;
-        choff[kc] = c0;
+        // element offset of the tap's source relative to the output voxel's tile slot
+        toff[kc] = ((dz * kChY + dy) * kChX + dx) * CIN + c0;
     }
-    const int nrx = (v.vx + 15) >> 4;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wave) * runs_per_wave;
-    for (int q = 0; q < runs_per_wave; q++) {
-        const int64_t r = r0 + q;
-        if (r >= nruns) break;
-        const int xr = (int)(r % nrx);
-        int64_t rest = r / nrx;
-        const int y = (int)(rest % v.vy);
-        rest /= v.vy;
-        const int z = (int)(rest % v.vz);
-        const int b = (int)(rest / v.vz);
-        const int x = xr * 16 + n;
+    __syncthreads();
+    for (int row = wave; row < kCtY * kCtZ; row += 4) {
+        const int ty = row % kCtY, tz = row / kCtY;
+        const int x = x0 + n, y = y0 + ty, z = z0 + tz;
+        const int base = (((tz + 1) * kChY + (ty + 1)) * kChX + (n + 1)) * CIN;
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int kc = 0; kc < NKC; kc++) {
             bf16x8 xv = {};
-            const int xs = x + off_x[kc], ys = y + off_y[kc], zs = z + off_z[kc];
-            if (tap_ok[kc] && xs >= 0 && xs < v.vx && ys >= 0 && ys < v.vy && zs >= 0 && zs < v.vz)
-                xv = *reinterpret_cast<const bf16x8 *>(X + vox_index(v, b, zs, ys, xs) * CIN + choff[kc]);
+            if (tap_ok[kc]) xv = *reinterpret_cast<const bf16x8 *>(tile + base + toff[kc]);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[kc], xv, acc, 0, 0, 0);
         }
-        if (x < v.vx) {
+        if (x < v.vx && y < v.vy && z < v.vz) {
             const int64_t vo = vox_index(v, b, z, y, x);
             if (MODE == 1) {
                 if (g == 0) Y32[vo] = acc[0] + bias[0];
@@ -629,31 +642,35 @@ static int net_init(uint64_t seed) {
     return pack_all();
 }
 
-static int64_t conv_runs(const Vol &v) { return (int64_t)v.nb * v.vz * v.vy * ((v.vx + 15) / 16); }
+
+static dim3 conv_grid(const Vol &v, int *ntz) {
+    *ntz = (v.vz + kCtZ - 1) / kCtZ;
+    return dim3((unsigned)((v.vx + kCtX - 1) / kCtX), (unsigned)((v.vy + kCtY - 1) / kCtY), (unsigned)(*ntz * v.nb));
+}
 
 static void launch_fwd(int L, const Vol &v, const __bf16 *in, __bf16 *out16, float *out32) {
     Net &N = g_net;
     hipStream_t s = ctx().stream;
-    const int64_t nr = conv_runs(v);
-    const int rpw = 4;
-    const dim3 grid(nblocks(nr, 4 * rpw)), blk(256);
+    int ntz;
+    const dim3 grid = conv_grid(v, &ntz), blk(256);
     const bf16x8 *W = reinterpret_cast<const bf16x8 *>(N.wf[L]);
     if (L == 0)
-        hipLaunchKernelGGL((k_conv<8, false, 0>), grid, blk, 0, s, in, W, N.bias, v, out16, nullptr, nullptr, rpw, nr);
+        hipLaunchKernelGGL((k_conv_lds<8, false, 0>), grid, blk, 0, s, in, W, N.bias, v, out16, nullptr, nullptr,
+                           ntz);
     else if (L < 3)
-        hipLaunchKernelGGL((k_conv<16, false, 0>), grid, blk, 0, s, in, W, N.bias + 16 * L, v, out16, nullptr,
-                           nullptr, rpw, nr);
+        hipLaunchKernelGGL((k_conv_lds<16, false, 0>), grid, blk, 0, s, in, W, N.bias + 16 * L, v, out16, nullptr,
+                           nullptr, ntz);
     else
-        hipLaunchKernelGGL((k_conv<16, false, 1>), grid, blk, 0, s, in, W, N.bias + 48, v, nullptr, out32, nullptr,
-                           rpw, nr);
+        hipLaunchKernelGGL((k_conv_lds<16, false, 1>), grid, blk, 0, s, in, W, N.bias + 48, v, nullptr, out32,
+                           nullptr, ntz);
 }
 
 static void launch_bwd_data(int L, const Vol &v, const __bf16 *dz, const __bf16 *gate, __bf16 *out) {
     Net &N = g_net;
-    const int64_t nr = conv_runs(v);
-    const int rpw = 4;
-    hipLaunchKernelGGL((k_conv<16, true, 2>), dim3(nblocks(nr, 4 * rpw)), dim3(256), 0, ctx().stream, dz,
-                       reinterpret_cast<const bf16x8 *>(N.wb[L]), nullptr, v, out, nullptr, gate, rpw, nr);
+    int ntz;
+    const dim3 grid = conv_grid(v, &ntz);
+    hipLaunchKernelGGL((k_conv_lds<16, true, 2>), grid, dim3(256), 0, ctx().stream, dz,
+                       reinterpret_cast<const bf16x8 *>(N.wb[L]), nullptr, v, out, nullptr, gate, ntz);
 }
 
 static int launch_dw(int L, const Vol &v, const __bf16 *x, const __bf16 *dz) {

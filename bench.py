@@ -26,8 +26,8 @@ Poisson modes (same stopping rule: ||b/dt - A p||_2 <= 1e-6 ||b/dt||_2):
                    multigrid V-cycle (red-black SOR smoothing, 2+2 sweeps per
                    level at omega 1.55, trilinear prolongation),
                    previous-step guess
-  mixed+learned -- mixed, starting from the CNN's guess when it has the
-                   smaller residual (headline)
+  mixed+learned -- mixed, starting from the CNN's guess when its residual
+                   is below half the previous-step guess's (headline)
 The headline `value` is steps/s of --mode; the --compare modes are measured
 in the same run and reported beside it, with the Poisson speedup of --mode
 over each.  The network is trained first on 1000 recorded solves of the
@@ -74,6 +74,11 @@ GMRES = {"m": 12, "k": 20, "omega": 1.9}
 # post-smoothing sweeps per level and their omega, sweeps and omega on the
 # coarsest level
 MG = {"m": 6, "nu1": 2, "nu2": 2, "omega": 1.55, "coarse": 8, "coarse_omega": 1.5}
+# The learned guess is kept when its residual is below this fraction of the
+# previous-step guess's.  With the multigrid solver a guess that is only
+# 10-40% better in residual norm was measured to double the Krylov steps of
+# the solve, so a 2x margin is asked for.
+LEARNED_ACCEPT = 0.5
 
 
 class Motion:
@@ -105,6 +110,7 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None,
         s.set_bodies(body)
     if net is not None:
         net.config(record=False, guess=mode == "mixed+learned")
+        net.set_accept(LEARNED_ACCEPT if mode == "mixed+learned" else 1.0)
     if mode == "fp64":
         s.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=True)
     elif mode == "fp64-cold":
@@ -361,7 +367,8 @@ def main():
                                    if a.mode == "gmres-sor" else
                                    f"{a.mode}: RB-SOR omega={a.omega}, {a.inner} sweeps between residual checks")
                                   + f", ||r||/||b|| <= {a.tol}, "
-                                  + ("CNN guess with previous-step fallback" if a.mode == "mixed+learned"
+                                  + (f"CNN guess, kept when its residual is below {LEARNED_ACCEPT} x the previous-step "
+                                     f"guess's, else the previous-step guess" if a.mode == "mixed+learned"
                                      else "previous-step guess" if a.mode in ("mixed", "gmres-sor", "mixed-dc", "fp64")
                                      else "zero guess"),
                        "parallelism": "1 GPU"},

@@ -502,12 +502,13 @@ __global__ void k_cnn_apply(const float *__restrict__ net, const double *__restr
     if (bd == bd) (colour ? pc1 : pc0)[f] = (colour ? pp1 : pp0)[f] + scal[0] * (double)net[idx];
 }
 
-// keep the guess only if its residual is below the previous-step guess's;
+// keep the guess only if its residual is below accept x the previous-step
+// guess's;
 // flag[0] = 1 when the guess is kept
 __global__ void k_cnn_select(const double *__restrict__ rel_prev, const double *__restrict__ rel_guess,
                              const double *__restrict__ pp0, const double *__restrict__ pp1, double *__restrict__ pc0,
-                             double *__restrict__ pc1, int64_t NC, double *__restrict__ flag) {
-    const bool keep = *rel_guess < *rel_prev;
+                             double *__restrict__ pc1, int64_t NC, double *__restrict__ flag, double accept) {
+    const bool keep = *rel_guess < accept * *rel_prev;
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (q == 0) flag[0] = keep ? 1.0 : 0.0;
     if (keep || q >= NC) return;
@@ -552,6 +553,7 @@ struct Net {
     int64_t set_sx = 0, set_sy = 0, set_sz = 0, set_nx = 0, set_ny = 0, set_nz = 0;
     // solver integration
     int record = 0, guess = 0;
+    double accept = 1.0;  // keep the guess when rel_guess < accept * rel_prev
     double *pprev[2] = {};
     int64_t pprev_nc = 0;
     int64_t solves = 0, kept = 0;
@@ -800,7 +802,7 @@ int cnn_before_solve(Ctx &c, const double *rel_prev, double *rel_guess, int (*re
                            N.dsc);
         if (resid(c, rel_guess)) return -1;
         hipLaunchKernelGGL(k_cnn_select, dim3(nblocks(c.NC, 256)), dim3(256), 0, c.stream, rel_prev, rel_guess,
-                           N.pprev[0], N.pprev[1], c.pc[0], c.pc[1], c.NC, N.dsc + 4);
+                           N.pprev[0], N.pprev[1], c.pc[0], c.pc[1], c.NC, N.dsc + 4, N.accept);
         IB3D_CHECK(hipEventRecord(e1, c.stream));
         IB3D_CHECK(hipGetLastError());
         IB3D_CHECK(hipEventSynchronize(e1));
@@ -977,6 +979,12 @@ int ib3d_cnn_adam(const float *grad, double lr, double beta1, double beta2, doub
     IB3D_REQUIRE(N.ready, "cnn_adam: network not initialised");
     IB3D_CHECK(h2d(N.grad, grad, kNParam * sizeof(float)));
     return adam_step(lr, beta1, beta2, eps);
+}
+
+int ib3d_cnn_set_accept(double ratio) {
+    IB3D_REQUIRE(ratio > 0.0 && ratio <= 1.0, "cnn_set_accept: ratio must lie in (0, 1]");
+    g_net.accept = ratio;
+    return 0;
 }
 
 int ib3d_cnn_config(int record, int guess) {

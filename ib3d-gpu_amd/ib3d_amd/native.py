@@ -83,6 +83,7 @@ def lib():
         "ib3d_cnn_gradient": [PF, PF, I64, I64, I64, I64, PF],
         "ib3d_cnn_adam": [PF, D, D, D, D],
         "ib3d_cnn_config": [I, I],
+        "ib3d_cnn_set_accept": [D],
         "ib3d_cnn_clear_pairs": [],
         "ib3d_cnn_train": [I, I, D, C.c_uint64, PD],
         "ib3d_cnn_stats": [PI64, PI64, PD, PD, PD],
@@ -354,6 +355,9 @@ class Net:
 
     def config(self, record=False, guess=False):
         _ok(self.L.ib3d_cnn_config(int(bool(record)), int(bool(guess))))
+
+    def set_accept(self, ratio):
+        _ok(self.L.ib3d_cnn_set_accept(float(ratio)))
 
     def pair_count(self):
         return int(self.L.ib3d_cnn_pair_count())

@@ -214,6 +214,12 @@ int ib3d_cnn_adam(const float *grad, double lr, double beta1, double beta2, doub
  * training pair on the device; guess = 1: every L2-mode solve starts from the
  * network's guess when it lowers the residual */
 int ib3d_cnn_config(int record, int guess);
+/* The guess is kept when its relative residual is below ratio x the
+ * previous-step guess's, ratio in (0, 1].  1 (the default) keeps any guess
+ * that lowers the residual.  A smaller ratio asks for a margin: with the
+ * multigrid solver (mode 4), a guess that lowers the residual norm by less
+ * than about 1.5x can make the solve need more Krylov steps, not fewer. */
+int ib3d_cnn_set_accept(double ratio);
 int64_t ib3d_cnn_pair_count(void);
 int ib3d_cnn_clear_pairs(void);
 /* Adam over 64^3 tiles (halo 4) of the recorded pairs; mean loss per epoch */

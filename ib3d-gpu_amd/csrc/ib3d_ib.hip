@@ -875,6 +875,7 @@ int count_tags(Ctx &c, int32_t out[4], bool scatter) {
 int cell_count(Ctx &c, int32_t counts[5]) {
     int32_t t4[4];
     if (count_tags(c, t4, true)) return -1;
+    c.geom_version++;
     counts[0] = (int32_t)c.ibCount;
     counts[1] = (int32_t)c.fluidCount;
     counts[2] = (int32_t)c.solidCount;

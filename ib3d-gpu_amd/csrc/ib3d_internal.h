@@ -165,6 +165,7 @@ struct Ctx {
     // statistics of the last Poisson solve
     int64_t last_sweeps = 0;
     int64_t last_outer = 0;
+    int64_t geom_version = 0;      // bumped whenever cellCount rebuilds the lists (the tags changed)
     double last_relres = 0.0, last_r0 = 0.0;
     double last_sor_ms = 0.0, last_poisson_ms = 0.0;
     hipEvent_t ev[4] = {};

@@ -1124,6 +1124,7 @@ static int sor_l2(Ctx &c, bool mixed, double bn, int64_t *sweeps, int64_t *outer
 // GMRES-IR buffers: V (m+1 vectors) and Z (m vectors) of 2*NC floats, H,
 // beta, y and reduction partials
 static float *g_kv = nullptr, *g_kz = nullptr;
+static int64_t g_z_geom = -1;
 static double *g_kh = nullptr, *g_kpart = nullptr, *g_kpart2 = nullptr;
 static int64_t g_k_nc = 0;
 static int g_k_m = 0;
@@ -1730,6 +1731,7 @@ static int gmres_buffers(Ctx &c, int m) {
     IB3D_CHECK(hipMalloc(&g_kpart2, (int64_t)kVecBlocks * sizeof(double)));
     g_k_nc = c.NC;
     g_k_m = m;
+    g_z_geom = -1;
     return 0;
 }
 
@@ -1792,11 +1794,26 @@ static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target)
     hipLaunchKernelGGL(k_update_pc, vb, tb, 0, c.stream, c.pc[0], c.pc[1], g_kz, stride, m, y, V0, NC, act, 0.5);
 }
 
+// Cells outside the solve must read 0 in the preconditioned vectors Z and
+// in the multigrid corrections: the sweeps never write a pair of such cells,
+// so after the geometry changes (a moving body) values left from the old
+// geometry would act as non-zero boundary values.  Clear them once per
+// geometry (g_z_geom: the geometry they were cleared for).
+static int clear_for_geometry(Ctx &c, int m) {
+    if (g_z_geom == c.geom_version) return 0;
+    IB3D_CHECK(dzero(g_kz, (int64_t)m * 2 * c.NC * sizeof(float)));
+    for (size_t l = 1; l < g_mg.size(); l++)
+        if (g_mg[l].x) IB3D_CHECK(dzero(g_mg[l].x, 2 * g_mg[l].NC * sizeof(float)));
+    g_z_geom = c.geom_version;
+    return 0;
+}
+
 static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double *rel, double *rel0) {
     const int m = c.krylov_m, k = c.mode == 4 ? 1 : c.check;
     if (sor_float_tables(c)) return -1;
     if (gmres_buffers(c, m)) return -1;
     if (c.mode == 4 && mg_setup(c)) return -1;
+    if (clear_for_geometry(c, m)) return -1;
     if (ensure_partial(nblocks(2 * c.NC, kSorBlock))) return -1;
     const int64_t max_outer = std::max<int64_t>(1, c.max_sweeps / ((int64_t)m * k));
     if (ensure_slots(2 * (max_outer + 2))) return -1;

@@ -46,3 +46,42 @@ def test_no_gpu_means_an_error_not_a_fallback():
     env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=120)
     assert "ERR" in out.stdout, out.stdout + out.stderr
+
+
+def _err(lib):
+    lib.ib3d_last_error.restype = ctypes.c_char_p
+    return lib.ib3d_last_error().decode()
+
+
+def test_solver_option_entry_points_validate_their_arguments():
+    """The Poisson-mode options are checked before anything touches a GPU,
+    so their error paths run here: a bad value returns nonzero with a
+    message and leaves the last good setting in place."""
+    from ib3d_amd import native
+    lib = native.lib()
+    D, I = ctypes.c_double, ctypes.c_int
+    lib.ib3d_set_poisson_mode.argtypes = [I, D, D, I, I, ctypes.c_int64]
+    lib.ib3d_set_krylov.argtypes = [I]
+    lib.ib3d_set_multigrid.argtypes = [I, I, D, I, D, I]
+    lib.ib3d_cnn_set_accept.argtypes = [D]
+    assert lib.ib3d_set_poisson_mode(4, 1.9, 1e-6, 20, 1, 0) == 0
+    assert lib.ib3d_set_poisson_mode(5, 1.9, 1e-6, 20, 1, 0) != 0
+    assert "mode must be" in _err(lib)
+    assert lib.ib3d_set_poisson_mode(3, 2.0, 1e-6, 20, 1, 0) != 0
+    assert "omega" in _err(lib)
+    assert lib.ib3d_set_krylov(6) == 0
+    for bad in (0, 24):
+        assert lib.ib3d_set_krylov(bad) != 0
+        assert "restart" in _err(lib)
+    assert lib.ib3d_set_multigrid(2, 2, 1.55, 8, 1.5, 0) == 0
+    cases = [((0, 2, 1.55, 8, 1.5, 0), "pre-smoothing"), ((2, -1, 1.55, 8, 1.5, 0), "post-smoothing"),
+             ((2, 2, 2.0, 8, 1.5, 0), "relaxation"), ((2, 2, 1.55, 0, 1.5, 0), "coarse sweeps"),
+             ((2, 2, 1.55, 8, 1.5, 1), "max_levels")]
+    for args, word in cases:
+        assert lib.ib3d_set_multigrid(*args) != 0
+        assert word in _err(lib), (args, _err(lib))
+    assert lib.ib3d_cnn_set_accept(0.5) == 0
+    for bad in (0.0, 1.5, -1.0):
+        assert lib.ib3d_cnn_set_accept(bad) != 0
+        assert "ratio" in _err(lib)
+    assert lib.ib3d_cnn_set_accept(1.0) == 0

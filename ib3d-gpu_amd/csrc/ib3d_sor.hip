@@ -638,38 +638,38 @@ k_matvec32_pz(const float *__restrict__ z0, const float *__restrict__ z1, const 
 static constexpr int kMaxKrylov = 24;
 static constexpr int kVecBlocks = 2048;
 
-// dot products of w with v_0..v_{nv-1} (vectors of n2 float2 pairs, NaN
-// entries skipped), per-block partials [i][block]
+// dot products of w with v_0..v_{NV-1} (vectors of n4 float4 groups, NaN
+// entries of w skipped), per-block partials [i][block]; NV is a template
+// parameter so the accumulators and loads unroll exactly
+template <int NV>
 __global__ void __launch_bounds__(kSorBlock)
-k_dots(const float *__restrict__ w, const float *__restrict__ V, int64_t stride, int nv, int64_t n2,
+k_dots(const float *__restrict__ w, const float *__restrict__ V, int64_t stride, int64_t n4,
        double *__restrict__ partial, const double *gate, double thr) {
     if (gated(gate, thr)) return;
-    __shared__ double sm[kMaxKrylov][kSorBlock / 64];
-    double s[kMaxKrylov];
+    __shared__ double sm[NV][kSorBlock / 64];
+    double s[NV];
 #pragma unroll
-    for (int i = 0; i < kMaxKrylov; i++) s[i] = 0.0;
-    for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < n2; q += (int64_t)gridDim.x * kSorBlock) {
-        const f2v a = reinterpret_cast<const f2v *>(w)[q];
-        const bool ok0 = a.x == a.x, ok1 = a.y == a.y;
+    for (int i = 0; i < NV; i++) s[i] = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < n4; q += (int64_t)gridDim.x * kSorBlock) {
+        const f4v a = reinterpret_cast<const f4v *>(w)[q];
+        const bool ok0 = a.x == a.x, ok1 = a.y == a.y, ok2 = a.z == a.z, ok3 = a.w == a.w;
 #pragma unroll
-        for (int i = 0; i < kMaxKrylov; i++) {
-            if (i < nv) {
-                const f2v b = reinterpret_cast<const f2v *>(V + i * stride)[q];
-                if (ok0) s[i] += (double)a.x * (double)b.x;
-                if (ok1) s[i] += (double)a.y * (double)b.y;
-            }
+        for (int i = 0; i < NV; i++) {
+            const f4v b = reinterpret_cast<const f4v *>(V + i * stride)[q];
+            if (ok0) s[i] += (double)a.x * (double)b.x;
+            if (ok1) s[i] += (double)a.y * (double)b.y;
+            if (ok2) s[i] += (double)a.z * (double)b.z;
+            if (ok3) s[i] += (double)a.w * (double)b.w;
         }
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < kMaxKrylov; i++) {
-        if (i < nv) {
-            double v = wave_sum(s[i]);
-            if (lane == 0) sm[i][wid] = v;
-        }
+    for (int i = 0; i < NV; i++) {
+        const double v = wave_sum(s[i]);
+        if (lane == 0) sm[i][wid] = v;
     }
     __syncthreads();
-    if ((int)threadIdx.x < nv)
+    if ((int)threadIdx.x < NV)
         partial[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] =
             ((sm[threadIdx.x][0] + sm[threadIdx.x][1]) + sm[threadIdx.x][2]) + sm[threadIdx.x][3];
 }
@@ -690,28 +690,65 @@ __global__ void k_rows_sum(const double *__restrict__ partial, int nblk, double 
     if (threadIdx.x == 0) out[blockIdx.x] = accumulate ? out[blockIdx.x] + sm[0] : sm[0];
 }
 
-// w -= sum_i h[i] v_i, then per-block partial sums of w^2 (NaN skipped)
+// w -= sum_i h[i] v_i (FP64 accumulation), then per-block partial sums of
+// w^2 (NaN skipped); float4 groups, NV unrolled
+template <int NV>
 __global__ void __launch_bounds__(kSorBlock)
-k_orth(float *__restrict__ w, const float *__restrict__ V, int64_t stride, int nv, const double *__restrict__ h,
-       int64_t n2, double *__restrict__ partial, const double *gate, double thr) {
+k_orth(float *__restrict__ w, const float *__restrict__ V, int64_t stride, const double *__restrict__ h, int64_t n4,
+       double *__restrict__ partial, const double *gate, double thr) {
     if (gated(gate, thr)) return;
+    double hv[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) hv[i] = h[i];
     double s = 0.0;
-    for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < n2; q += (int64_t)gridDim.x * kSorBlock) {
-        f2v a = reinterpret_cast<f2v *>(w)[q];
-        double ax = a.x, ay = a.y;
-        for (int i = 0; i < nv; i++) {
-            const f2v b = reinterpret_cast<const f2v *>(V + i * stride)[q];
-            ax -= h[i] * (double)b.x;
-            ay -= h[i] * (double)b.y;
+    for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < n4; q += (int64_t)gridDim.x * kSorBlock) {
+        f4v a = reinterpret_cast<f4v *>(w)[q];
+        double ax = a.x, ay = a.y, az = a.z, aw = a.w;
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+            const f4v b = reinterpret_cast<const f4v *>(V + i * stride)[q];
+            ax -= hv[i] * (double)b.x;
+            ay -= hv[i] * (double)b.y;
+            az -= hv[i] * (double)b.z;
+            aw -= hv[i] * (double)b.w;
         }
         a.x = (float)ax;
         a.y = (float)ay;
-        reinterpret_cast<f2v *>(w)[q] = a;
+        a.z = (float)az;
+        a.w = (float)aw;
+        reinterpret_cast<f4v *>(w)[q] = a;
         if (a.x == a.x) s += (double)a.x * (double)a.x;
         if (a.y == a.y) s += (double)a.y * (double)a.y;
+        if (a.z == a.z) s += (double)a.z * (double)a.z;
+        if (a.w == a.w) s += (double)a.w * (double)a.w;
     }
     block_sum_to(s, partial, blockIdx.x);
 }
+
+// launch k_dots<nv> / k_orth<nv> for nv = 1..kMaxKrylov-1
+#define IB3D_GS_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) \
+    X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23)
+static void launch_dots(int nv, dim3 g, dim3 b, hipStream_t st, const float *w, const float *V, int64_t stride,
+                        int64_t n4, double *partial, const double *gate) {
+    switch (nv) {
+#define IB3D_DOTS(N) \
+    case N: hipLaunchKernelGGL(k_dots<N>, g, b, 0, st, w, V, stride, n4, partial, gate, 0.5); break;
+        IB3D_GS_CASES(IB3D_DOTS)
+#undef IB3D_DOTS
+        default: break;
+    }
+}
+static void launch_orth(int nv, dim3 g, dim3 b, hipStream_t st, float *w, const float *V, int64_t stride,
+                        const double *h, int64_t n4, double *partial, const double *gate) {
+    switch (nv) {
+#define IB3D_ORTH(N) \
+    case N: hipLaunchKernelGGL(k_orth<N>, g, b, 0, st, w, V, stride, h, n4, partial, gate, 0.5); break;
+        IB3D_GS_CASES(IB3D_ORTH)
+#undef IB3D_ORTH
+        default: break;
+    }
+}
+#undef IB3D_GS_CASES
 
 // partial sums of x^2 (NaN skipped)
 __global__ void __launch_bounds__(kSorBlock)
@@ -1845,9 +1882,9 @@ static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target)
         hipLaunchKernelGGL(k_matvec32_pz<false>, pz_grid(c), tb, 0, c.stream, Z, Z + NC, V0, V0 + NC, Vn, Vn + NC,
                            g_axf, g_ayf, g_azf, g, aj, 0.5);
         double *hj = H + (int64_t)(m + 1) * j;
-        hipLaunchKernelGGL(k_dots, vb, tb, 0, c.stream, Vn, g_kv, stride, j + 1, n2, g_kpart, aj, 0.5);
+        launch_dots(j + 1, vb, tb, c.stream, Vn, g_kv, stride, n2 / 2, g_kpart, aj);
         hipLaunchKernelGGL(k_rows_sum, dim3(j + 1), dim3(256), 0, c.stream, g_kpart, kVecBlocks, hj, 0, aj, 0.5);
-        hipLaunchKernelGGL(k_orth, vb, tb, 0, c.stream, Vn, g_kv, stride, j + 1, hj, n2, g_kpart2, aj, 0.5);
+        launch_orth(j + 1, vb, tb, c.stream, Vn, g_kv, stride, hj, n2 / 2, g_kpart2, aj);
         hipLaunchKernelGGL(k_sqrt_sum, dim3(1), dim3(256), 0, c.stream, g_kpart2, kVecBlocks, hj + j + 1, aj, 0.5);
         if (j + 1 < m)  // (before k_givens, which rotates h[j+1] to 0)
             hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, Vn, Vn, hj + j + 1, n2, aj, 0.5);

@@ -845,20 +845,35 @@ __global__ void k_lsq(const double *__restrict__ H, int m, const double *gv, con
     }
 }
 
-// pc += sum_j y_j z_j on cells the solve updates (mask not NaN)
+// pc += sum_j y_j z_j on cells the solve updates (mask not NaN); one pair of
+// cells of one colour per thread (NC is even, so pairs never straddle the
+// two colour arrays); y_j = 0 (steps that did not run) skips z_j
 __global__ void __launch_bounds__(kSorBlock)
 k_update_pc(double *__restrict__ p0, double *__restrict__ p1, const float *__restrict__ Z, int64_t stride, int m,
             const double *__restrict__ y, const float *__restrict__ mask, int64_t NC, const double *gate,
             double thr) {
     if (gated(gate, thr)) return;
-    for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < 2 * NC;
-         q += (int64_t)gridDim.x * kSorBlock) {
-        if (!(mask[q] == mask[q])) continue;
-        double d = 0.0;
-        for (int j = 0; j < m; j++)
-            if (y[j] != 0.0) d += y[j] * (double)Z[j * stride + q];
+    int nz = 0;  // steps after the last non-zero y add nothing
+    for (int j = 0; j < m; j++)
+        if (y[j] != 0.0) nz = j + 1;
+    const int64_t npair = NC;  // 2*NC cells = NC pairs
+    for (int64_t q2 = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q2 < npair;
+         q2 += (int64_t)gridDim.x * kSorBlock) {
+        const int64_t q = 2 * q2;
+        const f2v mk = *reinterpret_cast<const f2v *>(mask + q);
+        const bool ok0 = mk.x == mk.x, ok1 = mk.y == mk.y;
+        if (!(ok0 || ok1)) continue;
+        double d0 = 0.0, d1 = 0.0;
+        for (int j = 0; j < nz; j++) {
+            const double yj = y[j];
+            if (yj == 0.0) continue;
+            const f2v z = *reinterpret_cast<const f2v *>(Z + j * stride + q);
+            d0 += yj * (double)z.x;
+            d1 += yj * (double)z.y;
+        }
         double *p = q < NC ? p0 + q : p1 + (q - NC);
-        *p += d;
+        if (ok0) p[0] += d0;
+        if (ok1) p[1] += d1;
     }
 }
 

@@ -104,6 +104,9 @@ static void free_all(Ctx &c) {
                      &c.cosG, &c.alpha3, &c.beta3, &c.gamma3, &c.normdis};
     for (auto pp : d1) dfree(*pp);
     for (int t = 0; t < 6; t++) dfree(c.tab[t]);
+    for (int t = 0; t < 3; t++) dfree(c.mtab[t]);
+    c.mtn[0] = c.mtn[1] = c.mtn[2] = 0;
+    c.mom_ready = false;
     dfree(c.e32[0]); dfree(c.r32[0]);   // colour 1 lives in the same blocks
     c.e32[1] = nullptr;
     c.r32[1] = nullptr;
@@ -255,6 +258,7 @@ static int non_uni_coeff(Ctx &c) {
     for (int t = 0; t < 6; t++)
         if (upload_vec(c.tab[t], c.htab[t])) return -1;
     c.coeff_ready = true;
+    c.mom_ready = false;
     return 0;
 }
 
@@ -605,6 +609,7 @@ int ib3d_upload(const char *name, const void *host, int64_t nbytes) {
         case F64_VEC: {
             std::memcpy(e.host->data() + 1, host, nbytes);
             IB3D_CHECK(h2d(e.dev, e.host->data(), e.host->size() * sizeof(double)));
+            c.mom_ready = false;   // the momentum tables read the spacings
             break;
         }
         case HOST_VEC: {

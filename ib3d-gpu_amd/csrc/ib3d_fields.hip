@@ -7,6 +7,9 @@
 // Fortran order (coalesced, i fastest) and test the int8 tag instead.
 // Arithmetic is written operand for operand like the reference (built with
 // -ffp-contract=off, correctly rounded division), so results are bitwise.
+// The predictor takes its index-only sub-expressions from per-axis tables
+// computed on the host in the same operations, and divides field values
+// through correctly rounded reciprocals (dv below), which is also exact.
 #include "ib3d_internal.h"
 
 namespace ib3d {
@@ -20,33 +23,84 @@ enum { UU = 0, VV = 1, WW = 2, UV = 3, VU = 4, WU = 5 };
 
 #define TB(set, c, i) (T.t[set][(c) * T.tn[set] + (i)])
 
-// 3rd-order Kawamura-Kuwahara convection term (subNavierStokes.f90:812-831)
+// Per-axis momentum tables, quantity-major with n+3 entries per quantity.
+// Each entry is a sub-expression of the momentum formula that depends on
+// one grid index only, evaluated once on the host in the same IEEE
+// operations as the formula (so reading it changes no bit), together with
+// the correctly rounded reciprocal of every denominator that divides a
+// field value.  "S" quantities serve the velocity component staggered along
+// the axis (u along x), "C" the two components across it.
+enum MomQ {
+    M_AS,        // d(i+1) / (d(i) + d(i+1)): face interpolation weight
+    M_CS,        // 2 / (d(i) + d(i+1)): second-difference weight
+    M_D1, M_RD1, // d(i+1) and 1/d(i+1)
+    M_D0, M_RD0, // d(i) and 1/d(i)
+    M_KS1, M_RKS1, M_KS2, M_RKS2,  // face table: t5*d(i+1), (2 t11)*d(i), reciprocals
+    M_US,        // 7 upwind factors at den = d(i+1), r = d(i)/d(i+1)
+    M_CC = M_US + 7,               // 2 / (0.5 ((d(i)+d(i+1)) + (d(i)+d(i-1))))
+    M_H1, M_RH1, // h1 = 0.5 (d(i) + d(i+1)) and 1/h1
+    M_H0, M_RH0, // h0 = 0.5 (d(i) + d(i-1)) and 1/h0
+    M_KC1, M_RKC1, M_KC2, M_RKC2,  // centre table: t5*h1, (2 t11)*h0, reciprocals
+    M_UC,        // 7 upwind factors at den = h1, r = h0/h1
+    M_NQ = M_UC + 7
+};
+
+struct MomTabs {
+    const double *m[3];
+    int mn[3];
+};
+
+// a / b, correctly rounded, from y = RN(1/b) and two FMA corrections of
+// a*y.  The first brings the quotient within one ulp; by Markstein's
+// theorem (y within half an ulp of 1/b, quotient within one ulp) the
+// remainder a - b*q is then exact and the second correction rounds
+// correctly.  The host enables this only when every denominator lies in
+// [2^-64, 2^64]; numerators outside [2^-900, 2^900] (zeros, NaN and
+// infinities among them) use the division instruction, so no intermediate
+// leaves the normal range and signed zeros come out as from a / b.
+template <bool FAST>
+__device__ __forceinline__ double dv(double a, double b, double y) {
+    if (!FAST) return a / b;
+    const double m = fabs(a);
+    if (!(m >= 0x1p-900 && m <= 0x1p900)) return a / b;
+    double q = a * y;
+    double r = fma(-b, q, a);
+    q = fma(r, y, q);
+    r = fma(-b, q, a);
+    return fma(r, y, q);
+}
+
+// 3rd-order Kawamura-Kuwahara convection term (subNavierStokes.f90:812-831);
+// dA = t5*dA', dB = (2 t11)*dB' come from the axis table with their reciprocals
+template <bool FAST>
 __device__ __forceinline__ double kk3(const Tabs &T, int set, int i, double vel, double f2, double f1, double f0,
-                                      double fm1, double fm2, double dA, double dB) {
+                                      double fm1, double fm2, double dA, double rA, double dB, double rB) {
     const double sa = (((TB(set, 0, i) * f2 + TB(set, 1, i) * f1) + TB(set, 2, i) * f0) + TB(set, 3, i) * fm1) +
                       TB(set, 4, i) * fm2;
     const double sk = (((TB(set, 6, i) * f2 + TB(set, 7, i) * f1) + TB(set, 8, i) * f0) + TB(set, 9, i) * fm1) +
                       TB(set, 10, i) * fm2;
-    return (vel * sa) / (TB(set, 5, i) * dA) + (fabs(vel) * sk) / ((2.0 * TB(set, 11, i)) * dB);
+    return dv<FAST>(vel * sa, dA, rA) + dv<FAST>(fabs(vel) * sk, dB, rB);
 }
 
-// non-uniform 1st-order upwind term (subNavierStokes.f90:847-866), al = 1
-__device__ __forceinline__ double up1(double vel, double den, double r, double fm, double f0, double fp) {
-    const double t1 = (((-1.0 / (r * (r + 1.0))) * fm - (1.0 - 1.0 / r) * f0) + (r / (r + 1.0)) * fp);
-    const double t2 = ((2.0 / (r * (r + 1.0))) * fm - (2.0 / r) * f0) + (2.0 / (r + 1.0)) * fp;
-    return (vel / den) * t1 - ((1.0 * fabs(vel)) * (1.0 / (2.0 * den))) * t2;
+// non-uniform 1st-order upwind term (subNavierStokes.f90:847-866), al = 1;
+// g[0..6*n] = -1/(r(r+1)), 1-1/r, r/(r+1), 2/(r(r+1)), 2/r, 2/(r+1), 1/(2 den)
+template <bool FAST>
+__device__ __forceinline__ double up1(const double *g, int n, double vel, double den, double rden, double fm,
+                                      double f0, double fp) {
+    const double t1 = ((g[0] * fm - g[n] * f0) + g[2 * n] * fp);
+    const double t2 = (g[3 * n] * fm - g[4 * n] * f0) + g[5 * n] * fp;
+    return dv<FAST>(vel, den, rden) * t1 - (fabs(vel) * g[6 * n]) * t2;
 }
 
-__global__ void __launch_bounds__(256)
-k_momentum(const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
-           const double *__restrict__ p, const int8_t *__restrict__ cell, double *__restrict__ ut,
-           double *__restrict__ vt, double *__restrict__ wt, double *__restrict__ ru, double *__restrict__ rv,
-           double *__restrict__ rw, const double *__restrict__ dxa, const double *__restrict__ dya,
-           const double *__restrict__ dza, Tabs T, int nx, int ny, int nz, double rev, double deltat) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
-    const int k = blockIdx.z + 2;
-    if (i > nx + 1 || j > ny + 1) return;
+// One fluid cell of the explicit predictor (subNavierStokes.f90:622-1122).
+// The Kawamura-Kuwahara stencil applies away from the walls and from
+// intercepted cells, the upwind stencil elsewhere.
+template <bool FAST>
+__device__ __forceinline__ void momentum_cell(
+    int i, int j, int k, const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
+    const double *__restrict__ p, const int8_t *__restrict__ cell, double *__restrict__ ut, double *__restrict__ vt,
+    double *__restrict__ wt, double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw,
+    const Tabs &T, const MomTabs &M, int nx, int ny, int nz, double rev, double deltat) {
     const int64_t sx = nx + 2, sxy = sx * (ny + 2);
     const int64_t c = (i - 1) + sx * ((j - 1) + (int64_t)(ny + 2) * (k - 1));
     if (cell[c] != 0) return;
@@ -55,9 +109,9 @@ k_momentum(const double *__restrict__ u, const double *__restrict__ v, const dou
 #define W(a, b, d) w[c + (a) + (b) * sx + (d) * sxy]
 #define P(a, b, d) p[c + (a) + (b) * sx + (d) * sxy]
 #define CL(a, b, d) cell[c + (a) + (b) * sx + (d) * sxy]
-    const double dxr = dxa[i + 1], dx = dxa[i], dxl = dxa[i - 1];
-    const double dye = dya[j + 1], dy = dya[j], dyw = dya[j - 1];
-    const double dzt = dza[k + 1], dz = dza[k], dzb = dza[k - 1];
+#define MX(q) M.m[0][(q) * M.mn[0] + i]
+#define MY(q) M.m[1][(q) * M.mn[1] + j]
+#define MZ(q) M.m[2][(q) * M.mn[2] + k]
     const double u22 = U(-1, 0, 0) - U(0, 0, 0), u4 = U(0, 0, 0) - U(1, 0, 0);
     const double u6 = U(0, -1, 0) - U(0, 0, 0), u8 = U(0, 0, 0) - U(0, 1, 0);
     const double u10 = U(0, 0, -1) - U(0, 0, 0), u12 = U(0, 0, 0) - U(0, 0, 1);
@@ -67,43 +121,39 @@ k_momentum(const double *__restrict__ u, const double *__restrict__ v, const dou
     const double w6 = W(-1, 0, 0) - W(0, 0, 0), w8 = W(0, 0, 0) - W(1, 0, 0);
     const double w10 = W(0, -1, 0) - W(0, 0, 0), w12 = W(0, 0, 0) - W(0, 1, 0);
     const double w14 = W(0, 0, -1) - W(0, 0, 0), w16 = W(0, 0, 0) - W(0, 0, 1);
-    const double dpdx = (P(0, 0, 0) - P(1, 0, 0)) / (0.5 * (dxr + dx));
-    const double dpdy = (P(0, 0, 0) - P(0, 1, 0)) / (0.5 * (dye + dy));
-    const double dpdz = (P(0, 0, 0) - P(0, 0, 1)) / (0.5 * (dzt + dz));
-    const double dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dy2yw = dy + dyw;
-    const double dz2zt = dz + dzt, dz2zb = dz + dzb;
+    const double dpdx = dv<FAST>(P(0, 0, 0) - P(1, 0, 0), MX(M_H1), MX(M_RH1));
+    const double dpdy = dv<FAST>(P(0, 0, 0) - P(0, 1, 0), MY(M_H1), MY(M_RH1));
+    const double dpdz = dv<FAST>(P(0, 0, 0) - P(0, 0, 1), MZ(M_H1), MZ(M_RH1));
     const bool no_ib = CL(1, 0, 0) != 2 && CL(-1, 0, 0) != 2 && CL(0, 1, 0) != 2 && CL(0, -1, 0) != 2 &&
                        CL(0, 0, 1) != 2 && CL(0, 0, -1) != 2;
+    const int mx = M.mn[0], my = M.mn[1], mz = M.mn[2];
 
     // U momentum (subNavierStokes.f90:792-897)
     {
-        const double vu_e = V(1, 0, 0) + (dxr / dx2xr) * (V(0, 0, 0) - V(1, 0, 0));
-        const double vu_w = V(1, -1, 0) + (dxr / dx2xr) * (V(0, -1, 0) - V(1, -1, 0));
+        const double ax = MX(M_AS);
+        const double vu_e = V(1, 0, 0) + ax * (V(0, 0, 0) - V(1, 0, 0));
+        const double vu_w = V(1, -1, 0) + ax * (V(0, -1, 0) - V(1, -1, 0));
         const double v_in_um = 0.5 * (vu_e + vu_w);
-        const double wu_n = W(1, 0, 0) + (dxr / dx2xr) * (W(0, 0, 0) - W(1, 0, 0));
-        const double wu_s = W(1, 0, -1) + (dxr / dx2xr) * (W(0, 0, -1) - W(1, 0, -1));
+        const double wu_n = W(1, 0, 0) + ax * (W(0, 0, 0) - W(1, 0, 0));
+        const double wu_s = W(1, 0, -1) + ax * (W(0, 0, -1) - W(1, 0, -1));
         const double w_in_um = 0.5 * (wu_n + wu_s);
         double a, b, d;
         if (i != 2 && i < nx && j != 2 && j < ny + 1 && k != 2 && k < nz + 1 && no_ib) {
-            const double ddy = 0.5 * (dya[j] + dya[j - 1]), ddye = 0.5 * (dya[j] + dya[j + 1]);
-            const double ddz = 0.5 * (dza[k] + dza[k - 1]), ddzr = 0.5 * (dza[k] + dza[k + 1]);
             const double uc = U(0, 0, 0);
-            a = kk3(T, UU, i, uc, U(2, 0, 0), U(1, 0, 0), uc, U(-1, 0, 0), U(-2, 0, 0), dxa[i + 1], dxa[i]);
-            b = kk3(T, VU, j, v_in_um, U(0, 2, 0), U(0, 1, 0), uc, U(0, -1, 0), U(0, -2, 0), ddye, ddy);
-            d = kk3(T, WU, k, w_in_um, U(0, 0, 2), U(0, 0, 1), uc, U(0, 0, -1), U(0, 0, -2), ddzr, ddz);
+            a = kk3<FAST>(T, UU, i, uc, U(2, 0, 0), U(1, 0, 0), uc, U(-1, 0, 0), U(-2, 0, 0), MX(M_KS1), MX(M_RKS1),
+                          MX(M_KS2), MX(M_RKS2));
+            b = kk3<FAST>(T, VU, j, v_in_um, U(0, 2, 0), U(0, 1, 0), uc, U(0, -1, 0), U(0, -2, 0), MY(M_KC1),
+                          MY(M_RKC1), MY(M_KC2), MY(M_RKC2));
+            d = kk3<FAST>(T, WU, k, w_in_um, U(0, 0, 2), U(0, 0, 1), uc, U(0, 0, -1), U(0, 0, -2), MZ(M_KC1),
+                          MZ(M_RKC1), MZ(M_KC2), MZ(M_RKC2));
         } else {
-            const double r1x = dxa[i] / dxa[i + 1];
-            const double r1yn = 0.5 * (dya[j] + dya[j - 1]), r1yd = 0.5 * (dya[j] + dya[j + 1]);
-            const double r1y = r1yn / r1yd;
-            const double r1zn = 0.5 * (dza[k] + dza[k - 1]), r1zd = 0.5 * (dza[k] + dza[k + 1]);
-            const double r1z = r1zn / r1zd;
-            a = up1(U(0, 0, 0), dxa[i + 1], r1x, U(-1, 0, 0), U(0, 0, 0), U(1, 0, 0));
-            b = up1(v_in_um, r1yd, r1y, U(0, -1, 0), U(0, 0, 0), U(0, 1, 0));
-            d = up1(w_in_um, r1zd, r1z, U(0, 0, -1), U(0, 0, 0), U(0, 0, 1));
+            a = up1<FAST>(&MX(M_US), mx, U(0, 0, 0), MX(M_D1), MX(M_RD1), U(-1, 0, 0), U(0, 0, 0), U(1, 0, 0));
+            b = up1<FAST>(&MY(M_UC), my, v_in_um, MY(M_H1), MY(M_RH1), U(0, -1, 0), U(0, 0, 0), U(0, 1, 0));
+            d = up1<FAST>(&MZ(M_UC), mz, w_in_um, MZ(M_H1), MZ(M_RH1), U(0, 0, -1), U(0, 0, 0), U(0, 0, 1));
         }
-        const double d2x = (2.0 / dx2xr) * ((-u4 / dxr) + (u22 / dx));
-        const double d2y = (2.0 / (0.5 * (dy2ye + dy2yw))) * ((-u8 / (0.5 * dy2ye)) + (u6 / (0.5 * dy2yw)));
-        const double d2z = (2.0 / (0.5 * (dz2zt + dz2zb))) * ((-u12 / (0.5 * dz2zt)) + (u10 / (0.5 * dz2zb)));
+        const double d2x = MX(M_CS) * (dv<FAST>(-u4, MX(M_D1), MX(M_RD1)) + dv<FAST>(u22, MX(M_D0), MX(M_RD0)));
+        const double d2y = MY(M_CC) * (dv<FAST>(-u8, MY(M_H1), MY(M_RH1)) + dv<FAST>(u6, MY(M_H0), MY(M_RH0)));
+        const double d2z = MZ(M_CC) * (dv<FAST>(-u12, MZ(M_H1), MZ(M_RH1)) + dv<FAST>(u10, MZ(M_H0), MZ(M_RH0)));
         const double xtt2 = rev * ((d2x + d2y) + d2z);
         const double res = ((-a - b) - d) + xtt2;
         ut[c] = U(0, 0, 0) + deltat * (res + dpdx);
@@ -111,33 +161,30 @@ k_momentum(const double *__restrict__ u, const double *__restrict__ v, const dou
     }
     // V momentum (subNavierStokes.f90:901-1006)
     {
-        const double uv_e = U(0, 1, 0) + (dye / dy2ye) * (U(0, 0, 0) - U(0, 1, 0));
-        const double uv_w = U(-1, 1, 0) + (dye / dy2ye) * (U(-1, 0, 0) - U(-1, 1, 0));
+        const double ay = MY(M_AS);
+        const double uv_e = U(0, 1, 0) + ay * (U(0, 0, 0) - U(0, 1, 0));
+        const double uv_w = U(-1, 1, 0) + ay * (U(-1, 0, 0) - U(-1, 1, 0));
         const double u_in_vm = 0.5 * (uv_e + uv_w);
-        const double wv_n = W(0, 1, 0) + (dye / dy2ye) * (W(0, 0, 0) - W(0, 1, 0));
-        const double wv_s = W(0, 1, -1) + (dye / dy2ye) * (W(0, 0, -1) - W(0, 1, -1));
+        const double wv_n = W(0, 1, 0) + ay * (W(0, 0, 0) - W(0, 1, 0));
+        const double wv_s = W(0, 1, -1) + ay * (W(0, 0, -1) - W(0, 1, -1));
         const double w_in_vm = 0.5 * (wv_n + wv_s);
         double a, b, d;
         if (i != 2 && i < nx + 1 && j != 2 && j < ny && k != 2 && k < nz + 1 && no_ib) {
-            const double ddx = 0.5 * (dxa[i] + dxa[i - 1]), ddxr = 0.5 * (dxa[i] + dxa[i + 1]);
-            const double ddz = 0.5 * (dza[k] + dza[k - 1]), ddzr = 0.5 * (dza[k] + dza[k + 1]);
             const double vc = V(0, 0, 0);
-            a = kk3(T, UV, i, u_in_vm, V(2, 0, 0), V(1, 0, 0), vc, V(-1, 0, 0), V(-2, 0, 0), ddxr, ddx);
-            b = kk3(T, VV, j, vc, V(0, 2, 0), V(0, 1, 0), vc, V(0, -1, 0), V(0, -2, 0), dya[j + 1], dya[j]);
-            d = kk3(T, WU, k, w_in_vm, V(0, 0, 2), V(0, 0, 1), vc, V(0, 0, -1), V(0, 0, -2), ddzr, ddz);
+            a = kk3<FAST>(T, UV, i, u_in_vm, V(2, 0, 0), V(1, 0, 0), vc, V(-1, 0, 0), V(-2, 0, 0), MX(M_KC1),
+                          MX(M_RKC1), MX(M_KC2), MX(M_RKC2));
+            b = kk3<FAST>(T, VV, j, vc, V(0, 2, 0), V(0, 1, 0), vc, V(0, -1, 0), V(0, -2, 0), MY(M_KS1), MY(M_RKS1),
+                          MY(M_KS2), MY(M_RKS2));
+            d = kk3<FAST>(T, WU, k, w_in_vm, V(0, 0, 2), V(0, 0, 1), vc, V(0, 0, -1), V(0, 0, -2), MZ(M_KC1),
+                          MZ(M_RKC1), MZ(M_KC2), MZ(M_RKC2));
         } else {
-            const double r1xn = 0.5 * (dxa[i] + dxa[i - 1]), r1xd = 0.5 * (dxa[i] + dxa[i + 1]);
-            const double r1x = r1xn / r1xd;
-            const double r1y = dya[j] / dya[j + 1];
-            const double r1zn = 0.5 * (dza[k] + dza[k - 1]), r1zd = 0.5 * (dza[k] + dza[k + 1]);
-            const double r1z = r1zn / r1zd;
-            a = up1(u_in_vm, r1xd, r1x, V(-1, 0, 0), V(0, 0, 0), V(1, 0, 0));
-            b = up1(V(0, 0, 0), dya[j + 1], r1y, V(0, -1, 0), V(0, 0, 0), V(0, 1, 0));
-            d = up1(w_in_vm, r1zd, r1z, V(0, 0, -1), V(0, 0, 0), V(0, 0, 1));
+            a = up1<FAST>(&MX(M_UC), mx, u_in_vm, MX(M_H1), MX(M_RH1), V(-1, 0, 0), V(0, 0, 0), V(1, 0, 0));
+            b = up1<FAST>(&MY(M_US), my, V(0, 0, 0), MY(M_D1), MY(M_RD1), V(0, -1, 0), V(0, 0, 0), V(0, 1, 0));
+            d = up1<FAST>(&MZ(M_UC), mz, w_in_vm, MZ(M_H1), MZ(M_RH1), V(0, 0, -1), V(0, 0, 0), V(0, 0, 1));
         }
-        const double d2x = (2.0 / (0.5 * (dx2xr + dx2xl))) * ((-v6 / (0.5 * dx2xr)) + (v4 / (0.5 * dx2xl)));
-        const double d2y = (2.0 / dy2ye) * ((-v10 / dye) + (v8 / dy));
-        const double d2z = (2.0 / (0.5 * (dz2zt + dz2zb))) * ((-v14 / (0.5 * dz2zt)) + (v12 / (0.5 * dz2zb)));
+        const double d2x = MX(M_CC) * (dv<FAST>(-v6, MX(M_H1), MX(M_RH1)) + dv<FAST>(v4, MX(M_H0), MX(M_RH0)));
+        const double d2y = MY(M_CS) * (dv<FAST>(-v10, MY(M_D1), MY(M_RD1)) + dv<FAST>(v8, MY(M_D0), MY(M_RD0)));
+        const double d2z = MZ(M_CC) * (dv<FAST>(-v14, MZ(M_H1), MZ(M_RH1)) + dv<FAST>(v12, MZ(M_H0), MZ(M_RH0)));
         const double ytt2 = rev * ((d2x + d2y) + d2z);
         const double res = ((-a - b) - d) + ytt2;
         vt[c] = V(0, 0, 0) + deltat * (res + dpdy);
@@ -145,33 +192,30 @@ k_momentum(const double *__restrict__ u, const double *__restrict__ v, const dou
     }
     // W momentum (subNavierStokes.f90:1010-1116)
     {
-        const double uw_e = U(0, 0, 1) + (dzt / dz2zt) * (U(0, 0, 0) - U(0, 0, 1));
-        const double uw_w = U(-1, 0, 1) + (dzt / dz2zt) * (U(-1, 0, 0) - U(-1, 0, 1));
+        const double az = MZ(M_AS);
+        const double uw_e = U(0, 0, 1) + az * (U(0, 0, 0) - U(0, 0, 1));
+        const double uw_w = U(-1, 0, 1) + az * (U(-1, 0, 0) - U(-1, 0, 1));
         const double u_in_wm = 0.5 * (uw_e + uw_w);
-        const double vw_n = V(0, 0, 1) + (dzt / dz2zt) * (V(0, 0, 0) - V(0, 0, 1));
-        const double vw_s = V(0, -1, 1) + (dzt / dz2zt) * (V(0, -1, 0) - V(0, -1, 1));
+        const double vw_n = V(0, 0, 1) + az * (V(0, 0, 0) - V(0, 0, 1));
+        const double vw_s = V(0, -1, 1) + az * (V(0, -1, 0) - V(0, -1, 1));
         const double v_in_wm = 0.5 * (vw_n + vw_s);
         double a, b, d;
         if (i != 2 && i < nx + 1 && j != 2 && j < ny + 1 && k != 2 && k < nz && no_ib) {
-            const double ddx = 0.5 * (dxa[i] + dxa[i - 1]), ddxr = 0.5 * (dxa[i] + dxa[i + 1]);
-            const double ddy = 0.5 * (dya[j] + dya[j - 1]), ddye = 0.5 * (dya[j] + dya[j + 1]);
             const double wc = W(0, 0, 0);
-            a = kk3(T, UV, i, u_in_wm, W(2, 0, 0), W(1, 0, 0), wc, W(-1, 0, 0), W(-2, 0, 0), ddxr, ddx);
-            b = kk3(T, VU, j, v_in_wm, W(0, 2, 0), W(0, 1, 0), wc, W(0, -1, 0), W(0, -2, 0), ddye, ddy);
-            d = kk3(T, WW, k, wc, W(0, 0, 2), W(0, 0, 1), wc, W(0, 0, -1), W(0, 0, -2), dza[k + 1], dza[k]);
+            a = kk3<FAST>(T, UV, i, u_in_wm, W(2, 0, 0), W(1, 0, 0), wc, W(-1, 0, 0), W(-2, 0, 0), MX(M_KC1),
+                          MX(M_RKC1), MX(M_KC2), MX(M_RKC2));
+            b = kk3<FAST>(T, VU, j, v_in_wm, W(0, 2, 0), W(0, 1, 0), wc, W(0, -1, 0), W(0, -2, 0), MY(M_KC1),
+                          MY(M_RKC1), MY(M_KC2), MY(M_RKC2));
+            d = kk3<FAST>(T, WW, k, wc, W(0, 0, 2), W(0, 0, 1), wc, W(0, 0, -1), W(0, 0, -2), MZ(M_KS1), MZ(M_RKS1),
+                          MZ(M_KS2), MZ(M_RKS2));
         } else {
-            const double r1xn = 0.5 * (dxa[i] + dxa[i - 1]), r1xd = 0.5 * (dxa[i] + dxa[i + 1]);
-            const double r1x = r1xn / r1xd;
-            const double r1yn = 0.5 * (dya[j] + dya[j - 1]), r1yd = 0.5 * (dya[j] + dya[j + 1]);
-            const double r1y = r1yn / r1yd;
-            const double r1z = dza[k] / dza[k + 1];
-            a = up1(u_in_wm, r1xd, r1x, W(-1, 0, 0), W(0, 0, 0), W(1, 0, 0));
-            b = up1(v_in_wm, r1yd, r1y, W(0, -1, 0), W(0, 0, 0), W(0, 1, 0));
-            d = up1(W(0, 0, 0), dza[k + 1], r1z, W(0, 0, -1), W(0, 0, 0), W(0, 0, 1));
+            a = up1<FAST>(&MX(M_UC), mx, u_in_wm, MX(M_H1), MX(M_RH1), W(-1, 0, 0), W(0, 0, 0), W(1, 0, 0));
+            b = up1<FAST>(&MY(M_UC), my, v_in_wm, MY(M_H1), MY(M_RH1), W(0, -1, 0), W(0, 0, 0), W(0, 1, 0));
+            d = up1<FAST>(&MZ(M_US), mz, W(0, 0, 0), MZ(M_D1), MZ(M_RD1), W(0, 0, -1), W(0, 0, 0), W(0, 0, 1));
         }
-        const double d2x = (2.0 / (0.5 * (dx2xr + dx2xl))) * ((-w8 / (0.5 * dx2xr)) + (w6 / (0.5 * dx2xl)));
-        const double d2y = (2.0 / (0.5 * (dy2ye + dy2yw))) * ((-w12 / (0.5 * dy2ye)) + (w10 / (0.5 * dy2yw)));
-        const double d2z = (2.0 / dz2zt) * ((-w16 / dzt) + (w14 / dz));
+        const double d2x = MX(M_CC) * (dv<FAST>(-w8, MX(M_H1), MX(M_RH1)) + dv<FAST>(w6, MX(M_H0), MX(M_RH0)));
+        const double d2y = MY(M_CC) * (dv<FAST>(-w12, MY(M_H1), MY(M_RH1)) + dv<FAST>(w10, MY(M_H0), MY(M_RH0)));
+        const double d2z = MZ(M_CS) * (dv<FAST>(-w16, MZ(M_D1), MZ(M_RD1)) + dv<FAST>(w14, MZ(M_D0), MZ(M_RD0)));
         const double ztt2 = rev * ((d2x + d2y) + d2z);
         const double res = ((-a - b) - d) + ztt2;
         wt[c] = W(0, 0, 0) + deltat * (res + dpdz);
@@ -182,6 +226,43 @@ k_momentum(const double *__restrict__ u, const double *__restrict__ v, const dou
 #undef W
 #undef P
 #undef CL
+#undef MX
+#undef MY
+#undef MZ
+}
+
+// Interior columns i in [3, nx-1], where the stencil choice depends on j, k
+// and the tags only.  A block is 64 x 4 with one wave per (j, k) row, so j
+// and k are wave-uniform: their table entries come through scalar loads and
+// the wall tests are scalar branches.
+template <bool FAST>
+__global__ void __launch_bounds__(256)
+k_momentum(const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
+           const double *__restrict__ p, const int8_t *__restrict__ cell, double *__restrict__ ut,
+           double *__restrict__ vt, double *__restrict__ wt, double *__restrict__ ru, double *__restrict__ rv,
+           double *__restrict__ rw, Tabs T, MomTabs M, int nx, int ny, int nz, double rev, double deltat) {
+    const int i = blockIdx.x * 64 + threadIdx.x + 3;
+    const int j = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * 4 + threadIdx.y)) + 2;
+    const int k = (int)blockIdx.z + 2;
+    if (j > ny + 1 || i > nx - 1) return;
+    momentum_cell<FAST>(i, j, k, u, v, w, p, cell, ut, vt, wt, ru, rv, rw, T, M, nx, ny, nz, rev, deltat);
+}
+
+// The wall columns i = 2, nx, nx+1 of every (j, k) row, kept out of the
+// interior waves so that those run one stencil branch.
+template <bool FAST>
+__global__ void __launch_bounds__(256)
+k_momentum_edge(const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
+                const double *__restrict__ p, const int8_t *__restrict__ cell, double *__restrict__ ut,
+                double *__restrict__ vt, double *__restrict__ wt, double *__restrict__ ru, double *__restrict__ rv,
+                double *__restrict__ rw, Tabs T, MomTabs M, int nx, int ny, int nz, double rev, double deltat) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * ny * nz) return;
+    const int s = t % 3, r = t / 3;
+    if (s == 1 && nx < 3) return;   // i = nx is then column 2 again
+    const int i = s == 0 ? 2 : nx + s - 1;
+    const int j = r % ny + 2, k = r / ny + 2;
+    momentum_cell<FAST>(i, j, k, u, v, w, p, cell, ut, vt, wt, ru, rv, rw, T, M, nx, ny, nz, rev, deltat);
 }
 
 // subBoundaryConditions.f90:141-156: inlet u = 1, reflected v and w; outlet
@@ -314,12 +395,105 @@ static Tabs tabs(const Ctx &c) {
     return T;
 }
 
+// The momentum formula's index-only sub-expressions per axis, evaluated in
+// the formula's own operations (host IEEE double, no contraction), and the
+// reciprocals of the denominators that divide field values.  The fast
+// division is enabled only if every such denominator lies in [2^-64, 2^64].
+static int build_mom_tables(Ctx &c) {
+    const std::vector<double> *dl[3] = {&c.hdx, &c.hdy, &c.hdz};
+    const int64_t nn[3] = {c.nx, c.ny, c.nz};
+    bool fast = true;
+    auto in_range = [](double d) {
+        const double m = std::fabs(d);
+        return m >= 0x1p-64 && m <= 0x1p64;
+    };
+    for (int a = 0; a < 3; a++) {
+        const int64_t n = nn[a], tn = n + 3, ftn = c.tn[a];
+        const std::vector<double> &d = *dl[a], &face = c.htab[a], &cent = c.htab[3 + a];
+        std::vector<double> h(M_NQ * tn, 0.0);
+        auto put = [&](int q, int64_t i, double x) { h[q * tn + i] = x; };
+        auto upwind = [&](int q, int64_t i, double den, double r) {
+            put(q, i, -1.0 / (r * (r + 1.0)));
+            put(q + 1, i, 1.0 - 1.0 / r);
+            put(q + 2, i, r / (r + 1.0));
+            put(q + 3, i, 2.0 / (r * (r + 1.0)));
+            put(q + 4, i, 2.0 / r);
+            put(q + 5, i, 2.0 / (r + 1.0));
+            put(q + 6, i, 1.0 / (2.0 * den));
+        };
+        for (int64_t i = 2; i <= n + 1; i++) {
+            const double d0 = d[i], d1 = d[i + 1], dm = d[i - 1];
+            const double h1 = 0.5 * (d0 + d1), h0 = 0.5 * (d0 + dm);
+            put(M_AS, i, d1 / (d0 + d1));
+            put(M_CS, i, 2.0 / (d0 + d1));
+            put(M_D1, i, d1);
+            put(M_RD1, i, 1.0 / d1);
+            put(M_D0, i, d0);
+            put(M_RD0, i, 1.0 / d0);
+            upwind(M_US, i, d1, d0 / d1);
+            put(M_CC, i, 2.0 / (0.5 * ((d0 + d1) + (d0 + dm))));
+            put(M_H1, i, h1);
+            put(M_RH1, i, 1.0 / h1);
+            put(M_H0, i, h0);
+            put(M_RH0, i, 1.0 / h0);
+            upwind(M_UC, i, h1, h0 / h1);
+            fast = fast && in_range(d0) && in_range(d1) && in_range(h0) && in_range(h1);
+            if (i >= 3 && i <= n) {   // where the Kawamura-Kuwahara stencil can apply
+                const double ks1 = face[5 * ftn + i] * d1, ks2 = (2.0 * face[11 * ftn + i]) * d0;
+                const double kc1 = cent[5 * ftn + i] * h1, kc2 = (2.0 * cent[11 * ftn + i]) * h0;
+                put(M_KS1, i, ks1);
+                put(M_RKS1, i, 1.0 / ks1);
+                put(M_KS2, i, ks2);
+                put(M_RKS2, i, 1.0 / ks2);
+                put(M_KC1, i, kc1);
+                put(M_RKC1, i, 1.0 / kc1);
+                put(M_KC2, i, kc2);
+                put(M_RKC2, i, 1.0 / kc2);
+                // the staggered component uses the face table for i <= n-1 only
+                fast = fast && in_range(kc1) && in_range(kc2) && (i == n || (in_range(ks1) && in_range(ks2)));
+            }
+        }
+        if (c.mtn[a] != tn) {
+            if (c.mtab[a]) IB3D_CHECK(hipFree(c.mtab[a]));
+            c.mtab[a] = nullptr;
+            c.mtn[a] = 0;
+            IB3D_CHECK(hipMalloc((void **)&c.mtab[a], h.size() * sizeof(double)));
+            c.mtn[a] = tn;
+        }
+        IB3D_CHECK(h2d(c.mtab[a], h.data(), h.size() * sizeof(double)));
+    }
+    c.mom_fast = fast;
+    c.mom_ready = true;
+    return 0;
+}
+
 int ns_momentum(Ctx &c, double rev, double deltat) {
     IB3D_REQUIRE(c.coeff_ready, "nsMomentum: non_uni_coeff has not run");
     IB3D_REQUIRE(c.lists_ready, "nsMomentum: cellCount has not run");
-    dim3 blk(64, 4, 1), grd(nblocks(c.nx, 64), nblocks(c.ny, 4), (unsigned)c.nz);
-    hipLaunchKernelGGL(k_momentum, grd, blk, 0, c.stream, c.u, c.v, c.w, c.p, c.tag, c.ut, c.vt, c.wt, c.resi_u,
-                       c.resi_v, c.resi_w, c.dx, c.dy, c.dz, tabs(c), (int)c.nx, (int)c.ny, (int)c.nz, rev, deltat);
+    if (!c.mom_ready && build_mom_tables(c)) return -1;
+    MomTabs M;
+    for (int a = 0; a < 3; a++) {
+        M.m[a] = c.mtab[a];
+        M.mn[a] = (int)c.mtn[a];
+    }
+    const Tabs T = tabs(c);
+    const int nx = (int)c.nx, ny = (int)c.ny, nz = (int)c.nz;
+    const dim3 blk(64, 4, 1);
+    const int nxi = std::max(0, nx - 3);   // interior columns 3..nx-1
+    const int nedge = 3 * ny * nz;
+    auto launch = [&](auto fast) {
+        constexpr bool F = decltype(fast)::value;
+        if (nxi > 0)
+            hipLaunchKernelGGL(k_momentum<F>, dim3(nblocks(nxi, 64), nblocks(ny, 4), (unsigned)nz), blk, 0,
+                               c.stream, c.u, c.v, c.w, c.p, c.tag, c.ut, c.vt, c.wt, c.resi_u, c.resi_v, c.resi_w, T,
+                               M, nx, ny, nz, rev, deltat);
+        hipLaunchKernelGGL(k_momentum_edge<F>, dim3(nblocks(nedge, 256)), dim3(256), 0, c.stream, c.u, c.v, c.w,
+                           c.p, c.tag, c.ut, c.vt, c.wt, c.resi_u, c.resi_v, c.resi_w, T, M, nx, ny, nz, rev, deltat);
+    };
+    if (c.mom_fast)
+        launch(std::true_type{});
+    else
+        launch(std::false_type{});
     IB3D_CHECK(hipGetLastError());
     return 0;
 }

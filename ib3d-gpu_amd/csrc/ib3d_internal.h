@@ -82,6 +82,11 @@ struct Ctx {
     double *tab[6] = {};
     int64_t tn[6] = {};
     bool coeff_ready = false;
+    // momentum tables per axis (index-only sub-expressions and reciprocals,
+    // built by ns_momentum from the grid and the convection tables)
+    double *mtab[3] = {};
+    int64_t mtn[3] = {};
+    bool mom_ready = false, mom_fast = false;
 
     // Laplacian weights: Acx(n,3) folded with the boundary conditions, kept
     // host-side in the reference's layout and on the device as per-index

@@ -140,3 +140,31 @@ def test_per_body_forces_match_oracle(pair):
         s.step()
         o.step(k == 0)
         np.testing.assert_allclose(s.stress_bodies(), o.stress_bodies(), rtol=1e-12, atol=1e-14)
+
+
+def test_momentum_bitwise_on_extreme_values(pair):
+    """The predictor divides field values by grid quantities through
+    precomputed reciprocals and FMA corrections; numerators outside the
+    range where that is exact (zeros of either sign, values near the
+    underflow threshold, values whose products exceed 2^900) must still give
+    the correctly rounded quotient.  Fields filled with such values agree
+    with the oracle bit for bit."""
+    s, o = pair("small")
+    nx, ny, nz = s.n
+    shape = (nx + 2, ny + 2, nz + 2)
+    rng = np.random.default_rng(11)
+    for name, base in [("u", 1.0), ("v", 0.0), ("w", 0.0), ("p", 0.0)]:
+        a = base + 0.3 * rng.standard_normal(shape)
+        m = rng.random(shape)
+        a[m < 0.10] = 0.0
+        a[(m >= 0.10) & (m < 0.15)] = -0.0
+        a[(m >= 0.15) & (m < 0.17)] = 1e-300 * np.sign(a[(m >= 0.15) & (m < 0.17)] + 0.5)
+        a[(m >= 0.17) & (m < 0.18)] = 5e-324
+        a[(m >= 0.18) & (m < 0.19)] = 1e150
+        a[:, :, 3:6] = a[:, :, 2:3]          # equal planes: zero differences
+        s.upload(name, a)
+        o.field(name)[...] = a
+    _say("[gpu] case=small momentum on extreme values")
+    s.nsMomentum(); o.call("ns_momentum")
+    for f in ["ut", "vt", "wt", "resi_u", "resi_v", "resi_w"]:
+        assert_bits(f"extreme nsMomentum {f}", s.download(f), o.array(f))

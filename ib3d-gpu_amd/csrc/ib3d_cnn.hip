@@ -14,11 +14,13 @@
 // then iterates to the same tolerance either way.
 //
 // Everything runs in hand-written kernels:
-//   k_conv_lds implicit-GEMM convolution on MFMA (v_mfma_f32_16x16x32_bf16):
-//              D[16 out-ch][16 voxels] = W[16 x 27*Cin] * im2col[27*Cin x 16];
-//              BF16 weights and activations, FP32 accumulation, the input
-//              tile staged in LDS.  With FLIP and transposed weights the same
-//              kernel is the backward-data convolution.
+//   k_conv_zm  forward implicit-GEMM convolution on MFMA
+//              (v_mfma_f32_16x16x32_bf16): D[16 out-ch][16 voxels] =
+//              W[16 x 27*Cin] * im2col[27*Cin x 16], BF16 weights and
+//              activations, FP32 accumulation; input planes streamed along z
+//              through LDS, each plane feeding all three z-taps.
+//   k_conv_lds the same product over a tile staged in LDS; with FLIP and
+//              transposed weights it is the backward-data convolution.
 //   k_conv_dw  weight gradient on MFMA: D[16 out][16 in] per tap =
 //              dZ^T[16 x 32 voxels] * X_tap[32 voxels x 16], both operands
 //              staged transposed in LDS; deterministic per-block partials.
@@ -33,6 +35,7 @@
 #include <algorithm>
 #include <cmath>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 namespace ib3d {
@@ -46,7 +49,7 @@ static constexpr int kCin[4] = {3, 16, 16, 16};
 static constexpr int kCout[4] = {16, 16, 16, 1};
 static constexpr int kWoff[4] = {0, 1312, 8240, 15168};
 static constexpr int kBoff[4] = {1296, 8224, 15152, 15600};
-static constexpr int kNkc[4] = {7, 14, 14, 14};
+static constexpr int kNkc[4] = {9, 15, 15, 15};   // forward K chunks (taps grouped by z plane)
 static constexpr int kTile = 64, kHalo = 4, kTileIn = kTile + 2 * kHalo;
 
 struct Vol {
@@ -58,7 +61,9 @@ __device__ __forceinline__ int64_t vox_index(const Vol &v, int b, int z, int y, 
 }
 
 // ------------------------------------------------------------------ packing
-// forward pack: row m = output channel, k = (tap, input channel)
+// forward pack: row m = output channel, k = (tap, input channel); chunk
+// kc = dz*cpz + c holds in-plane taps 2c, 2c+1 (16 stored channels) or
+// 4c .. 4c+3 (8 stored channels) of z-tap dz, padded with zeros past tap 8
 __global__ void k_pack_fwd(const float *__restrict__ prm, int woff, int boff, int cin, int cout, int cinst, int nkc,
                            __bf16 *__restrict__ out, float *__restrict__ bias16) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,15 +71,17 @@ __global__ void k_pack_fwd(const float *__restrict__ prm, int woff, int boff, in
     if (idx >= nkc * 512) return;
     const int j = idx & 7, l = (idx >> 3) & 63, kc = idx >> 9;
     const int m = l & 15, g = l >> 4;
-    int t, ci;
+    const int cpz = nkc / 3, dz = kc / cpz, c = kc - dz * cpz;
+    int ip, ci;
     if (cinst == 16) {
-        t = 2 * kc + (g >> 1);
+        ip = 2 * c + (g >> 1);
         ci = 8 * (g & 1) + j;
     } else {
-        t = 4 * kc + g;
+        ip = 4 * c + g;
         ci = j;
     }
-    const float w = (t < 27 && ci < cin && m < cout) ? prm[woff + (t * cin + ci) * cout + m] : 0.0f;
+    const int t = 9 * dz + ip;
+    const float w = (ip < 9 && ci < cin && m < cout) ? prm[woff + (t * cin + ci) * cout + m] : 0.0f;
     out[idx] = (__bf16)w;
 }
 
@@ -91,9 +98,10 @@ __global__ void k_pack_bwd(const float *__restrict__ prm, int woff, int cin, int
 }
 
 // ------------------------------------------------------------ convolution
-// MODE 0: + bias, ReLU, 16 BF16 channels out.  MODE 1: + bias, channel 0 as
-// FP32 (the output layer).  MODE 2: times ReLU'(gate) (backward data), 16
-// BF16 channels out.  FLIP: source offset -d instead of +d.
+// Tile-staged form, used for the backward-data pass (MODE 2: times
+// ReLU'(gate), 16 BF16 channels out; FLIP: source offset -d instead of +d).
+// MODE 0 (+ bias, ReLU) and MODE 1 (+ bias, channel 0 as FP32) are the
+// forward epilogues; the forward pass itself runs in k_conv_zm below.
 //
 // A 256-thread workgroup computes 16 (x) x 4 (y) x 4 (z) output voxels.  It
 // loads their 18 x 6 x 6 input neighbourhood once into LDS (zero outside the
@@ -186,6 +194,186 @@ k_conv_lds(const __bf16 *__restrict__ X, const bf16x8 *__restrict__ W, const flo
                 *reinterpret_cast<bf16x4 *>(Y16 + vo * 16 + 4 * g) = ov;
             }
         }
+    }
+}
+
+// Forward convolution marching along z (MODE 0: + bias, ReLU, 16 BF16
+// channels out; MODE 1: + bias, channel 0 as FP32).  A 256-thread workgroup
+// owns a 16 (x) x 16 (y) column of output voxels over kZmZ consecutive
+// planes and streams the input planes z0-1 .. z1 through a two-slot LDS
+// ring (18 x 18 voxels: the one-voxel x-y halo; zero outside the volume).
+// Each input plane is consumed once: the B operand of every in-plane tap is
+// read from LDS once and multiplied by the weights of all three z-taps, into
+// the accumulators of the output planes p-1, p and p+1 (so LDS delivers a
+// third of the operand bytes a per-output-plane im2col would).  The K
+// chunks group the taps by plane: chunk dz*CPZ + c holds in-plane taps
+// 2c, 2c+1 x 16 channels (4c .. 4c+3 x 8 channels for the 8-channel input),
+// the tenth (twelfth) slot padded with zero weights.  While a plane's MFMAs
+// run, the next two planes are in flight in registers (a plane's loads have
+// two planes of work to arrive); the nearer one goes to the other slot
+// afterwards, behind one barrier per plane.  Wave w computes output rows
+// y0+4w .. y0+4w+3.  A staged plane is split by channel half ([half][voxel]
+// [8 channels]) so the 16 lanes reading one row's voxels touch consecutive
+// 16-byte words.
+static constexpr int kZmX = 16, kZmY = 16, kZmZ = 16;
+static constexpr int kZpX = kZmX + 2, kZpV = kZpX * (kZmY + 2);
+
+template <int CIN, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_conv_zm(const __bf16 *__restrict__ X, const bf16x8 *__restrict__ W, const float *__restrict__ bias, Vol v,
+          __bf16 *__restrict__ Y16, float *__restrict__ Y32, int nzc) {
+    constexpr int CPZ = CIN == 16 ? 5 : 3;   // K chunks per z-tap (9 in-plane taps)
+    constexpr int NKC = 3 * CPZ;
+    constexpr int VU = CIN / 8;              // 16-byte units per voxel
+    constexpr int PE = kZpV * CIN;           // BF16 elements per staged plane
+    constexpr int PU = kZpV * VU;            // 16-byte units per staged plane
+    constexpr int UPT = (PU + 255) / 256;
+    __shared__ __attribute__((aligned(16))) __bf16 ring[2 * PE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, g = lane >> 4;
+    const int x0 = (int)blockIdx.x * kZmX, y0 = (int)blockIdx.y * kZmY;
+    const int b = (int)blockIdx.z / nzc, z0 = ((int)blockIdx.z - b * nzc) * kZmZ;
+    const int z1 = min(z0 + kZmZ, v.vz);
+    const int64_t pstride = (int64_t)v.vy * v.vx * CIN;
+    const __bf16 *Xb = X + (int64_t)b * v.vz * pstride;
+
+    // this thread's staging units, the same in every plane
+    int soff[UPT];
+    int64_t goff[UPT];   // clamped into the plane; gok: inside the volume in x / y
+    bool gok[UPT];
+#pragma unroll
+    for (int u = 0; u < UPT; u++) {
+        const int unit = (int)threadIdx.x + 256 * u;
+        const int vox = unit / VU, part = unit - vox * VU;
+        const int xs = x0 - 1 + vox % kZpX, ys = y0 - 1 + vox / kZpX;
+        soff[u] = unit < PU ? part * (kZpV * 8) + vox * 8 : -1;
+        gok[u] = unit < PU && xs >= 0 && xs < v.vx && ys >= 0 && ys < v.vy;
+        goff[u] = ((int64_t)min(max(ys, 0), v.vy - 1) * v.vx + min(max(xs, 0), v.vx - 1)) * CIN + 8 * part;
+    }
+    // planes p+1 and p+2 are in flight in rg[(p+1)&1] and rg[p&1] while
+    // plane p is consumed (the parity is a template argument, so no copy
+    // between the sets waits for a load)
+    bf16x8 rg[2][UPT];
+    // loads without control flow (clamped address, zero selected after), so
+    // that the wait for a plane leaves the later planes' loads in flight
+    auto load_plane = [&](int zs, bf16x8 *r) {
+        const bool zok = zs >= 0 && zs < v.vz;
+        const __bf16 *pl = Xb + min(max(zs, 0), v.vz - 1) * pstride;
+#pragma unroll
+        for (int u = 0; u < UPT; u++) {
+            const bf16x8 val = *reinterpret_cast<const bf16x8 *>(pl + goff[u]);
+            r[u] = (gok[u] && zok) ? val : bf16x8{};
+        }
+    };
+    auto store_plane = [&](int slot, const bf16x8 *r) {
+#pragma unroll
+        for (int u = 0; u < UPT; u++)
+            if (soff[u] >= 0) *reinterpret_cast<bf16x8 *>(ring + slot * PE + soff[u]) = r[u];
+    };
+
+    bf16x8 wr[NKC];
+#pragma unroll
+    for (int kc = 0; kc < NKC; kc++) wr[kc] = W[kc * 64 + lane];
+    int toff[CPZ];
+#pragma unroll
+    for (int c = 0; c < CPZ; c++) {
+        const int ip = CIN == 16 ? 2 * c + (g >> 1) : 4 * c + g;
+        const int part = CIN == 16 ? (g & 1) : 0;
+        const int ipp = ip < 9 ? ip : 4;   // padded slots (zero weights) read the centre
+        toff[c] = part * (kZpV * 8) + ((ipp / 3 - 1) * kZpX + (ipp % 3 - 1)) * 8;
+    }
+    int rowoff[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) rowoff[rr] = ((4 * wave + rr + 1) * kZpX + n + 1) * 8;
+    float bv[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) bv[i] = MODE == 0 ? bias[4 * g + i] : bias[0];
+
+    // Three accumulator sets hold the output planes p-1, p, p+1 while input
+    // plane p is consumed; the plane loop is unrolled by three so that their
+    // roles rotate by renaming.  Every plane runs all three z-taps: the
+    // results for planes outside z0 .. z1-1 (at the two ends of the column)
+    // are computed and dropped, which keeps the matrix work free of branches.
+    f32x4 acc[3][4];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) acc[q][rr] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    int cur = 0;
+    auto plane = [&](int p, auto role, auto parity) {
+        constexpr int M = decltype(role)::value, Z = (M + 1) % 3, P = (M + 2) % 3;
+        constexpr int Q = decltype(parity)::value;
+        load_plane(p + 2, rg[Q]);   // past z1 (unused) when p + 2 > z1
+        const __bf16 *src = ring + cur * PE;
+        // the B operands of chunk c+1 are read while chunk c's 12 MFMAs run
+        bf16x8 xb[2][4];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) xb[0][rr] = *reinterpret_cast<const bf16x8 *>(src + toff[0] + rowoff[rr]);
+#pragma unroll
+        for (int c = 0; c < CPZ; c++) {
+            if (c + 1 < CPZ) {
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++)
+                    xb[(c + 1) & 1][rr] = *reinterpret_cast<const bf16x8 *>(src + toff[c + 1] + rowoff[rr]);
+            }
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const bf16x8 xv = xb[c & 1][rr];
+                acc[M][rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[2 * CPZ + c], xv, acc[M][rr], 0, 0, 0);
+                acc[Z][rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[CPZ + c], xv, acc[Z][rr], 0, 0, 0);
+                acc[P][rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[c], xv, acc[P][rr], 0, 0, 0);
+            }
+        }
+        if (p - 1 >= z0) {   // output plane p-1 has all three z-taps
+            const int x = x0 + n, z = p - 1;
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int y = y0 + 4 * wave + rr;
+                if (x < v.vx && y < v.vy) {
+                    const int64_t vo = vox_index(v, b, z, y, x);
+                    if (MODE == 1) {
+                        if (g == 0) Y32[vo] = acc[M][rr][0] + bv[0];
+                    } else {
+                        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                        bf16x4 ov;
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const float val = acc[M][rr][i] + bv[i];
+                            ov[i] = (__bf16)(val > 0.0f ? val : 0.0f);
+                        }
+                        *reinterpret_cast<bf16x4 *>(Y16 + vo * 16 + 4 * g) = ov;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) acc[M][rr] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};   // becomes plane p+2
+        // plane p+1 goes to the other slot, whose plane p-1 every wave has
+        // finished reading before the previous barrier
+        if (p < z1) store_plane(cur ^ 1, rg[Q ^ 1]);
+        cur ^= 1;
+        __syncthreads();
+    };
+
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    load_plane(z0 - 1, rg[0]);
+    store_plane(0, rg[0]);
+    load_plane(z0, rg[1]);
+    __syncthreads();
+    for (int p = z0 - 1; p <= z1; p += 6) {   // roles cycle by 3, register sets by 2
+        plane(p, I0{}, I0{});
+        if (p + 1 > z1) break;
+        plane(p + 1, I1{}, I1{});
+        if (p + 2 > z1) break;
+        plane(p + 2, I2{}, I0{});
+        if (p + 3 > z1) break;
+        plane(p + 3, I0{}, I1{});
+        if (p + 4 > z1) break;
+        plane(p + 4, I1{}, I0{});
+        if (p + 5 > z1) break;
+        plane(p + 5, I2{}, I1{});
     }
 }
 
@@ -400,25 +588,23 @@ __global__ void __launch_bounds__(256)
 k_cnn_scales(const float *__restrict__ r0, const float *__restrict__ r1, const double *__restrict__ p0,
              const double *__restrict__ p1, const double *__restrict__ b0, const double *__restrict__ b1,
              const double *__restrict__ ax, const double *__restrict__ ay, const double *__restrict__ az, int sx,
-             int sy, int sz, int sxp, int64_t H, double *__restrict__ partial) {
+             int sy, int sz, int sxp, int H, FastDiv dsx, FastDiv dsy, double *__restrict__ partial) {
     __shared__ double sm[3][4];
-    const int64_t P = (int64_t)sx * sy * sz;
-    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int P = sx * sy * sz;
     double a = 0.0, bb = 0.0, cnt = 0.0;
-    if (idx < P) {
-        const int ii = (int)(idx % sx);
-        const int64_t row = idx / sx;
-        const int jj = (int)(row % sy), kk = (int)(row / sy);
+    for (int idx = (int)blockIdx.x * 256 + (int)threadIdx.x; idx < P; idx += (int)gridDim.x * 256) {
+        const int row = (int)fdiv((uint32_t)idx, dsx), ii = idx - row * sx;
+        const int kk = (int)fdiv((uint32_t)row, dsy), jj = row - kk * sy;
         const int colour = (ii + jj + kk) & 1;
-        const int64_t f = row * H + (ii >> 1);
+        const int64_t f = (int64_t)row * H + (ii >> 1);
         const double bd = (colour ? b1 : b0)[f];
         if (bd == bd) {
             const double diag = -((ax[sxp + ii] + ay[sy + jj]) + az[sz + kk]);
             const double rr = (double)(colour ? r1 : r0)[f] / diag;
             const double pp = (colour ? p1 : p0)[f];
-            a = rr * rr;
-            bb = pp * pp;
-            cnt = 1.0;
+            a += rr * rr;
+            bb += pp * pp;
+            cnt += 1.0;
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -465,15 +651,15 @@ __global__ void __launch_bounds__(256)
 k_cnn_input(const float *__restrict__ r0, const float *__restrict__ r1, const double *__restrict__ p0,
             const double *__restrict__ p1, const double *__restrict__ b0, const double *__restrict__ b1,
             const double *__restrict__ ax, const double *__restrict__ ay, const double *__restrict__ az, int sx,
-            int sy, int sz, int sxp, int64_t H, const double *__restrict__ scal, __bf16 *__restrict__ X) {
-    const int64_t P = (int64_t)sx * sy * sz;
-    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+            int sy, int sz, int sxp, int H, FastDiv dsx, FastDiv dsy, const double *__restrict__ scal,
+            __bf16 *__restrict__ X) {
+    const int P = sx * sy * sz;
+    const int idx = (int)blockIdx.x * 256 + (int)threadIdx.x;
     if (idx >= P) return;
-    const int ii = (int)(idx % sx);
-    const int64_t row = idx / sx;
-    const int jj = (int)(row % sy), kk = (int)(row / sy);
+    const int row = (int)fdiv((uint32_t)idx, dsx), ii = idx - row * sx;
+    const int kk = (int)fdiv((uint32_t)row, dsy), jj = row - kk * sy;
     const int colour = (ii + jj + kk) & 1;
-    const int64_t f = row * H + (ii >> 1);
+    const int64_t f = (int64_t)row * H + (ii >> 1);
     const double bd = (colour ? b1 : b0)[f];
     bf16x8 o = {};
     if (bd == bd) {
@@ -482,22 +668,22 @@ k_cnn_input(const float *__restrict__ r0, const float *__restrict__ r1, const do
         o[1] = (__bf16)1.0f;
         o[2] = (__bf16)(float)((colour ? p1 : p0)[f] * scal[3]);
     }
-    *reinterpret_cast<bf16x8 *>(X + idx * 8) = o;
+    *reinterpret_cast<bf16x8 *>(X + (int64_t)idx * 8) = o;
 }
 
 // guess = p_prev + s_r * net on fluid cells (both colours)
 __global__ void k_cnn_apply(const float *__restrict__ net, const double *__restrict__ pp0,
                             const double *__restrict__ pp1, const double *__restrict__ b0,
                             const double *__restrict__ b1, double *__restrict__ pc0, double *__restrict__ pc1,
-                            int sx, int sy, int sz, int64_t H, const double *__restrict__ scal) {
-    const int64_t P = (int64_t)sx * sy * sz;
-    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+                            int sx, int sy, int sz, int H, FastDiv dsx, FastDiv dsy,
+                            const double *__restrict__ scal) {
+    const int P = sx * sy * sz;
+    const int idx = (int)blockIdx.x * 256 + (int)threadIdx.x;
     if (idx >= P) return;
-    const int ii = (int)(idx % sx);
-    const int64_t row = idx / sx;
-    const int jj = (int)(row % sy), kk = (int)(row / sy);
+    const int row = (int)fdiv((uint32_t)idx, dsx), ii = idx - row * sx;
+    const int kk = (int)fdiv((uint32_t)row, dsy), jj = row - kk * sy;
     const int colour = (ii + jj + kk) & 1;
-    const int64_t f = row * H + (ii >> 1);
+    const int64_t f = (int64_t)row * H + (ii >> 1);
     const double bd = (colour ? b1 : b0)[f];
     if (bd == bd) (colour ? pc1 : pc0)[f] = (colour ? pp1 : pp0)[f] + scal[0] * (double)net[idx];
 }
@@ -653,18 +839,17 @@ static dim3 conv_grid(const Vol &v, int *ntz) {
 static void launch_fwd(int L, const Vol &v, const __bf16 *in, __bf16 *out16, float *out32) {
     Net &N = g_net;
     hipStream_t s = ctx().stream;
-    int ntz;
-    const dim3 grid = conv_grid(v, &ntz), blk(256);
+    const int nzc = (v.vz + kZmZ - 1) / kZmZ;
+    const dim3 grid((unsigned)((v.vx + kZmX - 1) / kZmX), (unsigned)((v.vy + kZmY - 1) / kZmY),
+                    (unsigned)(nzc * v.nb)),
+        blk(256);
     const bf16x8 *W = reinterpret_cast<const bf16x8 *>(N.wf[L]);
     if (L == 0)
-        hipLaunchKernelGGL((k_conv_lds<8, false, 0>), grid, blk, 0, s, in, W, N.bias, v, out16, nullptr, nullptr,
-                           ntz);
+        hipLaunchKernelGGL((k_conv_zm<8, 0>), grid, blk, 0, s, in, W, N.bias, v, out16, nullptr, nzc);
     else if (L < 3)
-        hipLaunchKernelGGL((k_conv_lds<16, false, 0>), grid, blk, 0, s, in, W, N.bias + 16 * L, v, out16, nullptr,
-                           nullptr, ntz);
+        hipLaunchKernelGGL((k_conv_zm<16, 0>), grid, blk, 0, s, in, W, N.bias + 16 * L, v, out16, nullptr, nzc);
     else
-        hipLaunchKernelGGL((k_conv_lds<16, false, 1>), grid, blk, 0, s, in, W, N.bias + 48, v, nullptr, out32,
-                           nullptr, ntz);
+        hipLaunchKernelGGL((k_conv_zm<16, 1>), grid, blk, 0, s, in, W, N.bias + 48, v, nullptr, out32, nzc);
 }
 
 static void launch_bwd_data(int L, const Vol &v, const __bf16 *dz, const __bf16 *gate, __bf16 *out) {
@@ -749,14 +934,17 @@ static int ensure_pprev(Ctx &c) {
 // into g_net.dsc[0..3]
 static int build_input(Ctx &c, __bf16 *dst) {
     Net &N = g_net;
-    const int64_t nb = nblocks(c.P, 256);
-    if (ensure_partial(3 * nb)) return -1;
-    hipLaunchKernelGGL(k_cnn_scales, dim3(nb), dim3(256), 0, c.stream, c.r32[0], c.r32[1], c.pc[0], c.pc[1],
-                       c.bdt[0], c.bdt[1], c.ax, c.ay, c.az, (int)c.sx, (int)c.sy, (int)c.sz, (int)c.sxp, c.H,
-                       c.partial);
-    hipLaunchKernelGGL(k_cnn_scales_finish, dim3(1), dim3(256), 0, c.stream, c.partial, nb, N.dsc);
+    IB3D_REQUIRE(c.P < (int64_t)1 << 31, "cnn: the grid must hold fewer than 2^31 cells");
+    const FastDiv dsx = make_fastdiv((int)c.sx), dsy = make_fastdiv((int)c.sy);
+    const int64_t nb = nblocks(c.P, 256), nr = std::min<int64_t>(nb, 1024);
+    if (ensure_partial(3 * nr)) return -1;
+    hipLaunchKernelGGL(k_cnn_scales, dim3(nr), dim3(256), 0, c.stream, c.r32[0], c.r32[1], c.pc[0], c.pc[1],
+                       c.bdt[0], c.bdt[1], c.ax, c.ay, c.az, (int)c.sx, (int)c.sy, (int)c.sz, (int)c.sxp, (int)c.H,
+                       dsx, dsy, c.partial);
+    hipLaunchKernelGGL(k_cnn_scales_finish, dim3(1), dim3(256), 0, c.stream, c.partial, nr, N.dsc);
     hipLaunchKernelGGL(k_cnn_input, dim3(nb), dim3(256), 0, c.stream, c.r32[0], c.r32[1], c.pc[0], c.pc[1], c.bdt[0],
-                       c.bdt[1], c.ax, c.ay, c.az, (int)c.sx, (int)c.sy, (int)c.sz, (int)c.sxp, c.H, N.dsc, dst);
+                       c.bdt[1], c.ax, c.ay, c.az, (int)c.sx, (int)c.sy, (int)c.sz, (int)c.sxp, (int)c.H, dsx, dsy,
+                       N.dsc, dst);
     IB3D_CHECK(hipGetLastError());
     return 0;
 }
@@ -798,8 +986,8 @@ int cnn_before_solve(Ctx &c, const double *rel_prev, double *rel_guess, int (*re
         const Vol v{(int)c.sx, (int)c.sy, (int)c.sz, 1};
         forward(v);
         hipLaunchKernelGGL(k_cnn_apply, dim3(nblocks(c.P, 256)), dim3(256), 0, c.stream, N.out, N.pprev[0],
-                           N.pprev[1], c.bdt[0], c.bdt[1], c.pc[0], c.pc[1], (int)c.sx, (int)c.sy, (int)c.sz, c.H,
-                           N.dsc);
+                           N.pprev[1], c.bdt[0], c.bdt[1], c.pc[0], c.pc[1], (int)c.sx, (int)c.sy, (int)c.sz,
+                           (int)c.H, make_fastdiv((int)c.sx), make_fastdiv((int)c.sy), N.dsc);
         if (resid(c, rel_guess)) return -1;
         hipLaunchKernelGGL(k_cnn_select, dim3(nblocks(c.NC, 256)), dim3(256), 0, c.stream, rel_prev, rel_guess,
                            N.pprev[0], N.pprev[1], c.pc[0], c.pc[1], c.NC, N.dsc + 4, N.accept);

@@ -181,6 +181,34 @@ Ctx &ctx();
 // launch helpers
 inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// Division by a launch-invariant divisor d (2 <= d < 2^31) for numerators
+// below 2^31: q = umulhi(n, m) >> sh with m = ceil(2^(31+l)/d), l =
+// ceil(log2 d) (Granlund-Montgomery; exact because m*d - 2^(31+l) < d <= 2^l).
+struct FastDiv {
+    uint32_t m = 0;
+    int sh = 0;
+    int d = 1;
+};
+
+static inline FastDiv make_fastdiv(int d) {
+    FastDiv f;
+    int l = 0;
+    while ((1ll << l) < d) l++;
+    const int s = 31 + l;
+    f.m = (uint32_t)(((1ull << s) + (uint64_t)d - 1) / (uint64_t)d);
+    f.sh = s - 32;
+    f.d = d;
+    return f;
+}
+
+__host__ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(n, f.m) >> f.sh;
+#else
+    return (uint32_t)(((uint64_t)n * f.m) >> (32 + f.sh));
+#endif
+}
+
 // max that keeps a NaN operand (fmax would drop it)
 __device__ inline double nan_max(double a, double b) { return (a != a || b != b) ? __builtin_nan("") : fmax(a, b); }
 

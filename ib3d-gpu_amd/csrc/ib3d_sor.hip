@@ -164,34 +164,6 @@ __global__ void k_finish_norm(const double *partial, int64_t n, double bn, doubl
     }
 }
 
-// Division by a launch-invariant divisor d (2 <= d < 2^31) for numerators
-// below 2^31: q = umulhi(n, m) >> sh with m = ceil(2^(31+l)/d), l =
-// ceil(log2 d) (Granlund-Montgomery; exact because m*d - 2^(31+l) < d <= 2^l).
-struct FastDiv {
-    uint32_t m = 0;
-    int sh = 0;
-    int d = 1;
-};
-
-static inline FastDiv make_fastdiv(int d) {
-    FastDiv f;
-    int l = 0;
-    while ((1ll << l) < d) l++;
-    const int s = 31 + l;
-    f.m = (uint32_t)(((1ull << s) + (uint64_t)d - 1) / (uint64_t)d);
-    f.sh = s - 32;
-    f.d = d;
-    return f;
-}
-
-__host__ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __umulhi(n, f.m) >> f.sh;
-#else
-    return (uint32_t)(((uint64_t)n * f.m) >> (32 + f.sh));
-#endif
-}
-
 struct SorGeom {
     int H, NC, sy, sx, sz, plane;  // plane = H*sy
     int sxp;                       // row stride of the x coefficient table (multiple of 8)

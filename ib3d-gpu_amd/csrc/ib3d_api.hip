@@ -911,6 +911,11 @@ int ib3d_set_multigrid(int nu1, int nu2, double omega, int coarse_sweeps, double
     return 0;
 }
 
+int ib3d_set_graphs(int on) {
+    ctx().use_graphs = on != 0;
+    return 0;
+}
+
 int ib3d_poisson_stats(int64_t *sweeps, int64_t *outer, double *relres, double *relres0, double *sor_ms,
                        double *poisson_ms) {
     Ctx &c = ctx();

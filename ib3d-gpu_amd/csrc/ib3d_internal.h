@@ -87,6 +87,8 @@ struct Ctx {
     double *mtab[3] = {};
     int64_t mtn[3] = {};
     bool mom_ready = false, mom_fast = false;
+    // GMRES-IR iterations replayed as a captured HIP graph (ib3d_set_graphs)
+    bool use_graphs = false;
 
     // Laplacian weights: Acx(n,3) folded with the boundary conditions, kept
     // host-side in the reference's layout and on the device as per-index

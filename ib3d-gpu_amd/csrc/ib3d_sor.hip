@@ -140,7 +140,7 @@ __global__ void k_sum_partials(const double *partial, int64_t n, double *out) {
 // rel > tol (the loop continues), 0 once converged.  When gated the previous
 // iteration's values are carried forward.
 __global__ void k_finish_norm(const double *partial, int64_t n, double bn, double tol, double *rel, double *conv,
-                              const double *gate) {
+                              const double *gate, const double *bnp) {
     __shared__ double sm[1024];
     if (gate && *gate < 0.5) {
         if (threadIdx.x == 0) {
@@ -158,6 +158,7 @@ __global__ void k_finish_norm(const double *partial, int64_t n, double bn, doubl
         __syncthreads();
     }
     if (threadIdx.x == 0) {
+        if (bnp) bn = *bnp;   // (a captured cycle reads the solve's norm from device memory)
         const double r = bn > 0.0 ? sqrt(sm[0]) / bn : 0.0;
         rel[0] = r;
         conv[0] = (r > tol || r != r) ? 1.0 : 0.0;
@@ -779,8 +780,9 @@ __global__ void k_act_init(double *act, int m, const double *gate, double thr) {
 }
 
 __global__ void k_givens(double *__restrict__ H, int m, int j, double *cs, double *sn, double *gv, const double *beta,
-                         double *act, double target) {
+                         double *act, double target, const double *targetp) {
     if (act[j] < 0.5 || threadIdx.x != 0) return;
+    if (targetp) target = *targetp;
     double *h = H + (int64_t)(m + 1) * j;
     if (j == 0) gv[0] = *beta;
     for (int i = 0; i < j; i++) {
@@ -930,7 +932,9 @@ int sor_float_tables(Ctx &c) {
 
 void gmres_release();
 void mg_release();
+void cycle_graph_release();
 void sor_release_tables() {
+    cycle_graph_release();
     mg_release();
     gmres_release();
     if (g_axf) { (void)hipFree(g_axf); (void)hipFree(g_ayf); (void)hipFree(g_azf); }
@@ -1038,7 +1042,7 @@ int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2) {
 // residual of the current p (+e when add) into r32 (when write_r32) and
 // partial sums; then rel[o] and conv[o], gated on conv[o-1]
 static void residual_async(Ctx &c, bool add, bool write_r32, double bn, double *rel, double *conv,
-                           const double *gate) {
+                           const double *gate, const double *bnp = nullptr) {
     const dim3 grid = pz_grid(c);
     const int64_t nb = (int64_t)grid.x * grid.y;
     const SorGeom g = geom(c);
@@ -1051,7 +1055,8 @@ static void residual_async(Ctx &c, bool add, bool write_r32, double bn, double *
     else if (write_r32) IB3D_RESID(false, true);
     else IB3D_RESID(false, false);
 #undef IB3D_RESID
-    hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, c.stream, c.partial, nb, bn, c.tol, rel, conv, gate);
+    hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, c.stream, c.partial, nb, bn, c.tol, rel, conv, gate,
+                       bnp);
 }
 
 static void add_increment(Ctx &c) {
@@ -1835,7 +1840,7 @@ void gmres_release() {
 // sweeps (mode 3) or one multigrid V-cycle (mode 4), stopping early once the
 // least-squares residual is at most `target`, then pc += Z y in FP64.  All
 // gated on the previous step's convergence flag.
-static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target) {
+static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target, const double *targetp = nullptr) {
     const SorGeom g = geom(c);
     const int64_t NC = c.NC, stride = 2 * NC, n2 = NC;  // n2: float pairs per vector
     double *H = g_kh, *beta = H + (int64_t)(m + 1) * m, *y = beta + 1, *cs = y + m, *sn = cs + m, *gv = sn + m,
@@ -1875,7 +1880,8 @@ static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target)
         hipLaunchKernelGGL(k_sqrt_sum, dim3(1), dim3(256), 0, c.stream, g_kpart2, kVecBlocks, hj + j + 1, aj, 0.5);
         if (j + 1 < m)  // (before k_givens, which rotates h[j+1] to 0)
             hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, Vn, Vn, hj + j + 1, n2, aj, 0.5);
-        hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, c.stream, H, m, j, cs, sn, gv, beta, act, target);
+        hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, c.stream, H, m, j, cs, sn, gv, beta, act, target,
+                           targetp);
     }
     hipLaunchKernelGGL(k_lsq, dim3(1), dim3(64), 0, c.stream, H, m, gv, act, y, c.scalars + 24);
     hipLaunchKernelGGL(k_update_pc, vb, tb, 0, c.stream, c.pc[0], c.pc[1], g_kz, stride, m, y, V0, NC, act, 0.5);
@@ -1892,6 +1898,98 @@ static int clear_for_geometry(Ctx &c, int m) {
     for (size_t l = 1; l < g_mg.size(); l++)
         if (g_mg[l].x) IB3D_CHECK(dzero(g_mg[l].x, 2 * g_mg[l].NC * sizeof(float)));
     g_z_geom = c.geom_version;
+    return 0;
+}
+
+// One outer iteration of GMRES-IR (the Krylov cycle, the FP64 residual and
+// its norm) as a captured HIP graph, replayed once per iteration in place of
+// its about 330 launches (ib3d_set_graphs; off by default: at 256x192x192
+// the step time is the same either way).  The graph is static, so everything that
+// varies from one iteration or solve to the next lives in device slots
+// S = scalars[32..38]: S[0] the previous iteration's relative residual,
+// S[1], S[2] this iteration's (rel, conv), S[3] the gate (conv of the
+// previous iteration), S[4] the iteration index o, S[5] ||b/dt||, S[6] the
+// Krylov target.  k_cycle_end files (rel, conv) under o in the solve's
+// arrays, as the launches of the uncaptured loop do, and moves the gate on.
+static constexpr int kCycSlot = 32;
+__global__ void k_cycle_init(double *S, const double *relv, const double *conv, double bn, double target) {
+    if (threadIdx.x != 0) return;
+    S[0] = relv[0];
+    S[3] = conv[0];
+    S[4] = 1.0;
+    S[5] = bn;
+    S[6] = target;
+}
+__global__ void k_cycle_end(double *S, double *relv, double *conv) {
+    if (threadIdx.x != 0) return;
+    const int64_t o = (int64_t)S[4];
+    relv[o] = S[1];
+    conv[o] = S[2];
+    S[0] = S[1];
+    S[3] = S[2];
+    S[4] = (double)(o + 1);
+}
+
+static hipGraphExec_t g_cyc_exec = nullptr;
+static std::vector<uint64_t> g_cyc_key;
+
+void cycle_graph_release() {
+    if (g_cyc_exec) (void)hipGraphExecDestroy(g_cyc_exec);
+    g_cyc_exec = nullptr;
+    g_cyc_key.clear();
+}
+
+// (re)captures the iteration graph when any buffer or parameter it bakes in
+// has changed
+static int cycle_graph(Ctx &c, int m, int k, double *relv, double *conv) {
+    std::vector<uint64_t> key;
+    auto P = [&](const void *q) { key.push_back((uint64_t)(uintptr_t)q); };
+    auto D = [&](double v) {
+        uint64_t u;
+        std::memcpy(&u, &v, 8);
+        key.push_back(u);
+    };
+    for (const void *q : {(const void *)c.pc[0], (const void *)c.pc[1], (const void *)c.r32[0],
+                          (const void *)c.r32[1], (const void *)c.e32[0], (const void *)c.e32[1],
+                          (const void *)c.bdt[0], (const void *)c.bdt[1], (const void *)c.ax, (const void *)c.ay,
+                          (const void *)c.az, (const void *)c.partial, (const void *)c.scalars, (const void *)relv,
+                          (const void *)conv, (const void *)g_axf, (const void *)g_ayf, (const void *)g_azf,
+                          (const void *)g_kv, (const void *)g_kz, (const void *)g_kh, (const void *)g_kpart,
+                          (const void *)g_kpart2})
+        P(q);
+    for (const MgLevel &L : g_mg)
+        for (const void *q : {(const void *)L.x, (const void *)L.b, (const void *)L.r, (const void *)L.ax,
+                              (const void *)L.ay, (const void *)L.az, (const void *)L.tx, (const void *)L.ty,
+                              (const void *)L.tz})
+            P(q);
+    key.push_back((uint64_t)m);
+    key.push_back((uint64_t)k);
+    key.push_back((uint64_t)c.mode);
+    key.push_back((uint64_t)c.NC);
+    key.push_back((uint64_t)c.mg_nu1);
+    key.push_back((uint64_t)c.mg_nu2);
+    key.push_back((uint64_t)c.mg_coarse);
+    key.push_back((uint64_t)g_mg_small);
+    key.push_back((uint64_t)g_mg_lds);
+    D(c.omega);
+    D(c.tol);
+    D(c.mg_omega);
+    D(c.mg_coarse_omega);
+    if (g_cyc_exec && key == g_cyc_key) return 0;
+    cycle_graph_release();
+    double *S = c.scalars + kCycSlot;
+    IB3D_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    gmres_cycle(c, m, k, S + 3, 0.0, S + 6);
+    residual_async(c, false, true, 0.0, S + 1, S + 2, S + 3, S + 5);
+    hipLaunchKernelGGL(k_cycle_end, dim3(1), dim3(64), 0, c.stream, S, relv, conv);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c.stream, &graph);
+    IB3D_CHECK(ec);
+    const hipError_t ei = hipGraphInstantiate(&g_cyc_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) g_cyc_exec = nullptr;
+    IB3D_CHECK(ei);
+    g_cyc_key = key;
     return 0;
 }
 
@@ -1923,9 +2021,18 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
     // cycle).
     int batch = (int)std::min<int64_t>(16, std::max<int64_t>(1, c.last_outer - 1));
     std::vector<double> hr, hc;
+    if (c.use_graphs) {
+        if (cycle_graph(c, m, k, relv, conv)) return -1;
+        hipLaunchKernelGGL(k_cycle_init, dim3(1), dim3(64), 0, c.stream, c.scalars + kCycSlot, relv, conv, bn,
+                           c.krylov_target * c.tol * bn);
+    }
     for (;;) {
         for (int q = 0; q < batch && o < max_outer; q++) {
             o++;
+            if (c.use_graphs) {
+                IB3D_CHECK(hipGraphLaunch(g_cyc_exec, c.stream));
+                continue;
+            }
             const double *gate = conv + (o - 1);
             gmres_cycle(c, m, k, gate, c.krylov_target * c.tol * bn);
             residual_async(c, false, true, bn, relv + o, conv + o, gate);

@@ -65,6 +65,7 @@ def lib():
         "ib3d_set_poisson_mode": [I, D, D, I, I, I64],
         "ib3d_set_krylov": [I],
         "ib3d_set_multigrid": [I, I, D, I, D, I],
+        "ib3d_set_graphs": [I],
         "ib3d_poisson_stats": [PI64, PI64, PD, PD, PD, PD],
         "ib3d_residual_norm": [PD, PD],
         "ib3d_step": [I, D, D, D, D, PD],
@@ -273,6 +274,11 @@ class Solver:
     def set_multigrid(self, nu1=2, nu2=2, omega=1.55, coarse_sweeps=8, coarse_omega=1.5, max_levels=0):
         _ok(self.L.ib3d_set_multigrid(int(nu1), int(nu2), float(omega), int(coarse_sweeps), float(coarse_omega),
                                       int(max_levels)))
+
+    def set_graphs(self, on=True):
+        """Replay each GMRES-IR iteration as a captured HIP graph or launch its
+        kernels one by one (the default); the results are identical."""
+        _ok(self.L.ib3d_set_graphs(1 if on else 0))
 
     def poisson_stats(self):
         sw, out = C.c_int64(), C.c_int64()

@@ -160,6 +160,12 @@ int ib3d_set_krylov(int restart);
  * 8, 1.5, 0. */
 int ib3d_set_multigrid(int nu1, int nu2, double omega, int coarse_sweeps, double coarse_omega,
                        int max_levels);
+/* modes 3 and 4: replay each refinement iteration (Krylov cycle, FP64
+ * residual and its norm) as one captured HIP graph (on != 0) or launch its
+ * kernels one by one (on == 0, the default).  Both run the same kernels in
+ * the same order and give identical results.  No counterpart in the
+ * reference, which has no GPU launch layer. */
+int ib3d_set_graphs(int on);
 /* Statistics of the last solve: sweeps (SOR sweeps of either precision;
  * V-cycles in mode 4),
  * outer iterations, final and initial relative residual, SOR ms, whole

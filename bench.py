@@ -23,7 +23,7 @@ Poisson modes (same stopping rule: ||b/dt - A p||_2 <= 1e-6 ||b/dt||_2):
                    previous-step guess
   mixed         -- the same FP64 refinement and FP32 flexible GMRES(6),
                    every Arnoldi step preconditioned by one FP32 geometric
-                   multigrid V-cycle (red-black SOR smoothing, 2+2 sweeps per
+                   multigrid V-cycle (red-black SOR smoothing, 4+4 sweeps per
                    level at omega 1.55, trilinear prolongation),
                    previous-step guess
   mixed+learned -- mixed, starting from the CNN's guess when its residual
@@ -73,7 +73,7 @@ GMRES = {"m": 12, "k": 20, "omega": 1.9}
 # the multigrid preconditioner: Arnoldi steps per refinement step, pre- and
 # post-smoothing sweeps per level and their omega, sweeps and omega on the
 # coarsest level
-MG = {"m": 6, "nu1": 2, "nu2": 2, "omega": 1.55, "coarse": 8, "coarse_omega": 1.5}
+MG = {"m": 6, "nu1": 4, "nu2": 4, "omega": 1.55, "coarse": 4, "coarse_omega": 1.5}
 # The learned guess is kept when its residual is below this fraction of the
 # previous-step guess's.  With the multigrid solver a guess that is only
 # 10-40% better in residual norm was measured to double the Krylov steps of

@@ -156,9 +156,9 @@ struct Ctx {
     int mode = 0;                  // 0 reference rule, 1 FP64 L2, 2 mixed FP32/FP64, 3 FP32 GMRES in FP64 refinement, 4 the same with a multigrid preconditioner
     int krylov_m = 20;             // modes 3, 4: Arnoldi steps per refinement step
     double krylov_target = 0.3;    // modes 3, 4: a cycle stops once its residual estimate is below this x tol x ||b/dt||
-    int mg_nu1 = 2, mg_nu2 = 2;    // mode 4: pre- and post-smoothing sweeps per level
+    int mg_nu1 = 4, mg_nu2 = 4;    // mode 4: pre- and post-smoothing sweeps per level
     double mg_omega = 1.55;        // mode 4: smoother relaxation factor
-    int mg_coarse = 8;             // mode 4: sweeps on the coarsest level
+    int mg_coarse = 4;             // mode 4: sweeps on the coarsest level
     double mg_coarse_omega = 1.5;  // mode 4: relaxation factor there
     int mg_levels = 0;             // mode 4: level cap (0: coarsen while every dimension halves)
     double omega = 1.99;

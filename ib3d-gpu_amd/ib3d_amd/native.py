@@ -271,7 +271,7 @@ class Solver:
     def set_krylov(self, restart):
         _ok(self.L.ib3d_set_krylov(int(restart)))
 
-    def set_multigrid(self, nu1=2, nu2=2, omega=1.55, coarse_sweeps=8, coarse_omega=1.5, max_levels=0):
+    def set_multigrid(self, nu1=4, nu2=4, omega=1.55, coarse_sweeps=4, coarse_omega=1.5, max_levels=0):
         _ok(self.L.ib3d_set_multigrid(int(nu1), int(nu2), float(omega), int(coarse_sweeps), float(coarse_omega),
                                       int(max_levels)))
 

@@ -156,8 +156,8 @@ int ib3d_set_krylov(int restart);
 /* mode 4 V-cycle: nu1 >= 1 pre- and nu2 >= 0 post-smoothing red-black SOR
  * sweeps at `omega` on every level but the coarsest, `coarse_sweeps` at
  * `coarse_omega` there; max_levels caps the hierarchy (0: halve every
- * interior dimension while each stays even and >= 4).  Defaults 2, 2, 1.55,
- * 8, 1.5, 0. */
+ * interior dimension while each stays even and >= 4).  Defaults 4, 4, 1.55,
+ * 4, 1.5, 0. */
 int ib3d_set_multigrid(int nu1, int nu2, double omega, int coarse_sweeps, double coarse_omega,
                        int max_levels);
 /* modes 3 and 4: replay each refinement iteration (Krylov cycle, FP64

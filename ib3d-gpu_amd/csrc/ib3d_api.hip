@@ -104,7 +104,10 @@ static void free_all(Ctx &c) {
                      &c.cosG, &c.alpha3, &c.beta3, &c.gamma3, &c.normdis};
     for (auto pp : d1) dfree(*pp);
     for (int t = 0; t < 6; t++) dfree(c.tab[t]);
-    for (int t = 0; t < 3; t++) dfree(c.mtab[t]);
+    for (int t = 0; t < 3; t++) {
+        dfree(c.mtab[t]);
+        dfree(c.mrec[t]);
+    }
     c.mtn[0] = c.mtn[1] = c.mtn[2] = 0;
     c.mom_ready = false;
     dfree(c.e32[0]); dfree(c.r32[0]);   // colour 1 lives in the same blocks

@@ -21,7 +21,6 @@ struct Tabs {
 
 enum { UU = 0, VV = 1, WW = 2, UV = 3, VU = 4, WU = 5 };
 
-#define TB(set, c, i) (T.t[set][(c) * T.tn[set] + (i)])
 
 // Per-axis momentum tables, quantity-major with n+3 entries per quantity.
 // Each entry is a sub-expression of the momentum formula that depends on
@@ -46,9 +45,16 @@ enum MomQ {
 };
 
 struct MomTabs {
-    const double *m[3];
+    const double *m[3];    // quantity-major tables (x axis: lanes read consecutive entries)
     int mn[3];
+    const double *rec[3];  // y, z axes: one kMomRec-double record per index (see below)
 };
+
+// y and z records: the M_NQ quantities, then the face (staggered) and the
+// centre (across) convection coefficients of that axis.  A wave shares one
+// j and one k, so a record arrives through a few wide scalar loads.
+static constexpr int kMomRec = 64, kRecS = M_NQ, kRecC = M_NQ + 12;
+static_assert(kRecC + 12 <= kMomRec, "momentum record overflow");
 
 // a / b, correctly rounded, from y = RN(1/b) and two FMA corrections of
 // a*y.  The first brings the quotient within one ulp; by Markstein's
@@ -72,13 +78,14 @@ __device__ __forceinline__ double dv(double a, double b, double y) {
 
 // 3rd-order Kawamura-Kuwahara convection term (subNavierStokes.f90:812-831);
 // dA = t5*dA', dB = (2 t11)*dB' come from the axis table with their reciprocals
+// (coefficient c of the stencil at tc[c * ts])
 template <bool FAST>
-__device__ __forceinline__ double kk3(const Tabs &T, int set, int i, double vel, double f2, double f1, double f0,
+__device__ __forceinline__ double kk3(const double *tc, int ts, double vel, double f2, double f1, double f0,
                                       double fm1, double fm2, double dA, double rA, double dB, double rB) {
-    const double sa = (((TB(set, 0, i) * f2 + TB(set, 1, i) * f1) + TB(set, 2, i) * f0) + TB(set, 3, i) * fm1) +
-                      TB(set, 4, i) * fm2;
-    const double sk = (((TB(set, 6, i) * f2 + TB(set, 7, i) * f1) + TB(set, 8, i) * f0) + TB(set, 9, i) * fm1) +
-                      TB(set, 10, i) * fm2;
+#define TC(c) tc[(c) * ts]
+    const double sa = (((TC(0) * f2 + TC(1) * f1) + TC(2) * f0) + TC(3) * fm1) + TC(4) * fm2;
+    const double sk = (((TC(6) * f2 + TC(7) * f1) + TC(8) * f0) + TC(9) * fm1) + TC(10) * fm2;
+#undef TC
     return dv<FAST>(vel * sa, dA, rA) + dv<FAST>(fabs(vel) * sk, dB, rB);
 }
 
@@ -109,9 +116,12 @@ __device__ __forceinline__ void momentum_cell(
 #define W(a, b, d) w[c + (a) + (b) * sx + (d) * sxy]
 #define P(a, b, d) p[c + (a) + (b) * sx + (d) * sxy]
 #define CL(a, b, d) cell[c + (a) + (b) * sx + (d) * sxy]
+    const double *ry = M.rec[1] + (int64_t)j * kMomRec, *rz = M.rec[2] + (int64_t)k * kMomRec;
+    const double *tuu = T.t[UU] + i, *tuv = T.t[UV] + i;
+    const int nuu = T.tn[UU], nuv = T.tn[UV];
 #define MX(q) M.m[0][(q) * M.mn[0] + i]
-#define MY(q) M.m[1][(q) * M.mn[1] + j]
-#define MZ(q) M.m[2][(q) * M.mn[2] + k]
+#define MY(q) ry[q]
+#define MZ(q) rz[q]
     const double u22 = U(-1, 0, 0) - U(0, 0, 0), u4 = U(0, 0, 0) - U(1, 0, 0);
     const double u6 = U(0, -1, 0) - U(0, 0, 0), u8 = U(0, 0, 0) - U(0, 1, 0);
     const double u10 = U(0, 0, -1) - U(0, 0, 0), u12 = U(0, 0, 0) - U(0, 0, 1);
@@ -126,7 +136,7 @@ __device__ __forceinline__ void momentum_cell(
     const double dpdz = dv<FAST>(P(0, 0, 0) - P(0, 0, 1), MZ(M_H1), MZ(M_RH1));
     const bool no_ib = CL(1, 0, 0) != 2 && CL(-1, 0, 0) != 2 && CL(0, 1, 0) != 2 && CL(0, -1, 0) != 2 &&
                        CL(0, 0, 1) != 2 && CL(0, 0, -1) != 2;
-    const int mx = M.mn[0], my = M.mn[1], mz = M.mn[2];
+    const int mx = M.mn[0], my = 1, mz = 1;
 
     // U momentum (subNavierStokes.f90:792-897)
     {
@@ -140,11 +150,11 @@ __device__ __forceinline__ void momentum_cell(
         double a, b, d;
         if (i != 2 && i < nx && j != 2 && j < ny + 1 && k != 2 && k < nz + 1 && no_ib) {
             const double uc = U(0, 0, 0);
-            a = kk3<FAST>(T, UU, i, uc, U(2, 0, 0), U(1, 0, 0), uc, U(-1, 0, 0), U(-2, 0, 0), MX(M_KS1), MX(M_RKS1),
+            a = kk3<FAST>(tuu, nuu, uc, U(2, 0, 0), U(1, 0, 0), uc, U(-1, 0, 0), U(-2, 0, 0), MX(M_KS1), MX(M_RKS1),
                           MX(M_KS2), MX(M_RKS2));
-            b = kk3<FAST>(T, VU, j, v_in_um, U(0, 2, 0), U(0, 1, 0), uc, U(0, -1, 0), U(0, -2, 0), MY(M_KC1),
+            b = kk3<FAST>(ry + kRecC, 1, v_in_um, U(0, 2, 0), U(0, 1, 0), uc, U(0, -1, 0), U(0, -2, 0), MY(M_KC1),
                           MY(M_RKC1), MY(M_KC2), MY(M_RKC2));
-            d = kk3<FAST>(T, WU, k, w_in_um, U(0, 0, 2), U(0, 0, 1), uc, U(0, 0, -1), U(0, 0, -2), MZ(M_KC1),
+            d = kk3<FAST>(rz + kRecC, 1, w_in_um, U(0, 0, 2), U(0, 0, 1), uc, U(0, 0, -1), U(0, 0, -2), MZ(M_KC1),
                           MZ(M_RKC1), MZ(M_KC2), MZ(M_RKC2));
         } else {
             a = up1<FAST>(&MX(M_US), mx, U(0, 0, 0), MX(M_D1), MX(M_RD1), U(-1, 0, 0), U(0, 0, 0), U(1, 0, 0));
@@ -171,11 +181,11 @@ __device__ __forceinline__ void momentum_cell(
         double a, b, d;
         if (i != 2 && i < nx + 1 && j != 2 && j < ny && k != 2 && k < nz + 1 && no_ib) {
             const double vc = V(0, 0, 0);
-            a = kk3<FAST>(T, UV, i, u_in_vm, V(2, 0, 0), V(1, 0, 0), vc, V(-1, 0, 0), V(-2, 0, 0), MX(M_KC1),
+            a = kk3<FAST>(tuv, nuv, u_in_vm, V(2, 0, 0), V(1, 0, 0), vc, V(-1, 0, 0), V(-2, 0, 0), MX(M_KC1),
                           MX(M_RKC1), MX(M_KC2), MX(M_RKC2));
-            b = kk3<FAST>(T, VV, j, vc, V(0, 2, 0), V(0, 1, 0), vc, V(0, -1, 0), V(0, -2, 0), MY(M_KS1), MY(M_RKS1),
+            b = kk3<FAST>(ry + kRecS, 1, vc, V(0, 2, 0), V(0, 1, 0), vc, V(0, -1, 0), V(0, -2, 0), MY(M_KS1), MY(M_RKS1),
                           MY(M_KS2), MY(M_RKS2));
-            d = kk3<FAST>(T, WU, k, w_in_vm, V(0, 0, 2), V(0, 0, 1), vc, V(0, 0, -1), V(0, 0, -2), MZ(M_KC1),
+            d = kk3<FAST>(rz + kRecC, 1, w_in_vm, V(0, 0, 2), V(0, 0, 1), vc, V(0, 0, -1), V(0, 0, -2), MZ(M_KC1),
                           MZ(M_RKC1), MZ(M_KC2), MZ(M_RKC2));
         } else {
             a = up1<FAST>(&MX(M_UC), mx, u_in_vm, MX(M_H1), MX(M_RH1), V(-1, 0, 0), V(0, 0, 0), V(1, 0, 0));
@@ -202,11 +212,11 @@ __device__ __forceinline__ void momentum_cell(
         double a, b, d;
         if (i != 2 && i < nx + 1 && j != 2 && j < ny + 1 && k != 2 && k < nz && no_ib) {
             const double wc = W(0, 0, 0);
-            a = kk3<FAST>(T, UV, i, u_in_wm, W(2, 0, 0), W(1, 0, 0), wc, W(-1, 0, 0), W(-2, 0, 0), MX(M_KC1),
+            a = kk3<FAST>(tuv, nuv, u_in_wm, W(2, 0, 0), W(1, 0, 0), wc, W(-1, 0, 0), W(-2, 0, 0), MX(M_KC1),
                           MX(M_RKC1), MX(M_KC2), MX(M_RKC2));
-            b = kk3<FAST>(T, VU, j, v_in_wm, W(0, 2, 0), W(0, 1, 0), wc, W(0, -1, 0), W(0, -2, 0), MY(M_KC1),
+            b = kk3<FAST>(ry + kRecC, 1, v_in_wm, W(0, 2, 0), W(0, 1, 0), wc, W(0, -1, 0), W(0, -2, 0), MY(M_KC1),
                           MY(M_RKC1), MY(M_KC2), MY(M_RKC2));
-            d = kk3<FAST>(T, WW, k, wc, W(0, 0, 2), W(0, 0, 1), wc, W(0, 0, -1), W(0, 0, -2), MZ(M_KS1), MZ(M_RKS1),
+            d = kk3<FAST>(rz + kRecS, 1, wc, W(0, 0, 2), W(0, 0, 1), wc, W(0, 0, -1), W(0, 0, -2), MZ(M_KS1), MZ(M_RKS1),
                           MZ(M_KS2), MZ(M_RKS2));
         } else {
             a = up1<FAST>(&MX(M_UC), mx, u_in_wm, MX(M_H1), MX(M_RH1), W(-1, 0, 0), W(0, 0, 0), W(1, 0, 0));
@@ -453,14 +463,26 @@ static int build_mom_tables(Ctx &c) {
                 fast = fast && in_range(kc1) && in_range(kc2) && (i == n || (in_range(ks1) && in_range(ks2)));
             }
         }
+        // y and z: the same quantities and this axis's two convection tables, per index
+        std::vector<double> r(a ? kMomRec * tn : 0, 0.0);
+        for (int64_t i = 0; a && i < tn; i++) {
+            for (int q = 0; q < M_NQ; q++) r[i * kMomRec + q] = h[q * tn + i];
+            for (int q = 0; q < 12 && i < ftn; q++) {
+                r[i * kMomRec + kRecS + q] = face[q * ftn + i];
+                r[i * kMomRec + kRecC + q] = cent[q * ftn + i];
+            }
+        }
         if (c.mtn[a] != tn) {
             if (c.mtab[a]) IB3D_CHECK(hipFree(c.mtab[a]));
-            c.mtab[a] = nullptr;
+            if (c.mrec[a]) IB3D_CHECK(hipFree(c.mrec[a]));
+            c.mtab[a] = c.mrec[a] = nullptr;
             c.mtn[a] = 0;
             IB3D_CHECK(hipMalloc((void **)&c.mtab[a], h.size() * sizeof(double)));
+            if (a) IB3D_CHECK(hipMalloc((void **)&c.mrec[a], r.size() * sizeof(double)));
             c.mtn[a] = tn;
         }
         IB3D_CHECK(h2d(c.mtab[a], h.data(), h.size() * sizeof(double)));
+        if (a) IB3D_CHECK(h2d(c.mrec[a], r.data(), r.size() * sizeof(double)));
     }
     c.mom_fast = fast;
     c.mom_ready = true;
@@ -475,6 +497,7 @@ int ns_momentum(Ctx &c, double rev, double deltat) {
     for (int a = 0; a < 3; a++) {
         M.m[a] = c.mtab[a];
         M.mn[a] = (int)c.mtn[a];
+        M.rec[a] = c.mrec[a];
     }
     const Tabs T = tabs(c);
     const int nx = (int)c.nx, ny = (int)c.ny, nz = (int)c.nz;

@@ -85,6 +85,7 @@ struct Ctx {
     // momentum tables per axis (index-only sub-expressions and reciprocals,
     // built by ns_momentum from the grid and the convection tables)
     double *mtab[3] = {};
+    double *mrec[3] = {};   // y, z: the same per index, with the convection tables
     int64_t mtn[3] = {};
     bool mom_ready = false, mom_fast = false;
     // GMRES-IR iterations replayed as a captured HIP graph (ib3d_set_graphs)

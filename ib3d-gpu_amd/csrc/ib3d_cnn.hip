@@ -215,7 +215,7 @@ k_conv_lds(const __bf16 *__restrict__ X, const bf16x8 *__restrict__ W, const flo
 // y0+4w .. y0+4w+3.  A staged plane is split by channel half ([half][voxel]
 // [8 channels]) so the 16 lanes reading one row's voxels touch consecutive
 // 16-byte words.
-static constexpr int kZmX = 16, kZmY = 16, kZmZ = 16;
+static constexpr int kZmX = 16, kZmY = 16, kZmZ = 24;
 static constexpr int kZpX = kZmX + 2, kZpV = kZpX * (kZmY + 2);
 
 template <int CIN, int MODE>

@@ -24,7 +24,7 @@
 //             levels, where a z-marching thread would have too few peers.
 //   k_resid_pz  FP64 residual b/dt - A p over both colours (optionally of
 //             p + e), written as FP32 for the next inner solve.
-//   k_matvec32_pz, k_dots, k_orth_scale, k_givens, k_lsq, k_update_pc
+//   k_matvec32_pz, k_dots, k_orth, k_givens, k_lsq, k_update_pc
 //             FP32 flexible GMRES inside the FP64 refinement (modes 3, 4).
 //   k_mg_*    the FP32 multigrid V-cycle that preconditions it in mode 4.
 //
@@ -619,11 +619,10 @@ __global__ void __launch_bounds__(kSorBlock)
 k_dots(const float *__restrict__ w, const float *__restrict__ V, int64_t stride, int64_t n4,
        double *__restrict__ partial, const double *gate, double thr) {
     if (gated(gate, thr)) return;
-    // rows 0..NV-1: <w, v_i>; row NV: <w, w>
-    __shared__ double sm[NV + 1][kSorBlock / 64];
-    double s[NV + 1];
+    __shared__ double sm[NV][kSorBlock / 64];
+    double s[NV];
 #pragma unroll
-    for (int i = 0; i <= NV; i++) s[i] = 0.0;
+    for (int i = 0; i < NV; i++) s[i] = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < n4; q += (int64_t)gridDim.x * kSorBlock) {
         const f4v a = reinterpret_cast<const f4v *>(w)[q];
         const bool ok0 = a.x == a.x, ok1 = a.y == a.y, ok2 = a.z == a.z, ok3 = a.w == a.w;
@@ -635,26 +634,22 @@ k_dots(const float *__restrict__ w, const float *__restrict__ V, int64_t stride,
             if (ok2) s[i] += (double)a.z * (double)b.z;
             if (ok3) s[i] += (double)a.w * (double)b.w;
         }
-        if (ok0) s[NV] += (double)a.x * (double)a.x;
-        if (ok1) s[NV] += (double)a.y * (double)a.y;
-        if (ok2) s[NV] += (double)a.z * (double)a.z;
-        if (ok3) s[NV] += (double)a.w * (double)a.w;
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i <= NV; i++) {
+    for (int i = 0; i < NV; i++) {
         const double v = wave_sum(s[i]);
         if (lane == 0) sm[i][wid] = v;
     }
     __syncthreads();
-    if ((int)threadIdx.x <= NV)
+    if ((int)threadIdx.x < NV)
         partial[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] =
             ((sm[threadIdx.x][0] + sm[threadIdx.x][1]) + sm[threadIdx.x][2]) + sm[threadIdx.x][3];
 }
 
-// block i < nrow-1: out[i] = sum of partial[i][0..nblk); block nrow-1: *ww
-__global__ void k_rows_sum_ww(const double *__restrict__ partial, int nblk, double *out, double *ww,
-                              const double *gate, double thr) {
+// block i: out[i] (+)= sum of partial[i][0..nblk)
+__global__ void k_rows_sum(const double *__restrict__ partial, int nblk, double *out, int accumulate,
+                           const double *gate, double thr) {
     if (gated(gate, thr)) return;
     __shared__ double sm[256];
     double s = 0.0;
@@ -665,35 +660,20 @@ __global__ void k_rows_sum_ww(const double *__restrict__ partial, int nblk, doub
         if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        if ((int)blockIdx.x + 1 < (int)gridDim.x)
-            out[blockIdx.x] = sm[0];
-        else
-            *ww = sm[0];
-    }
+    if (threadIdx.x == 0) out[blockIdx.x] = accumulate ? out[blockIdx.x] + sm[0] : sm[0];
 }
 
-// v_new = (w - sum_i h[i] v_i) / h[NV] in one pass, with h[NV] = ||w - sum h_i v_i||
-// = sqrt(<w, w> - sum_i h_i^2) (the v_i are orthonormal; <w, w> = *ww from the
-// dots pass), so no second pass scales the vector.  Cancellation beyond
-// 1e-14 <w, w> counts as breakdown (h[NV] = 0, v_new = 0).  Every block
-// derives the same h[NV]; block 0 stores it.
+// w -= sum_i h[i] v_i (FP64 accumulation), then per-block partial sums of
+// w^2 (NaN skipped); float4 groups, NV unrolled
 template <int NV>
 __global__ void __launch_bounds__(kSorBlock)
-k_orth_scale(float *__restrict__ w, const float *__restrict__ V, int64_t stride, double *__restrict__ h,
-             const double *__restrict__ ww, int64_t n4, const double *gate, double thr) {
+k_orth(float *__restrict__ w, const float *__restrict__ V, int64_t stride, const double *__restrict__ h, int64_t n4,
+       double *__restrict__ partial, const double *gate, double thr) {
     if (gated(gate, thr)) return;
     double hv[NV];
-    double hh = 0.0;
 #pragma unroll
-    for (int i = 0; i < NV; i++) {
-        hv[i] = h[i];
-        hh += hv[i] * hv[i];
-    }
-    const double w2 = *ww, r2 = w2 - hh;
-    const double hn = (r2 > 1e-14 * w2 && r2 == r2) ? sqrt(r2) : 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) h[NV] = hn;
-    const double inv = hn > 0.0 ? 1.0 / hn : 0.0;
+    for (int i = 0; i < NV; i++) hv[i] = h[i];
+    double s = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q < n4; q += (int64_t)gridDim.x * kSorBlock) {
         f4v a = reinterpret_cast<f4v *>(w)[q];
         double ax = a.x, ay = a.y, az = a.z, aw = a.w;
@@ -705,15 +685,20 @@ k_orth_scale(float *__restrict__ w, const float *__restrict__ V, int64_t stride,
             az -= hv[i] * (double)b.z;
             aw -= hv[i] * (double)b.w;
         }
-        a.x = (float)(ax * inv);
-        a.y = (float)(ay * inv);
-        a.z = (float)(az * inv);
-        a.w = (float)(aw * inv);
+        a.x = (float)ax;
+        a.y = (float)ay;
+        a.z = (float)az;
+        a.w = (float)aw;
         reinterpret_cast<f4v *>(w)[q] = a;
+        if (a.x == a.x) s += (double)a.x * (double)a.x;
+        if (a.y == a.y) s += (double)a.y * (double)a.y;
+        if (a.z == a.z) s += (double)a.z * (double)a.z;
+        if (a.w == a.w) s += (double)a.w * (double)a.w;
     }
+    block_sum_to(s, partial, blockIdx.x);
 }
 
-// launch k_dots<nv> / k_orth_scale<nv> for nv = 1..kMaxKrylov-1
+// launch k_dots<nv> / k_orth<nv> for nv = 1..kMaxKrylov-1
 #define IB3D_GS_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) \
     X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23)
 static void launch_dots(int nv, dim3 g, dim3 b, hipStream_t st, const float *w, const float *V, int64_t stride,
@@ -726,11 +711,11 @@ static void launch_dots(int nv, dim3 g, dim3 b, hipStream_t st, const float *w, 
         default: break;
     }
 }
-static void launch_orth_scale(int nv, dim3 g, dim3 b, hipStream_t st, float *w, const float *V, int64_t stride,
-                              double *h, const double *ww, int64_t n4, const double *gate) {
+static void launch_orth(int nv, dim3 g, dim3 b, hipStream_t st, float *w, const float *V, int64_t stride,
+                        const double *h, int64_t n4, double *partial, const double *gate) {
     switch (nv) {
 #define IB3D_ORTH(N) \
-    case N: hipLaunchKernelGGL(k_orth_scale<N>, g, b, 0, st, w, V, stride, h, ww, n4, gate, 0.5); break;
+    case N: hipLaunchKernelGGL(k_orth<N>, g, b, 0, st, w, V, stride, h, n4, partial, gate, 0.5); break;
         IB3D_GS_CASES(IB3D_ORTH)
 #undef IB3D_ORTH
         default: break;
@@ -1817,8 +1802,8 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
 }
 
 
-// g_kh: H ((m+1) x m, column-major), beta, y[m], cs[m], sn[m], g[m+1], act[m+1], <w, w>
-static int64_t kh_size(int m) { return (int64_t)(m + 1) * m + 1 + 3 * m + 2 * (m + 1) + 1; }
+// g_kh: H ((m+1) x m, column-major), beta, y[m], cs[m], sn[m], g[m+1], act[m+1]
+static int64_t kh_size(int m) { return (int64_t)(m + 1) * m + 1 + 3 * m + 2 * (m + 1); }
 
 static int gmres_buffers(Ctx &c, int m) {
     if (g_kv && g_k_nc == c.NC && g_k_m == m) return 0;
@@ -1859,7 +1844,7 @@ static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target,
     const SorGeom g = geom(c);
     const int64_t NC = c.NC, stride = 2 * NC, n2 = NC;  // n2: float pairs per vector
     double *H = g_kh, *beta = H + (int64_t)(m + 1) * m, *y = beta + 1, *cs = y + m, *sn = cs + m, *gv = sn + m,
-           *act = gv + (m + 1), *ww = act + (m + 1);
+           *act = gv + (m + 1);
     const dim3 vb(kVecBlocks), tb(kSorBlock);
     const float om = (float)c.omega, omm = (float)(1.0 - c.omega);
     float *V0 = g_kv;
@@ -1889,11 +1874,12 @@ static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target,
         hipLaunchKernelGGL(k_matvec32_pz<false>, pz_grid(c), tb, 0, c.stream, Z, Z + NC, V0, V0 + NC, Vn, Vn + NC,
                            g_axf, g_ayf, g_azf, g, aj, 0.5);
         double *hj = H + (int64_t)(m + 1) * j;
-        // classical Gram-Schmidt: one pass for <w, v_i> and <w, w>, one that
-        // orthogonalises and normalises (before k_givens rotates h[j+1] to 0)
         launch_dots(j + 1, vb, tb, c.stream, Vn, g_kv, stride, n2 / 2, g_kpart, aj);
-        hipLaunchKernelGGL(k_rows_sum_ww, dim3(j + 2), dim3(256), 0, c.stream, g_kpart, kVecBlocks, hj, ww, aj, 0.5);
-        launch_orth_scale(j + 1, vb, tb, c.stream, Vn, g_kv, stride, hj, ww, n2 / 2, aj);
+        hipLaunchKernelGGL(k_rows_sum, dim3(j + 1), dim3(256), 0, c.stream, g_kpart, kVecBlocks, hj, 0, aj, 0.5);
+        launch_orth(j + 1, vb, tb, c.stream, Vn, g_kv, stride, hj, n2 / 2, g_kpart2, aj);
+        hipLaunchKernelGGL(k_sqrt_sum, dim3(1), dim3(256), 0, c.stream, g_kpart2, kVecBlocks, hj + j + 1, aj, 0.5);
+        if (j + 1 < m)  // (before k_givens, which rotates h[j+1] to 0)
+            hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, Vn, Vn, hj + j + 1, n2, aj, 0.5);
         hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, c.stream, H, m, j, cs, sn, gv, beta, act, target,
                            targetp);
     }

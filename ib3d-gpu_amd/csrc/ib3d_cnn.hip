@@ -14,6 +14,8 @@
 // then iterates to the same tolerance either way.
 //
 // Everything runs in hand-written kernels:
+//   k_conv_ring the 8-channel input layer: output planes in turn from a
+//              three-plane LDS ring, taps in order.
 //   k_conv_zm  forward implicit-GEMM convolution on MFMA
 //              (v_mfma_f32_16x16x32_bf16): D[16 out-ch][16 voxels] =
 //              W[16 x 27*Cin] * im2col[27*Cin x 16], BF16 weights and
@@ -49,7 +51,7 @@ static constexpr int kCin[4] = {3, 16, 16, 16};
 static constexpr int kCout[4] = {16, 16, 16, 1};
 static constexpr int kWoff[4] = {0, 1312, 8240, 15168};
 static constexpr int kBoff[4] = {1296, 8224, 15152, 15600};
-static constexpr int kNkc[4] = {9, 15, 15, 15};   // forward K chunks (taps grouped by z plane)
+static constexpr int kNkc[4] = {7, 15, 15, 15};   // forward K chunks (layers 1-3: taps grouped by z plane)
 static constexpr int kTile = 64, kHalo = 4, kTileIn = kTile + 2 * kHalo;
 
 struct Vol {
@@ -61,27 +63,32 @@ __device__ __forceinline__ int64_t vox_index(const Vol &v, int b, int z, int y, 
 }
 
 // ------------------------------------------------------------------ packing
-// forward pack: row m = output channel, k = (tap, input channel); chunk
-// kc = dz*cpz + c holds in-plane taps 2c, 2c+1 (16 stored channels) or
-// 4c .. 4c+3 (8 stored channels) of z-tap dz, padded with zeros past tap 8
+// forward pack: row m = output channel, k = (tap, input channel).  grouped:
+// chunk kc = dz*cpz + c holds in-plane taps 2c, 2c+1 (16 stored channels)
+// or 4c .. 4c+3 (8 stored channels) of z-tap dz, padded with zeros past tap
+// 8 (k_conv_zm); otherwise chunk kc holds taps 2kc, 2kc+1 or 4kc .. 4kc+3
+// in order, zeros past tap 26 (k_conv_ring)
 __global__ void k_pack_fwd(const float *__restrict__ prm, int woff, int boff, int cin, int cout, int cinst, int nkc,
-                           __bf16 *__restrict__ out, float *__restrict__ bias16) {
+                           int grouped, __bf16 *__restrict__ out, float *__restrict__ bias16) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < 16) bias16[idx] = idx < cout ? prm[boff + idx] : 0.0f;
     if (idx >= nkc * 512) return;
     const int j = idx & 7, l = (idx >> 3) & 63, kc = idx >> 9;
     const int m = l & 15, g = l >> 4;
-    const int cpz = nkc / 3, dz = kc / cpz, c = kc - dz * cpz;
-    int ip, ci;
-    if (cinst == 16) {
-        ip = 2 * c + (g >> 1);
-        ci = 8 * (g & 1) + j;
-    } else {
-        ip = 4 * c + g;
-        ci = j;
+    int t, ci;
+    bool ok;
+    if (grouped) {
+        const int cpz = nkc / 3, dz = kc / cpz, c = kc - dz * cpz;
+        const int ip = cinst == 16 ? 2 * c + (g >> 1) : 4 * c + g;
+        ci = cinst == 16 ? 8 * (g & 1) + j : j;
+        t = 9 * dz + ip;
+        ok = ip < 9;
+    } else {   // taps in order: 2 (16 channels) or 4 (8 channels) per chunk
+        t = cinst == 16 ? 2 * kc + (g >> 1) : 4 * kc + g;
+        ci = cinst == 16 ? 8 * (g & 1) + j : j;
+        ok = t < 27;
     }
-    const int t = 9 * dz + ip;
-    const float w = (ip < 9 && ci < cin && m < cout) ? prm[woff + (t * cin + ci) * cout + m] : 0.0f;
+    const float w = (ok && ci < cin && m < cout) ? prm[woff + (t * cin + ci) * cout + m] : 0.0f;
     out[idx] = (__bf16)w;
 }
 
@@ -194,6 +201,145 @@ k_conv_lds(const __bf16 *__restrict__ X, const bf16x8 *__restrict__ W, const flo
                 *reinterpret_cast<bf16x4 *>(Y16 + vo * 16 + 4 * g) = ov;
             }
         }
+    }
+}
+
+// Forward convolution of the 8-channel input layer, one output plane per
+// step (MODE 0: + bias, ReLU, 16 BF16 channels out).  With 8 channels a K
+// chunk of 32 holds four taps, so grouping the taps by z plane (as
+// k_conv_zm does) would pad 9 in-plane taps to 12; here the 27 taps run in
+// order (7 chunks) and each output plane reads its three input planes from
+// an LDS ring instead.  A 256-thread workgroup owns a 16 (x) x 16 (y)
+// column of output voxels over kRgZ consecutive planes; the ring holds the
+// three input planes with a one-voxel x-y halo (18 x 18 voxels).  Each step
+// loads the next plane into registers before the MFMAs of the current one
+// and writes it over the oldest slot afterwards.  Wave w computes output
+// rows y0+4w .. y0+4w+3 with four independent accumulators.
+static constexpr int kRgX = 16, kRgY = 16, kRgZ = 16;
+static constexpr int kRpX = kRgX + 2, kRpV = kRpX * (kRgY + 2);
+
+template <int CIN, int MODE>
+__global__ void __launch_bounds__(256)
+k_conv_ring(const __bf16 *__restrict__ X, const bf16x8 *__restrict__ W, const float *__restrict__ bias, Vol v,
+          __bf16 *__restrict__ Y16, float *__restrict__ Y32, int nzc) {
+    constexpr int NKC = CIN == 16 ? 14 : 7;
+    constexpr int VU = CIN / 8;        // 16-byte units per voxel
+    constexpr int PE = kRpV * CIN;     // BF16 elements per staged plane
+    constexpr int PU = kRpV * VU;      // 16-byte units per staged plane
+    constexpr int UPT = (PU + 255) / 256;
+    __shared__ __attribute__((aligned(16))) __bf16 ring[3 * PE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, g = lane >> 4;
+    const int x0 = (int)blockIdx.x * kRgX, y0 = (int)blockIdx.y * kRgY;
+    const int b = (int)blockIdx.z / nzc, z0 = ((int)blockIdx.z - b * nzc) * kRgZ;
+    const int z1 = min(z0 + kRgZ, v.vz);
+    const int64_t pstride = (int64_t)v.vy * v.vx * CIN;
+    const __bf16 *Xb = X + (int64_t)b * v.vz * pstride;
+
+    // this thread's staging units, the same in every plane
+    int soff[UPT];
+    int64_t goff[UPT];   // -1: beyond the plane or outside the volume in x / y
+#pragma unroll
+    for (int u = 0; u < UPT; u++) {
+        const int unit = (int)threadIdx.x + 256 * u;
+        const int vox = unit / VU, part = unit - vox * VU;
+        const int xs = x0 - 1 + vox % kRpX, ys = y0 - 1 + vox / kRpX;
+        soff[u] = unit < PU ? part * (kRpV * 8) + vox * 8 : -1;
+        goff[u] = (unit < PU && xs >= 0 && xs < v.vx && ys >= 0 && ys < v.vy)
+                      ? ((int64_t)ys * v.vx + xs) * CIN + 8 * part
+                      : -1;
+    }
+    bf16x8 r[UPT];
+    auto load_plane = [&](int zs) {
+#pragma unroll
+        for (int u = 0; u < UPT; u++) {
+            bf16x8 val = {};
+            if (goff[u] >= 0 && zs >= 0 && zs < v.vz)
+                val = *reinterpret_cast<const bf16x8 *>(Xb + zs * pstride + goff[u]);
+            r[u] = val;
+        }
+    };
+    auto store_plane = [&](int slot) {
+#pragma unroll
+        for (int u = 0; u < UPT; u++)
+            if (soff[u] >= 0) *reinterpret_cast<bf16x8 *>(ring + slot * PE + soff[u]) = r[u];
+    };
+
+    bf16x8 wr[NKC];
+    int toff[NKC], tdz[NKC];
+#pragma unroll
+    for (int kc = 0; kc < NKC; kc++) {
+        wr[kc] = W[kc * 64 + lane];
+        int t, part;
+        if (CIN == 16) {
+            t = 2 * kc + (g >> 1);
+            part = g & 1;
+        } else {
+            t = 4 * kc + g;
+            part = 0;
+        }
+        // padded K slots (t = 27) carry zero weights; point them at the centre tap
+        const int tt = t < 27 ? t : 13;
+        const int dx = tt % 3 - 1, dy = (tt / 3) % 3 - 1;
+        tdz[kc] = tt / 9;
+        toff[kc] = part * (kRpV * 8) + (dy * kRpX + dx) * 8;
+    }
+    float bv[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) bv[i] = MODE == 0 ? bias[4 * g + i] : bias[0];
+    int rowoff[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) rowoff[rr] = ((4 * wave + rr + 1) * kRpX + n + 1) * 8;
+
+    load_plane(z0 - 1);
+    store_plane(0);
+    load_plane(z0);
+    store_plane(1);
+    load_plane(z0 + 1);
+    store_plane(2);
+    __syncthreads();
+    int s0 = 0;   // slot of plane z-1; planes z and z+1 follow cyclically
+    for (int z = z0; z < z1; z++) {
+        const bool more = z + 2 <= z1;   // plane z+2 feeds output plane z+1
+        if (more) load_plane(z + 2);
+        f32x4 acc[4];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) acc[rr] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kc = 0; kc < NKC; kc++) {
+            int sl = s0 + tdz[kc];
+            sl = sl >= 3 ? sl - 3 : sl;
+            const __bf16 *src = ring + sl * PE + toff[kc];
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const bf16x8 xv = *reinterpret_cast<const bf16x8 *>(src + rowoff[rr]);
+                acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[kc], xv, acc[rr], 0, 0, 0);
+            }
+        }
+        const int x = x0 + n;
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            const int y = y0 + 4 * wave + rr;
+            if (x < v.vx && y < v.vy) {
+                const int64_t vo = vox_index(v, b, z, y, x);
+                if (MODE == 1) {
+                    if (g == 0) Y32[vo] = acc[rr][0] + bv[0];
+                } else {
+                    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                    bf16x4 ov;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const float val = acc[rr][i] + bv[i];
+                        ov[i] = (__bf16)(val > 0.0f ? val : 0.0f);
+                    }
+                    *reinterpret_cast<bf16x4 *>(Y16 + vo * 16 + 4 * g) = ov;
+                }
+            }
+        }
+        __syncthreads();
+        if (more) store_plane(s0);   // plane z+2 replaces plane z-1
+        s0 = s0 == 2 ? 0 : s0 + 1;
+        __syncthreads();
     }
 }
 
@@ -772,7 +918,7 @@ static int pack_all() {
     for (int L = 0; L < 4; L++) {
         const int cinst = L == 0 ? 8 : 16;
         hipLaunchKernelGGL(k_pack_fwd, dim3(nblocks(kNkc[L] * 512, 256)), dim3(256), 0, s, N.prm, kWoff[L],
-                           kBoff[L], kCin[L], kCout[L], cinst, kNkc[L], N.wf[L], N.bias + 16 * L);
+                           kBoff[L], kCin[L], kCout[L], cinst, kNkc[L], L > 0 ? 1 : 0, N.wf[L], N.bias + 16 * L);
         if (L > 0)
             hipLaunchKernelGGL(k_pack_bwd, dim3(nblocks(14 * 512, 256)), dim3(256), 0, s, N.prm, kWoff[L], kCin[L],
                                kCout[L], N.wb[L]);
@@ -844,8 +990,12 @@ static void launch_fwd(int L, const Vol &v, const __bf16 *in, __bf16 *out16, flo
                     (unsigned)(nzc * v.nb)),
         blk(256);
     const bf16x8 *W = reinterpret_cast<const bf16x8 *>(N.wf[L]);
-    if (L == 0)
-        hipLaunchKernelGGL((k_conv_zm<8, 0>), grid, blk, 0, s, in, W, N.bias, v, out16, nullptr, nzc);
+    if (L == 0) {
+        const int nrc = (v.vz + kRgZ - 1) / kRgZ;
+        const dim3 rgrid((unsigned)((v.vx + kRgX - 1) / kRgX), (unsigned)((v.vy + kRgY - 1) / kRgY),
+                         (unsigned)(nrc * v.nb));
+        hipLaunchKernelGGL((k_conv_ring<8, 0>), rgrid, blk, 0, s, in, W, N.bias, v, out16, nullptr, nrc);
+    }
     else if (L < 3)
         hipLaunchKernelGGL((k_conv_zm<16, 0>), grid, blk, 0, s, in, W, N.bias + 16 * L, v, out16, nullptr, nzc);
     else

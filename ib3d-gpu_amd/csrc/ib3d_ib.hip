@@ -1017,6 +1017,60 @@ static int upload_plan(Ctx &c, ForcePlan &P, const std::vector<int> &order, cons
 // table (linear probing, keys loc*4+comp, -1 = empty).
 static void levels(const std::vector<int> &ids, const std::vector<int> &oi, int sx, int sy, std::vector<int> &src,
                    std::vector<int> &level, std::vector<std::pair<int, int>> &finals, std::vector<int> &fcomp) {
+    // The last writer of every (location, component) the updates touch.  A
+    // body's updates stay within a small box of the grid: there a dense
+    // table over the box replaces the hash (same lookups, no probing), and
+    // scanning it in index order gives the final writers already sorted.
+    // (0-based grid coordinates; location = x + sx (y + sy z).)
+    int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (int u : ids) {
+        const int *o = &oi[8 * (size_t)u];
+        const int comp = o[0];
+        const int ax = o[2] - (comp == 0) - 1, ay = o[3] - (comp == 1) - 1, az = o[4] - (comp == 2) - 1;
+        const int64_t w = o[1];
+        const int wx = (int)(w % sx), wy = (int)((w / sx) % sy), wz = (int)(w / ((int64_t)sx * sy));
+        const int cl[3] = {std::min(ax, wx), std::min(ay, wy), std::min(az, wz)};
+        const int ch[3] = {std::max(ax + 1, wx), std::max(ay + 1, wy), std::max(az + 1, wz)};
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], cl[d]);
+            hi[d] = std::max(hi[d], ch[d]);
+        }
+    }
+    const int64_t bx = ids.empty() ? 0 : hi[0] - lo[0] + 1, by = ids.empty() ? 0 : hi[1] - lo[1] + 1,
+                  bz = ids.empty() ? 0 : hi[2] - lo[2] + 1;
+    if (!ids.empty() && bx * by * bz <= ((int64_t)1 << 22)) {
+        std::vector<int> last((size_t)(bx * by * bz * 4), -1);
+        auto at = [&](int x, int y, int z, int comp) -> int & {
+            return last[(size_t)((((int64_t)(z - lo[2]) * by + (y - lo[1])) * bx + (x - lo[0])) * 4 + comp)];
+        };
+        for (int u : ids) {
+            const int *o = &oi[8 * (size_t)u];
+            const int comp = o[0];
+            const int ax = o[2] - (comp == 0) - 1, ay = o[3] - (comp == 1) - 1, az = o[4] - (comp == 2) - 1;
+            int lv = 0;
+            for (int t = 0; t < 8; t++) {
+                const int sw = at(ax + (t & 1), ay + ((t >> 1) & 1), az + (t >> 2), comp);
+                src[8 * (size_t)u + t] = sw;
+                if (sw >= 0) lv = std::max(lv, level[sw] + 1);
+            }
+            level[u] = lv;
+            const int64_t w = o[1];
+            at((int)(w % sx), (int)((w / sx) % sy), (int)(w / ((int64_t)sx * sy)), comp) = u;
+        }
+        finals.clear();
+        fcomp.clear();
+        for (int z = lo[2]; z <= hi[2]; z++)
+            for (int y = lo[1]; y <= hi[1]; y++)
+                for (int x = lo[0]; x <= hi[0]; x++)
+                    for (int comp = 0; comp < 4; comp++) {
+                        const int u = at(x, y, z, comp);
+                        if (u >= 0) {
+                            finals.push_back({(int)(x + (int64_t)sx * (y + (int64_t)sy * z)), u});
+                            fcomp.push_back(comp);
+                        }
+                    }
+        return;
+    }
     size_t cap = 16;
     while (cap < 4 * ids.size()) cap <<= 1;
     const size_t mask = cap - 1;

@@ -1013,8 +1013,9 @@ static int upload_plan(Ctx &c, ForcePlan &P, const std::vector<int> &order, cons
 }
 
 // Dependency levels of the serial loop over `ids` (in serial order).  The
-// latest writer of each (location, component) lives in an open-addressing
-// table (linear probing, keys loc*4+comp, -1 = empty).
+// latest writer of each (location, component) lives in a dense table over
+// the touched box or, for a large box, an open-addressing table (linear
+// probing, keys loc*4+comp, -1 = empty).
 static void levels(const std::vector<int> &ids, const std::vector<int> &oi, int sx, int sy, std::vector<int> &src,
                    std::vector<int> &level, std::vector<std::pair<int, int>> &finals, std::vector<int> &fcomp) {
     // The last writer of every (location, component) the updates touch.  A
@@ -1112,6 +1113,18 @@ static void levels(const std::vector<int> &ids, const std::vector<int> &oi, int 
     }
 }
 
+// ids ordered by level, serial order kept within a level (a stable
+// counting sort: levels are small integers)
+static std::vector<int> by_level(const std::vector<int> &ids, const std::vector<int> &level) {
+    int maxl = 0;
+    for (int u : ids) maxl = std::max(maxl, level[u]);
+    std::vector<int> start(maxl + 2, 0), out(ids.size());
+    for (int u : ids) start[level[u] + 1]++;
+    for (int l = 0; l <= maxl; l++) start[l + 1] += start[l];
+    for (int u : ids) out[start[level[u]]++] = u;
+    return out;
+}
+
 int build_force_plans(Ctx &c) {
     c.plans_ready = false;
     c.vplan.n_updates = c.pplan.n_updates = 0;
@@ -1165,12 +1178,10 @@ int build_force_plans(Ctx &c) {
         pids.push_back(7 * n + 6);
     }
     levels(vids, oi, (int)c.sx, (int)c.sy, src, level, finals, fcomp);
-    std::vector<int> vorder(vids);
-    std::stable_sort(vorder.begin(), vorder.end(), [&](int a, int b) { return level[a] < level[b]; });
+    const std::vector<int> vorder = by_level(vids, level);
     if (upload_plan(c, c.vplan, vorder, oi, od, src, level, finals, fcomp)) return -1;
     levels(pids, oi, (int)c.sx, (int)c.sy, src, level, finals, fcomp);
-    std::vector<int> porder(pids);
-    std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) { return level[a] < level[b]; });
+    const std::vector<int> porder = by_level(pids, level);
     if (upload_plan(c, c.pplan, porder, oi, od, src, level, finals, fcomp)) return -1;
     c.plans_ready = true;
     return 0;

@@ -25,6 +25,15 @@ __device__ __forceinline__ int64_t lin(int i, int j, int k, int sx, int sy) {
     return (int64_t)(i - 1) + (int64_t)sx * ((int64_t)(j - 1) + (int64_t)sy * (k - 1));
 }
 
+// Passes x through an empty asm so that work on it stays inside the branch
+// that needs it: in the nearest-element loops the update branch is rarely
+// taken, and a square root hoisted out of it costs ~20 FP64 instructions on
+// every element.
+__device__ __forceinline__ double unhoisted(double x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // subSearch.f90:28-35
 __global__ void k_shift_nodes(const double *x, const double *y, const double *z, double *x1, double *y1,
                               double *z1, int64_t n, double xs, double ys, double zs) {
@@ -166,58 +175,28 @@ __global__ void k_retag_mark(const int *ib, int nb, int nx, int ny, int nz, uint
     if (!(old & bit)) list[atomicAdd(count, 1)] = (int)c;
 }
 
-// subSearch.f90:364-439 for the queued cells: nearest centroid and the
-// 8-corner test, the same arithmetic as k_tagging.  A cell that was solid
-// and is no longer takes the wall velocity (the reference's body never
-// moves, so it has no rule for such cells).  Clears its queue flag.
-__global__ void __launch_bounds__(kTagTile)
-k_retag_list(const int *list, const int *count, int sx, int sy, const double *__restrict__ xp,
-             const double *__restrict__ yp, const double *__restrict__ zp, const double *__restrict__ x1,
-             const double *__restrict__ y1, const double *__restrict__ z1, const double *__restrict__ xc,
-             const double *__restrict__ yc, const double *__restrict__ zc, const double *__restrict__ ca,
-             const double *__restrict__ cb, const double *__restrict__ cg, int ne, int8_t *tag, int *minElem,
-             uint8_t *flag, double *u, double *v, double *w, double wu, double wv, double ww, int *err) {
-    __shared__ double sx_[kTagTile], sy_[kTagTile], sz_[kTagTile];
-    const int nq = *count;
-    if ((int)(blockIdx.x * blockDim.x) >= nq) return;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = q < nq;
-    int i = 0, j = 0, k = 0;
-    int64_t c = 0;
-    if (active) {
-        c = list[q];
-        i = (int)(c % sx) + 1;
-        const int64_t r = c / sx;
-        j = (int)(r % sy) + 1;
-        k = (int)(r / sy) + 1;
-    }
-    const double n1x = active ? xp[i] : 0.0, n1y = active ? yp[j] : 0.0, n1z = active ? zp[k] : 0.0;
-    double minDis = 1e14, smin = __builtin_huge_val();
-    int nel = 0;
-    for (int base = 0; base < ne; base += kTagTile) {
-        const int m = base + threadIdx.x;
-        __syncthreads();
-        if (m < ne) {
-            sx_[threadIdx.x] = xc[m];
-            sy_[threadIdx.x] = yc[m];
-            sz_[threadIdx.x] = zc[m];
-        }
-        __syncthreads();
-        const int cnt = min(kTagTile, ne - base);
-        for (int t = 0; t < cnt; t++) {
-            const double dy = n1y - sy_[t], dx = n1x - sx_[t], dz = n1z - sz_[t];
-            const double s = ((dy * dy) + (dx * dx)) + (dz * dz);
-            if (s < smin) {
-                const double dis = sqrt(s);
-                if (dis < minDis) {
-                    minDis = dis;
-                    smin = s;
-                    nel = base + t + 1;
-                }
-            }
+// One step of the first-nearest-element scan: sqrt only when the square is
+// below the current minimum's (sqrt is monotone, so a larger square can
+// never give a strictly smaller distance); ties keep the earlier element.
+__device__ __forceinline__ void nearest_update(double s, int elem1, double &smin, double &minDis, int &nel) {
+    if (s < smin) {
+        const double dis = sqrt(unhoisted(s));
+        if (dis < minDis) {
+            minDis = dis;
+            smin = s;
+            nel = elem1;
         }
     }
-    if (!active) return;
+}
+
+// Tag of one queued cell from its nearest element nel (1-based, 0 = none).
+__device__ __forceinline__ void retag_cell(int64_t c, int i, int j, int k, int nel, const double *__restrict__ x1,
+                                           const double *__restrict__ y1, const double *__restrict__ z1,
+                                           const double *__restrict__ xc, const double *__restrict__ yc,
+                                           const double *__restrict__ zc, const double *__restrict__ ca,
+                                           const double *__restrict__ cb, const double *__restrict__ cg, int8_t *tag,
+                                           int *minElem, uint8_t *flag, double *u, double *v, double *w, double wu,
+                                           double wv, double ww, int *err) {
     flag[c] = 0;
     if (nel < 1) {
         atomicOr(err, 1);
@@ -251,6 +230,79 @@ k_retag_list(const int *list, const int *count, int sx, int sy, const double *__
     minElem[c] = nel;
 }
 
+// subSearch.f90:364-439 for the queued cells: nearest centroid and the
+// 8-corner test, the same arithmetic as k_tagging.  A cell that was solid
+// and is no longer takes the wall velocity (the reference's body never
+// moves, so it has no rule for such cells).  Clears its queue flag.
+// A block takes 64 queued cells at a time; its 16 waves each scan one
+// contiguous slice of the elements (uniform loads), and wave 0 merges the
+// slices in element order with a strict '<', which gives the same first
+// nearest element as one pass over all of them.
+static constexpr int kRtSlices = 16;
+__global__ void __launch_bounds__(64 * kRtSlices)
+k_retag_list(const int *list, const int *count, int sx, int sy, const double *__restrict__ xp,
+             const double *__restrict__ yp, const double *__restrict__ zp, const double *__restrict__ x1,
+             const double *__restrict__ y1, const double *__restrict__ z1, const double *__restrict__ xc,
+             const double *__restrict__ yc, const double *__restrict__ zc, const double *__restrict__ ca,
+             const double *__restrict__ cb, const double *__restrict__ cg, int ne, int8_t *tag, int *minElem,
+             uint8_t *flag, double *u, double *v, double *w, double wu, double wv, double ww, int *err) {
+    __shared__ double s_dis[kRtSlices][64];
+    __shared__ int s_nel[kRtSlices][64];
+    const int nq = *count;
+    const int lane = threadIdx.x & 63;
+    const int sl = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int chunk = (ne + kRtSlices - 1) / kRtSlices;
+    const int m0 = min(ne, sl * chunk), m1 = min(ne, m0 + chunk);
+    for (int q0 = blockIdx.x * 64; q0 < nq; q0 += gridDim.x * 64) {
+        const int q = q0 + lane;
+        const bool active = q < nq;
+        int i = 0, j = 0, k = 0;
+        int64_t c = 0;
+        if (active) {
+            c = list[q];
+            i = (int)(c % sx) + 1;
+            const int64_t r = c / sx;
+            j = (int)(r % sy) + 1;
+            k = (int)(r / sy) + 1;
+        }
+        const double n1x = active ? xp[i] : 0.0, n1y = active ? yp[j] : 0.0, n1z = active ? zp[k] : 0.0;
+        double minDis = 1e14, smin = __builtin_huge_val();
+        int nel = 0;
+        // four squared distances per step (their loads batch), then the
+        // updates in element order
+        int m = m0;
+        for (; m + 4 <= m1; m += 4) {
+            double s4[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const double dy = n1y - yc[m + t], dx = n1x - xc[m + t], dz = n1z - zc[m + t];
+                s4[t] = ((dy * dy) + (dx * dx)) + (dz * dz);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) nearest_update(s4[t], m + t + 1, smin, minDis, nel);
+        }
+        for (; m < m1; m++) {
+            const double dy = n1y - yc[m], dx = n1x - xc[m], dz = n1z - zc[m];
+            nearest_update(((dy * dy) + (dx * dx)) + (dz * dz), m + 1, smin, minDis, nel);
+        }
+        s_dis[sl][lane] = minDis;
+        s_nel[sl][lane] = nel;
+        __syncthreads();
+        if (sl == 0 && active) {
+            for (int t = 1; t < kRtSlices; t++) {
+                const double d = s_dis[t][lane];
+                if (d < minDis) {
+                    minDis = d;
+                    nel = s_nel[t][lane];
+                }
+            }
+            retag_cell(c, i, j, k, nel, x1, y1, z1, xc, yc, zc, ca, cb, cg, tag, minElem, flag, u, v, w, wu, wv,
+                       ww, err);
+        }
+        __syncthreads();  // s_dis and s_nel are rewritten by the next group
+    }
+}
+
 // per interior row (j,k): counts of fluid, solid, intercepted, red-fluid
 // Per-row counts of fluid (tag 0), solid (1), intercepted (2) and red
 // fluid cells; one wave per row, 64 consecutive cells per ballot.
@@ -279,35 +331,54 @@ __global__ void __launch_bounds__(256) k_row_counts(const int8_t *tag, int nx, i
     }
 }
 
-// exclusive scan of the 4 count arrays (one block, in place), totals at the end
-__global__ void k_scan_counts(int *counts, int nrows, int *totals) {
-    __shared__ int sm[1024];
-    for (int a = 0; a < 4; a++) {
-        int *cnt = counts + (int64_t)a * nrows;
-        const int per = (nrows + blockDim.x - 1) / blockDim.x;
-        const int lo = threadIdx.x * per, hi = min(nrows, lo + per);
-        int s = 0;
-        for (int r = lo; r < hi; r++) s += cnt[r];
-        sm[threadIdx.x] = s;
+// Exclusive scan, in place, of the 4 per-row count arrays; one block per
+// array. Each pass covers 8192 rows: 8 consecutive rows per thread, a wave
+// scan of the thread sums, then a scan of the 16 wave sums; the running
+// carry moves to the next pass. totals[a] = sum of array a.
+__global__ void __launch_bounds__(1024) k_scan_counts(int *counts, int nrows, int *totals) {
+    __shared__ int wsum[16];
+    __shared__ int pass_total;
+    int *cnt = counts + (int64_t)blockIdx.x * nrows;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int carry = 0;
+    for (int base = 0; base < nrows; base += 8192) {
+        const int lo = base + (int)threadIdx.x * 8;
+        int v[8], s = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            v[q] = lo + q < nrows ? cnt[lo + q] : 0;
+            s += v[q];
+        }
+        int x = s;  // inclusive scan over the wave
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            int run = 0;
-            for (int t = 0; t < (int)blockDim.x; t++) {
-                const int v = sm[t];
-                sm[t] = run;
-                run += v;
+        if (wv == 0) {
+            const int w = lane < 16 ? wsum[lane] : 0;
+            int z = w;
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                const int y = __shfl_up(z, d, 64);
+                if (lane >= d) z += y;
             }
-            totals[a] = run;
+            if (lane < 16) wsum[lane] = z - w;
+            if (lane == 15) pass_total = z;
         }
         __syncthreads();
-        int run = sm[threadIdx.x];
-        for (int r = lo; r < hi; r++) {
-            const int v = cnt[r];
-            cnt[r] = run;
-            run += v;
+        int run = carry + wsum[wv] + x - s;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (lo + q < nrows) cnt[lo + q] = run;
+            run += v[q];
         }
-        __syncthreads();
+        carry += pass_total;
+        __syncthreads();  // wsum and pass_total are rewritten by the next pass
     }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 // order-preserving scatter into (count,3) lists, (k,j,i) order; one wave
@@ -771,7 +842,8 @@ int selective_retagging(Ctx &c, int32_t counts[3]) {
     if (nb > 0) {
         hipLaunchKernelGGL(k_retag_mark, dim3(nblocks(216 * nb, 256)), dim3(256), 0, c.stream, c.ibList, (int)nb,
                            (int)c.nx, (int)c.ny, (int)c.nz, c.rt_flag, c.rt_list, c.rt_count);
-        hipLaunchKernelGGL(k_retag_list, dim3(nblocks(cap, kTagTile)), dim3(kTagTile), 0, c.stream, c.rt_list,
+        const unsigned grid = (unsigned)std::min<int64_t>(nblocks(cap, 64), 1024);
+        hipLaunchKernelGGL(k_retag_list, dim3(grid), dim3(64 * kRtSlices), 0, c.stream, c.rt_list,
                            c.rt_count, (int)c.sx, (int)c.sy, c.xp, c.yp, c.zp, c.x1, c.y1, c.z1, c.xcent, c.ycent,
                            c.zcent, c.cosA, c.cosB, c.cosG, (int)c.ibElems, c.tag, c.minElem, c.rt_flag, c.u, c.v,
                            c.w, c.wall_vel[0], c.wall_vel[1], c.wall_vel[2], g_err);
@@ -848,7 +920,7 @@ int count_tags(Ctx &c, int32_t out[4], bool scatter) {
     int *cnt = g_cnt, *tot = g_tot;
     hipLaunchKernelGGL(k_row_counts, dim3(nblocks(nrows, 4)), dim3(256), 0, c.stream, c.tag, (int)c.nx, (int)c.ny,
                        (int)c.nz, cnt);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c.stream, cnt, nrows, tot);
+    hipLaunchKernelGGL(k_scan_counts, dim3(4), dim3(1024), 0, c.stream, cnt, nrows, tot);
     IB3D_CHECK(hipGetLastError());
     int h[4];
     IB3D_CHECK(hipMemcpyAsync(h, tot, sizeof h, hipMemcpyDeviceToHost, c.stream));

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <thread>
 #include <unordered_map>
 
 namespace ib3d {
@@ -162,17 +163,40 @@ k_tagging(int i0, int i1, int j0, int j1, int k0, int k1, int sx, int sy, const 
 // intercepted cell; each cell enters the list once (the reference tags
 // overlapping blocks repeatedly, always with the same result).  Cells
 // outside the padded index range 1..n+2 are not queued.
-__global__ void k_retag_mark(const int *ib, int nb, int nx, int ny, int nz, uint8_t *flag, int *list, int *count) {
+__global__ void __launch_bounds__(1024)
+k_retag_mark(const int *ib, int nb, int nx, int ny, int nz, uint8_t *flag, int *list, int *count) {
+    // one atomicAdd on the list counter per block: a cell's slot = block base
+    // + cells newly queued by lower waves + set ballot bits below its lane
+    __shared__ int s_wave[16];
+    __shared__ int s_base;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (int64_t)nb * 216) return;
-    const int n = (int)(q / 216), o = (int)(q % 216);
-    const int i = ib[n] - 2 + o % 6, j = ib[nb + n] - 2 + (o / 6) % 6, k = ib[2 * nb + n] - 2 + o / 36;
-    if (i < 1 || i > nx + 2 || j < 1 || j > ny + 2 || k < 1 || k > nz + 2) return;
-    const int64_t c = lin(i, j, k, nx + 2, ny + 2);
-    unsigned int *word = reinterpret_cast<unsigned int *>(flag + (c & ~(int64_t)3));
-    const unsigned int bit = 1u << (8 * (c & 3));
-    const unsigned int old = atomicOr(word, bit);
-    if (!(old & bit)) list[atomicAdd(count, 1)] = (int)c;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    bool add = false;
+    int64_t c = 0;
+    if (q < (int64_t)nb * 216) {
+        const int n = (int)(q / 216), o = (int)(q % 216);
+        const int i = ib[n] - 2 + o % 6, j = ib[nb + n] - 2 + (o / 6) % 6, k = ib[2 * nb + n] - 2 + o / 36;
+        if (i >= 1 && i <= nx + 2 && j >= 1 && j <= ny + 2 && k >= 1 && k <= nz + 2) {
+            c = lin(i, j, k, nx + 2, ny + 2);
+            unsigned int *word = reinterpret_cast<unsigned int *>(flag + (c & ~(int64_t)3));
+            const unsigned int bit = 1u << (8 * (c & 3));
+            add = !(atomicOr(word, bit) & bit);
+        }
+    }
+    const unsigned long long bal = __ballot(add);
+    if (lane == 0) s_wave[wv] = __popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int t = 0; t < (int)(blockDim.x >> 6); t++) {
+            const int v = s_wave[t];
+            s_wave[t] = run;
+            run += v;
+        }
+        s_base = run > 0 ? atomicAdd(count, run) : 0;
+    }
+    __syncthreads();
+    if (add) list[s_base + s_wave[wv] + __popcll(bal & ((1ull << lane) - 1))] = (int)c;
 }
 
 // One step of the first-nearest-element scan: sqrt only when the square is
@@ -840,7 +864,7 @@ int selective_retagging(Ctx &c, int32_t counts[3]) {
     IB3D_CHECK(hipMemsetAsync(c.rt_count, 0, sizeof(int), c.stream));
     if (dev_err(c)) return -1;
     if (nb > 0) {
-        hipLaunchKernelGGL(k_retag_mark, dim3(nblocks(216 * nb, 256)), dim3(256), 0, c.stream, c.ibList, (int)nb,
+        hipLaunchKernelGGL(k_retag_mark, dim3(nblocks(216 * nb, 1024)), dim3(1024), 0, c.stream, c.ibList, (int)nb,
                            (int)c.nx, (int)c.ny, (int)c.nz, c.rt_flag, c.rt_list, c.rt_count);
         const unsigned grid = (unsigned)std::min<int64_t>(nblocks(cap, 64), 1024);
         hipLaunchKernelGGL(k_retag_list, dim3(grid), dim3(64 * kRtSlices), 0, c.stream, c.rt_list,
@@ -998,17 +1022,32 @@ static void free_plan(ForcePlan &p) {
 static int *g_doi = nullptr;        // device: stencil records of the 7 updates per IB cell
 static double *g_dod = nullptr;
 static int64_t g_dcap = 0;
-static void *g_pin = nullptr;       // pinned host staging
-static size_t g_pin_cap = 0;
+struct PinBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+static PinBuf g_pin;     // pinned staging of the plan uploads
+static PinBuf g_pin_dl;  // pinned copy of the stencil records, read in place by the host passes
 
-static int pinned(size_t bytes) {
-    if (bytes <= g_pin_cap) return 0;
-    if (g_pin) IB3D_CHECK(hipHostFree(g_pin));
-    g_pin = nullptr;
-    IB3D_CHECK(hipHostMalloc(&g_pin, bytes, hipHostMallocDefault));
-    g_pin_cap = bytes;
+static int pinned(PinBuf &b, size_t bytes) {
+    if (bytes <= b.cap) return 0;
+    if (b.p) IB3D_CHECK(hipHostFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    IB3D_CHECK(hipHostMalloc(&b.p, bytes, hipHostMallocDefault));
+    b.cap = bytes;
     return 0;
 }
+
+// Host work arrays of one plan, kept across rebuilds.  src and level are
+// indexed by update id and written for every id of the plan before they
+// are read, so they are never cleared.
+struct PlanWork {
+    std::vector<int> src, level, pos_of, last, order;
+    std::vector<std::pair<int, int>> finals;
+    std::vector<int> fcomp;
+};
+static PlanWork g_vwork, g_pwork;
 
 void release_plans(Ctx &c) {
     free_plan(c.vplan);
@@ -1018,20 +1057,24 @@ void release_plans(Ctx &c) {
     g_doi = nullptr;
     g_dod = nullptr;
     g_dcap = 0;
-    if (g_pin) (void)hipHostFree(g_pin);
-    g_pin = nullptr;
-    g_pin_cap = 0;
+    for (PinBuf *b : {&g_pin, &g_pin_dl}) {
+        if (b->p) (void)hipHostFree(b->p);
+        b->p = nullptr;
+        b->cap = 0;
+    }
+    g_vwork = PlanWork();
+    g_pwork = PlanWork();
 }
 
 // Upload a plan: the updates in level order (8 stencil ints, 8 sources, 5
 // doubles each) and the final-writer table [loc | upd | comp] (3*n_final
 // ints).  Device buffers grow only; all arrays go through one pinned staging
 // area and are copied with a single wait.
-static int upload_plan(Ctx &c, ForcePlan &P, const std::vector<int> &order, const std::vector<int> &oi,
-                       const std::vector<double> &od, const std::vector<int> &src, const std::vector<int> &level,
-                       const std::vector<std::pair<int, int>> &finals, const std::vector<int> &fcomp) {
+static int upload_plan(Ctx &c, ForcePlan &P, PlanWork &W, const int *oi, const double *od) {
+    const std::vector<int> &order = W.order, &src = W.src, &level = W.level, &fcomp = W.fcomp;
+    const std::vector<std::pair<int, int>> &finals = W.finals;
     const int n = (int)order.size();
-    std::vector<int> pos_of(oi.size() / 8, -1);
+    std::vector<int> &pos_of = W.pos_of;  // read only at ids of this plan
     for (int q = 0; q < n; q++) pos_of[order[q]] = q;
     const int m = std::max(1, n), nf = std::max(1, (int)finals.size());
     if (m > P.cap_upd) {
@@ -1051,8 +1094,8 @@ static int upload_plan(Ctx &c, ForcePlan &P, const std::vector<int> &order, cons
     }
     const size_t b_oi = 8 * (size_t)m * sizeof(int), b_od = 5 * (size_t)m * sizeof(double),
                  b_fin = 3 * (size_t)nf * sizeof(int);
-    if (pinned(b_od + 2 * b_oi + b_fin)) return -1;
-    double *sod = static_cast<double *>(g_pin);
+    if (pinned(g_pin, b_od + 2 * b_oi + b_fin)) return -1;
+    double *sod = static_cast<double *>(g_pin.p);
     int *soi = reinterpret_cast<int *>(sod + 5 * (size_t)m), *ssrc = soi + 8 * (size_t)m, *fin = ssrc + 8 * (size_t)m;
     int maxl = -1;
     for (int q = 0; q < n; q++) {
@@ -1088,8 +1131,9 @@ static int upload_plan(Ctx &c, ForcePlan &P, const std::vector<int> &order, cons
 // latest writer of each (location, component) lives in a dense table over
 // the touched box or, for a large box, an open-addressing table (linear
 // probing, keys loc*4+comp, -1 = empty).
-static void levels(const std::vector<int> &ids, const std::vector<int> &oi, int sx, int sy, std::vector<int> &src,
-                   std::vector<int> &level, std::vector<std::pair<int, int>> &finals, std::vector<int> &fcomp) {
+static void levels(const std::vector<int> &ids, const int *oi, int sx, int sy, PlanWork &W) {
+    std::vector<int> &src = W.src, &level = W.level, &fcomp = W.fcomp;
+    std::vector<std::pair<int, int>> &finals = W.finals;
     // The last writer of every (location, component) the updates touch.  A
     // body's updates stay within a small box of the grid: there a dense
     // table over the box replaces the hash (same lookups, no probing), and
@@ -1112,7 +1156,8 @@ static void levels(const std::vector<int> &ids, const std::vector<int> &oi, int 
     const int64_t bx = ids.empty() ? 0 : hi[0] - lo[0] + 1, by = ids.empty() ? 0 : hi[1] - lo[1] + 1,
                   bz = ids.empty() ? 0 : hi[2] - lo[2] + 1;
     if (!ids.empty() && bx * by * bz <= ((int64_t)1 << 22)) {
-        std::vector<int> last((size_t)(bx * by * bz * 4), -1);
+        std::vector<int> &last = W.last;
+        last.assign((size_t)(bx * by * bz * 4), -1);
         auto at = [&](int x, int y, int z, int comp) -> int & {
             return last[(size_t)((((int64_t)(z - lo[2]) * by + (y - lo[1])) * bx + (x - lo[0])) * 4 + comp)];
         };
@@ -1187,14 +1232,25 @@ static void levels(const std::vector<int> &ids, const std::vector<int> &oi, int 
 
 // ids ordered by level, serial order kept within a level (a stable
 // counting sort: levels are small integers)
-static std::vector<int> by_level(const std::vector<int> &ids, const std::vector<int> &level) {
+static void by_level(const std::vector<int> &ids, PlanWork &W) {
+    const std::vector<int> &level = W.level;
     int maxl = 0;
     for (int u : ids) maxl = std::max(maxl, level[u]);
-    std::vector<int> start(maxl + 2, 0), out(ids.size());
+    std::vector<int> start(maxl + 2, 0);
+    std::vector<int> &out = W.order;
+    out.resize(ids.size());
     for (int u : ids) start[level[u] + 1]++;
     for (int l = 0; l <= maxl; l++) start[l + 1] += start[l];
     for (int u : ids) out[start[level[u]]++] = u;
-    return out;
+}
+
+// Levels, order and final writers of one plan, in its work arrays.
+static void plan_host(const std::vector<int> &ids, const int *oi, int sx, int sy, size_t nu, PlanWork &W) {
+    if (W.src.size() < 8 * nu) W.src.resize(8 * nu);
+    if (W.level.size() < nu) W.level.resize(nu);
+    if (W.pos_of.size() < nu) W.pos_of.resize(nu);
+    levels(ids, oi, sx, sy, W);
+    by_level(ids, W);
 }
 
 int build_force_plans(Ctx &c) {
@@ -1228,19 +1284,14 @@ int build_force_plans(Ctx &c) {
     hipLaunchKernelGGL(k_force_stencil, dim3(nblocks(nu7, 256)), dim3(256), 0, c.stream, A, g_doi, g_dod, g_err);
     IB3D_CHECK(hipGetLastError());
     const size_t b_oi = 8 * (size_t)nu7 * sizeof(int), b_od = 5 * (size_t)nu7 * sizeof(double);
-    if (pinned(b_oi + b_od)) return -1;
-    IB3D_CHECK(hipMemcpyAsync(g_pin, g_doi, b_oi, hipMemcpyDeviceToHost, c.stream));
-    IB3D_CHECK(hipMemcpyAsync(static_cast<char *>(g_pin) + b_oi, g_dod, b_od, hipMemcpyDeviceToHost, c.stream));
+    if (pinned(g_pin_dl, b_oi + b_od)) return -1;
+    IB3D_CHECK(hipMemcpyAsync(g_pin_dl.p, g_doi, b_oi, hipMemcpyDeviceToHost, c.stream));
+    IB3D_CHECK(hipMemcpyAsync(static_cast<char *>(g_pin_dl.p) + b_oi, g_dod, b_od, hipMemcpyDeviceToHost, c.stream));
     const int err = read_err(c, g_err);  // waits for the stream
     IB3D_REQUIRE(err == 0, "forcing: an image point has no bracketing grid lines");
-    const int *poi = static_cast<const int *>(g_pin);
-    const double *pod = reinterpret_cast<const double *>(static_cast<char *>(g_pin) + b_oi);
-    const std::vector<int> oi(poi, poi + 8 * nu7);
-    const std::vector<double> od(pod, pod + 5 * nu7);
+    const int *oi = static_cast<const int *>(g_pin_dl.p);
+    const double *od = reinterpret_cast<const double *>(static_cast<char *>(g_pin_dl.p) + b_oi);
     const size_t nu = 7 * (size_t)nb;
-    std::vector<int> src(8 * nu, -1), level(nu, 0);
-    std::vector<std::pair<int, int>> finals;
-    std::vector<int> fcomp;
     // velocity: per cell u2 u1 v2 v1 w2 w1 in serial order
     std::vector<int> vids, pids;
     vids.reserve(6 * (size_t)nb);
@@ -1249,12 +1300,21 @@ int build_force_plans(Ctx &c) {
         for (int kind = 0; kind < 6; kind++) vids.push_back(7 * n + kind);
         pids.push_back(7 * n + 6);
     }
-    levels(vids, oi, (int)c.sx, (int)c.sy, src, level, finals, fcomp);
-    const std::vector<int> vorder = by_level(vids, level);
-    if (upload_plan(c, c.vplan, vorder, oi, od, src, level, finals, fcomp)) return -1;
-    levels(pids, oi, (int)c.sx, (int)c.sy, src, level, finals, fcomp);
-    const std::vector<int> porder = by_level(pids, level);
-    if (upload_plan(c, c.pplan, porder, oi, od, src, level, finals, fcomp)) return -1;
+    // The two plans share no update and no work array: the pressure plan's
+    // levels are found on a second thread while the velocity plan's are
+    // found and uploaded.
+    const int sx = (int)c.sx, sy = (int)c.sy;
+    std::thread pth;
+    try {
+        pth = std::thread([&] { plan_host(pids, oi, sx, sy, nu, g_pwork); });
+    } catch (...) {
+        plan_host(pids, oi, sx, sy, nu, g_pwork);
+    }
+    plan_host(vids, oi, sx, sy, nu, g_vwork);
+    const int rv = upload_plan(c, c.vplan, g_vwork, oi, od);
+    if (pth.joinable()) pth.join();
+    if (rv) return -1;
+    if (upload_plan(c, c.pplan, g_pwork, oi, od)) return -1;
     c.plans_ready = true;
     return 0;
 }

@@ -607,6 +607,7 @@ int ib3d_upload(const char *name, const void *host, int64_t nbytes) {
             }
             IB3D_CHECK(h2d(c.tag, t.data(), c.P));
             c.lists_ready = c.normdist_ready = c.plans_ready = false;
+            c.plans_pending = false;
             break;
         }
         case F64_VEC: {

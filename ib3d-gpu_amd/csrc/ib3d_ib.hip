@@ -16,6 +16,7 @@
 #include "ib3d_internal.h"
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <thread>
 #include <unordered_map>
@@ -880,6 +881,7 @@ int selective_retagging(Ctx &c, int32_t counts[3]) {
     counts[1] = t4[0];
     counts[2] = t4[1];
     c.lists_ready = c.normdist_ready = c.plans_ready = false;
+    c.plans_pending = false;
     c.stress_ready = false;
     return 0;
 }
@@ -907,6 +909,7 @@ int tagging(Ctx &c, const int64_t w[6], int32_t counts[3]) {
     c.lists_ready = false;
     c.normdist_ready = false;
     c.plans_ready = false;
+    c.plans_pending = false;
     return 0;
 }
 
@@ -980,6 +983,7 @@ int cell_count(Ctx &c, int32_t counts[5]) {
     c.lists_ready = true;
     c.normdist_ready = false;
     c.plans_ready = false;
+    c.plans_pending = false;
     return 0;
 }
 
@@ -1004,7 +1008,7 @@ int norm_distance(Ctx &c) {
     IB3D_CHECK(hipGetLastError());
     IB3D_REQUIRE(read_err(c, g_err) == 0, "computeNormDistance: intercepted cell outside the search window");
     c.normdist_ready = true;
-    return build_force_plans(c);
+    return plans_launch(c);
 }
 
 static void free_plan(ForcePlan &p) {
@@ -1253,8 +1257,14 @@ static void plan_host(const std::vector<int> &ids, const int *oi, int sx, int sy
     by_level(ids, W);
 }
 
-int build_force_plans(Ctx &c) {
+// The forcing plans are built in two halves.  plans_launch (at the end of
+// computeNormDistance) queues the stencil kernel and the copy of its records
+// and error flag to pinned memory, and records an event.  plans_finish, at
+// the first forcing call, waits for that event only: the predictor's kernels
+// queued after it keep the GPU busy while the host finds the levels.
+int plans_launch(Ctx &c) {
     c.plans_ready = false;
+    c.plans_pending = false;
     c.vplan.n_updates = c.pplan.n_updates = 0;
     c.vplan.n_final = c.pplan.n_final = 0;
     const int nb = (int)c.ibCount;
@@ -1284,13 +1294,28 @@ int build_force_plans(Ctx &c) {
     hipLaunchKernelGGL(k_force_stencil, dim3(nblocks(nu7, 256)), dim3(256), 0, c.stream, A, g_doi, g_dod, g_err);
     IB3D_CHECK(hipGetLastError());
     const size_t b_oi = 8 * (size_t)nu7 * sizeof(int), b_od = 5 * (size_t)nu7 * sizeof(double);
-    if (pinned(g_pin_dl, b_oi + b_od)) return -1;
-    IB3D_CHECK(hipMemcpyAsync(g_pin_dl.p, g_doi, b_oi, hipMemcpyDeviceToHost, c.stream));
-    IB3D_CHECK(hipMemcpyAsync(static_cast<char *>(g_pin_dl.p) + b_oi, g_dod, b_od, hipMemcpyDeviceToHost, c.stream));
-    const int err = read_err(c, g_err);  // waits for the stream
+    if (pinned(g_pin_dl, b_oi + b_od + sizeof(int))) return -1;
+    char *dl = static_cast<char *>(g_pin_dl.p);
+    IB3D_CHECK(hipMemcpyAsync(dl, g_doi, b_oi, hipMemcpyDeviceToHost, c.stream));
+    IB3D_CHECK(hipMemcpyAsync(dl + b_oi, g_dod, b_od, hipMemcpyDeviceToHost, c.stream));
+    IB3D_CHECK(hipMemcpyAsync(dl + b_oi + b_od, g_err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    IB3D_CHECK(hipEventRecord(c.ev[4], c.stream));
+    c.plans_pending = true;
+    return 0;
+}
+
+static int plans_finish(Ctx &c) {
+    c.plans_pending = false;
+    const int nb = (int)c.ibCount;  // unchanged since plans_launch: a new list clears the pending flag
+    const int64_t nu7 = 7 * (int64_t)nb;
+    const size_t b_oi = 8 * (size_t)nu7 * sizeof(int), b_od = 5 * (size_t)nu7 * sizeof(double);
+    IB3D_CHECK(hipEventSynchronize(c.ev[4]));
+    const char *dl = static_cast<const char *>(g_pin_dl.p);
+    int err = 0;
+    std::memcpy(&err, dl + b_oi + b_od, sizeof(int));
     IB3D_REQUIRE(err == 0, "forcing: an image point has no bracketing grid lines");
     const int *oi = static_cast<const int *>(g_pin_dl.p);
-    const double *od = reinterpret_cast<const double *>(static_cast<char *>(g_pin_dl.p) + b_oi);
+    const double *od = reinterpret_cast<const double *>(dl + b_oi);
     const size_t nu = 7 * (size_t)nb;
     // velocity: per cell u2 u1 v2 v1 w2 w1 in serial order
     std::vector<int> vids, pids;
@@ -1344,11 +1369,13 @@ static int run_plan(Ctx &c, ForcePlan &P) {
 }
 
 int velocity_forcing(Ctx &c) {
+    if (c.plans_pending && plans_finish(c)) return -1;
     IB3D_REQUIRE(c.plans_ready, "velocityForcing1: computeNormDistance has not run");
     return run_plan(c, c.vplan);
 }
 
 int pressure_forcing(Ctx &c) {
+    if (c.plans_pending && plans_finish(c)) return -1;
     IB3D_REQUIRE(c.plans_ready, "pressureForcing1: computeNormDistance has not run");
     return run_plan(c, c.pplan);
 }

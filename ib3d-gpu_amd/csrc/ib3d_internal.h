@@ -136,6 +136,7 @@ struct Ctx {
     bool normdist_ready = false;
     ForcePlan vplan, pplan;
     bool plans_ready = false;
+    bool plans_pending = false;    // stencils computed and on their way to the host; plans not built yet
 
     // stress stencil cache [5][3][ibElems]: Elemcell, ucell, vcell, wcell, pcell
     int *scell = nullptr;
@@ -176,7 +177,7 @@ struct Ctx {
     int64_t geom_version = 0;      // bumped whenever cellCount rebuilds the lists (the tags changed)
     double last_relres = 0.0, last_r0 = 0.0;
     double last_sor_ms = 0.0, last_poisson_ms = 0.0;
-    hipEvent_t ev[4] = {};
+    hipEvent_t ev[5] = {};         // [0..3] solve timing, [4] forcing stencils copied to the host
 };
 
 Ctx &ctx();
@@ -254,7 +255,7 @@ int fields_init(Ctx &c);
 int poisson_solve(Ctx &c, double deltat, double eps1, int64_t *niter, double *divmax, double *derr_std);
 int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2);
 int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);
-int build_force_plans(Ctx &c);
+int plans_launch(Ctx &c);
 int shift_nodes_moving(Ctx &c, const double base[3], const double disp[3]);
 int selective_retagging(Ctx &c, int32_t counts[3]);
 int stress_bodies(Ctx &c, double *coef);

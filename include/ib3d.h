@@ -94,7 +94,9 @@ int ib3d_computeSurfaceNorm(int64_t inor);
 int ib3d_tagging(const int64_t window[6], int32_t counts[3]);
 /* subSearch.f90:515-596; counts = {ib, fluid, solid, red, black} */
 int ib3d_cellCount(int32_t counts[5]);
-/* subSearch.f90:599-678 (also prepares the forcing plans) */
+/* subSearch.f90:599-678.  Also starts the forcing plans: their stencils are
+ * computed here and the plans are finished by the next velocityForcing1 or
+ * pressureForcing1, which also reports a stencil error. */
 int ib3d_computeNormDistance(void);
 /* subInitialConditions.f90:2-48 */
 int ib3d_initialConditions(void);

@@ -28,9 +28,9 @@ __device__ __forceinline__ int64_t lin(int i, int j, int k, int sx, int sy) {
 }
 
 // Passes x through an empty asm so that work on it stays inside the branch
-// that needs it: in the nearest-element loops the update branch is rarely
-// taken, and a square root hoisted out of it costs ~20 FP64 instructions on
-// every element.
+// that needs it: in the retagging's nearest-element loop the update branch
+// is rarely taken, and a square root hoisted out of it costs ~20 FP64
+// instructions on every element.
 __device__ __forceinline__ double unhoisted(double x) {
     asm volatile("" : "+v"(x));
     return x;

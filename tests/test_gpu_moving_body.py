@@ -120,3 +120,37 @@ def test_warm_started_modes_agree_with_cold_fp64_on_a_moving_body(case_dir):
             e = float(np.max(np.abs(runs[name][f] - runs["cold"][f])) / np.max(np.abs(runs["cold"][f])))
             _say(f"[gpu] moving body, {name} warm vs cold FP64: {f} rel diff {e:.2e}")
             assert e < 1e-6, (name, f, e)
+
+
+def test_forcing_plans_follow_the_latest_geometry(case_dir):
+    """computeNormDistance only queues the forcing stencils; the plans are
+    finished by the next forcing call.  A cellCount after it (new lists)
+    drops the queued plans, so the forcing call fails as it would without
+    computeNormDistance; computeNormDistance again, and the forcing matches
+    a run without the extra calls, bit for bit."""
+    import ctypes as C
+    from ib3d_amd.native import IB3DError, Solver
+    d = case_dir("tiny")
+
+    def run(extra):
+        s = Solver.from_case(d)
+        try:
+            s.move_body((0.0, 0.02, 0.0))
+            s.nsMomentum()
+            s.velocityBC()
+            s.solidCellBC()
+            if extra:
+                cnt5 = np.zeros(5, dtype=np.int32)
+                assert s.L.ib3d_cellCount(cnt5.ctypes.data_as(C.POINTER(C.c_int32))) == 0
+                with pytest.raises(IB3DError, match="computeNormDistance has not run"):
+                    s.velocityForcing1()
+                assert s.L.ib3d_computeNormDistance() == 0
+                assert s.L.ib3d_computeNormDistance() == 0   # a second queue replaces the first
+            s.velocityForcing1()
+            s.pressureForcing1()
+            return {f: dig(s.download(f)) for f in ("ut", "vt", "wt", "p")}
+        finally:
+            s.close()
+
+    plain = run(False)
+    assert run(True) == plain

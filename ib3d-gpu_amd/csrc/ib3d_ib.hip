@@ -356,54 +356,77 @@ __global__ void __launch_bounds__(256) k_row_counts(const int8_t *tag, int nx, i
     }
 }
 
-// Exclusive scan, in place, of the 4 per-row count arrays; one block per
-// array. Each pass covers 8192 rows: 8 consecutive rows per thread, a wave
-// scan of the thread sums, then a scan of the 16 wave sums; the running
-// carry moves to the next pass. totals[a] = sum of array a.
-__global__ void __launch_bounds__(1024) k_scan_counts(int *counts, int nrows, int *totals) {
-    __shared__ int wsum[16];
-    __shared__ int pass_total;
-    int *cnt = counts + (int64_t)blockIdx.x * nrows;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int carry = 0;
-    for (int base = 0; base < nrows; base += 8192) {
-        const int lo = base + (int)threadIdx.x * 8;
-        int v[8], s = 0;
+// Exclusive scan, in place, of the 4 per-row count arrays in tiles of
+// kScanTile rows: k_tile_sums sums every tile, then k_scan_counts scans each
+// tile from its offset (the sum of the tiles before it).  Grids are
+// (tiles, 4); 1024 threads take 8 consecutive rows each.  totals[a] = sum
+// of array a, written by the last tile's block.
+static constexpr int kScanTile = 8192;
+__device__ __forceinline__ void load_rows8(const int *cnt, int lo, int nrows, int v[8], int &s) {
+    s = 0;
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            v[q] = lo + q < nrows ? cnt[lo + q] : 0;
-            s += v[q];
-        }
-        int x = s;  // inclusive scan over the wave
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
-        }
-        if (lane == 63) wsum[wv] = x;
-        __syncthreads();
-        if (wv == 0) {
-            const int w = lane < 16 ? wsum[lane] : 0;
-            int z = w;
-#pragma unroll
-            for (int d = 1; d < 16; d <<= 1) {
-                const int y = __shfl_up(z, d, 64);
-                if (lane >= d) z += y;
-            }
-            if (lane < 16) wsum[lane] = z - w;
-            if (lane == 15) pass_total = z;
-        }
-        __syncthreads();
-        int run = carry + wsum[wv] + x - s;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            if (lo + q < nrows) cnt[lo + q] = run;
-            run += v[q];
-        }
-        carry += pass_total;
-        __syncthreads();  // wsum and pass_total are rewritten by the next pass
+    for (int q = 0; q < 8; q++) {
+        v[q] = lo + q < nrows ? cnt[lo + q] : 0;
+        s += v[q];
     }
-    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+__global__ void __launch_bounds__(1024) k_tile_sums(const int *counts, int nrows, int *tsum) {
+    __shared__ int wsum[16];
+    const int *cnt = counts + (int64_t)blockIdx.y * nrows;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int v[8], s;
+    load_rows8(cnt, (int)blockIdx.x * kScanTile + (int)threadIdx.x * 8, nrows, v, s);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) wsum[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < 16; w++) t += wsum[w];
+        tsum[blockIdx.y * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_counts(int *counts, int nrows, const int *tsum, int *totals) {
+    __shared__ int wsum[16];
+    __shared__ int base;
+    int *cnt = counts + (int64_t)blockIdx.y * nrows;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x == 0) {
+        int o = 0;
+        for (int t = 0; t < (int)blockIdx.x; t++) o += tsum[blockIdx.y * gridDim.x + t];
+        base = o;
+    }
+    const int lo = (int)blockIdx.x * kScanTile + (int)threadIdx.x * 8;
+    int v[8], s;
+    load_rows8(cnt, lo, nrows, v, s);
+    int x = s;  // inclusive scan over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    if (wv == 0) {
+        const int w = lane < 16 ? wsum[lane] : 0;
+        int z = w;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const int y = __shfl_up(z, d, 64);
+            if (lane >= d) z += y;
+        }
+        if (lane < 16) wsum[lane] = z - w;
+        if (lane == 15 && blockIdx.x == gridDim.x - 1) totals[blockIdx.y] = base + z;
+    }
+    __syncthreads();
+    int run = base + wsum[wv] + x - s;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (lo + q < nrows) cnt[lo + q] = run;
+        run += v[q];
+    }
 }
 
 // order-preserving scatter into (count,3) lists, (k,j,i) order; one wave
@@ -938,16 +961,19 @@ static int grow_list(int *&p, int64_t &cap, int64_t n) {
 
 int count_tags(Ctx &c, int32_t out[4], bool scatter) {
     const int nrows = (int)(c.ny * c.nz);
+    const int ntile = (nrows + kScanTile - 1) / kScanTile;
     if (nrows > g_cnt_rows) {
         if (g_cnt) IB3D_CHECK(hipFree(g_cnt));
         if (!g_tot) IB3D_CHECK(hipMalloc(&g_tot, 4 * sizeof(int)));
-        IB3D_CHECK(hipMalloc(&g_cnt, 4 * (size_t)nrows * sizeof(int)));
+        // 4 count arrays, then 4 x ntile tile sums
+        IB3D_CHECK(hipMalloc(&g_cnt, (4 * (size_t)nrows + 4 * (size_t)ntile) * sizeof(int)));
         g_cnt_rows = nrows;
     }
-    int *cnt = g_cnt, *tot = g_tot;
+    int *cnt = g_cnt, *tot = g_tot, *tsum = g_cnt + 4 * (size_t)nrows;
     hipLaunchKernelGGL(k_row_counts, dim3(nblocks(nrows, 4)), dim3(256), 0, c.stream, c.tag, (int)c.nx, (int)c.ny,
                        (int)c.nz, cnt);
-    hipLaunchKernelGGL(k_scan_counts, dim3(4), dim3(1024), 0, c.stream, cnt, nrows, tot);
+    hipLaunchKernelGGL(k_tile_sums, dim3(ntile, 4), dim3(1024), 0, c.stream, cnt, nrows, tsum);
+    hipLaunchKernelGGL(k_scan_counts, dim3(ntile, 4), dim3(1024), 0, c.stream, cnt, nrows, tsum, tot);
     IB3D_CHECK(hipGetLastError());
     int h[4];
     IB3D_CHECK(hipMemcpyAsync(h, tot, sizeof h, hipMemcpyDeviceToHost, c.stream));

@@ -181,7 +181,11 @@ k_retag_mark(const int *ib, int nb, int nx, int ny, int nz, uint8_t *flag, int *
             c = lin(i, j, k, nx + 2, ny + 2);
             unsigned int *word = reinterpret_cast<unsigned int *>(flag + (c & ~(int64_t)3));
             const unsigned int bit = 1u << (8 * (c & 3));
-            add = !(atomicOr(word, bit) & bit);
+            // overlapping blocks queue the same cells many times: a relaxed
+            // load that already sees the bit skips the atomic (only the
+            // atomic that sets the bit queues the cell)
+            if (!(__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit))
+                add = !(atomicOr(word, bit) & bit);
         }
     }
     const unsigned long long bal = __ballot(add);

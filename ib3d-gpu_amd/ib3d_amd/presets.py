@@ -1,0 +1,50 @@
+"""Poisson-solver presets shared by the benchmark and the GPU tests.
+
+Every L2 preset stops at ||b/dt - A pc||_2 <= tol ||b/dt||_2, checked in FP64.
+
+  reference  -- the reference's rule (subPcorVcor.f90:148-212): FP64
+                red-black SOR, omega 1.99, stop when max|pc - pco| < eps1,
+                zero guess
+  fp64-cold  -- FP64 red-black SOR at `omega` from a zero guess
+  fp64       -- the same from the previous step's correction
+  mixed-dc   -- FP32 red-black SOR sweeps on the defect equation inside an
+                FP64 defect-correction loop, previous-step guess
+  gmres-sor  -- FP64 refinement, FP32 flexible GMRES(12), every Arnoldi step
+                preconditioned by 20 FP32 red-black SOR sweeps at omega 1.9
+  mixed      -- FP64 refinement, FP32 flexible GMRES(6), every Arnoldi step
+                preconditioned by one FP32 multigrid V-cycle (4+4 red-black
+                SOR sweeps per level at omega 1.55, 4 on the coarsest)
+"""
+
+# FP32 GMRES with SOR preconditioning: Arnoldi steps per refinement step,
+# FP32 SOR sweeps per preconditioner application and their omega
+GMRES_SOR = {"m": 12, "k": 20, "omega": 1.9}
+# the multigrid preconditioner: Arnoldi steps per refinement step, pre- and
+# post-smoothing sweeps per level and their omega, sweeps and omega on the
+# coarsest level
+MULTIGRID = {"m": 6, "nu1": 4, "nu2": 4, "omega": 1.55, "coarse": 4, "coarse_omega": 1.5}
+
+MODES = ("reference", "fp64-cold", "fp64", "mixed-dc", "gmres-sor", "mixed")
+
+
+def apply(solver, mode, omega=1.99, tol=1e-6, inner=20, gmres=None, multigrid=None):
+    """Configure `solver` (an ib3d_amd.native.Solver) for one preset."""
+    g = dict(GMRES_SOR, **(gmres or {}))
+    mg = dict(MULTIGRID, **(multigrid or {}))
+    if mode == "reference":
+        solver.set_poisson_mode(0)
+    elif mode == "fp64-cold":
+        solver.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=False)
+    elif mode == "fp64":
+        solver.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=True)
+    elif mode == "mixed-dc":
+        solver.set_poisson_mode(2, omega=omega, tol=tol, check=inner, warm=True)
+    elif mode == "gmres-sor":
+        solver.set_poisson_mode(3, omega=g["omega"], tol=tol, check=g["k"], warm=True)
+        solver.set_krylov(g["m"])
+    elif mode == "mixed":
+        solver.set_poisson_mode(4, tol=tol, warm=True)
+        solver.set_krylov(mg["m"])
+        solver.set_multigrid(mg["nu1"], mg["nu2"], mg["omega"], mg["coarse"], mg["coarse_omega"], 0)
+    else:
+        raise ValueError(f"unknown Poisson preset {mode!r}; known: {', '.join(MODES)}")

@@ -137,6 +137,13 @@ class Oracle:
         self._ok(self.L.orc_stress_bodies(self.h, _dp(out)))
         return out.reshape(self.nbody, 4)
 
+    def set_wall_motion(self, vel, acc):
+        """Wall velocity and body acceleration used by the forcing, the solid
+        cells and the surface forces (zero: the reference's fixed body)."""
+        v = np.array(vel, dtype=np.float64)
+        a = np.array(acc, dtype=np.float64)
+        self._ok(self.L.orc_set_wall_motion(self.h, _dp(v), _dp(a)))
+
     def move_body(self, disp, vel=(0.0, 0.0, 0.0), acc=(0.0, 0.0, 0.0)):
         """Prescribed rigid motion: nodes at (initial placement + disp), then
         computeSurfaceNorm, selectiveRetagging, cellCount and

@@ -219,3 +219,217 @@ def test_tagging_sphere_invariants(tmp_path):
                     assert cell[i - 1, j - 1, k - 1] == 1
                 if rr.min() > 0.6 * 1.02:
                     assert cell[i - 1, j - 1, k - 1] == 0
+
+
+# ---------------------------------------------------------------------------
+# Moving wall.  The reference's body never moves: it fits the ghost velocity
+# through usurf = 0 (subForcing.f90:144-145) and the ghost pressure through
+# dp/dn = 0, with the moving form -(dv/dt) cosBeta left in a comment
+# (subForcing.f90:20).  The oracle's generalisation (wall velocity U_w in the
+# fit, on solid cells and in the wall shear; dp/dn = -a.n; cells that stop
+# being solid take U_w) is pinned here by fields for which the fit is exact.
+# The plate is tilted so that every component of U_w, a and n enters, with
+# distinct values: a dropped wall term, a wrong sign or a wrong component
+# breaks these checks.
+# ---------------------------------------------------------------------------
+TILT_N = np.array([0.35, -0.45, 1.0]) / np.linalg.norm([0.35, -0.45, 1.0])
+TILT_X0 = np.array([0.05, -0.03, 0.07])
+WALL_U = np.array([0.37, -0.61, 0.23])
+
+
+def tilted_plate_case(d, n=(16, 16, 16), h=0.25, clip=1.25, spacing=0.2):
+    """A plane through TILT_X0 with unit normal TILT_N (fluid on the +n side),
+    triangulated on the part inside the box [-clip, clip]^3 so that every
+    element centroid and surface probe lies inside the grid; the search
+    window keeps the intercepted cells three cells away from the domain ends."""
+    t1 = np.cross(TILT_N, [1.0, 0.0, 0.0])
+    t1 /= np.linalg.norm(t1)
+    t2 = np.cross(TILT_N, t1)
+    m = int(np.ceil(2.0 * np.sqrt(3.0) * clip / spacing))
+    a = spacing * (np.arange(2 * m + 1) - m)
+    pts = TILT_X0[None, None, :] + a[:, None, None] * t1 + a[None, :, None] * t2
+    idx = np.arange(pts.shape[0] * pts.shape[1]).reshape(pts.shape[:2])
+    verts = pts.reshape(-1, 3)
+    tris = []
+    for i in range(2 * m):
+        for j in range(2 * m):
+            for tri in ((idx[i, j], idx[i + 1, j], idx[i + 1, j + 1]), (idx[i, j], idx[i + 1, j + 1], idx[i, j + 1])):
+                if np.all(np.abs(verts[list(tri)]) <= clip):
+                    p1, p2, p3 = verts[list(tri)]
+                    if np.dot(np.cross(p2 - p1, p3 - p1), TILT_N) < 0:
+                        tri = (tri[0], tri[2], tri[1])
+                    tris.append(tri)
+    used = np.unique(np.array(tris))
+    remap = -np.ones(len(verts), dtype=np.int64)
+    remap[used] = np.arange(len(used))
+    verts = verts[used]
+    tris = remap[np.array(tris)]
+    axes = uniform_case(d, n=n, h=h, mesh=(verts, tris),
+                        window=[5, n[0] - 2, 5, n[1] - 2, 5, n[2] - 2])
+    return axes
+
+
+def _dist(o, xname, yname, zname, sl):
+    """Signed distance to the tilted plane of every staggered node of a field
+    (u sits at xu(i+1), v at yv(j+1), w at zw(k+1))."""
+    x = o.array(xname)[sl[0]]
+    y = o.array(yname)[sl[1]]
+    z = o.array(zname)[sl[2]]
+    return ((x[:, None, None] - TILT_X0[0]) * TILT_N[0] + (y[None, :, None] - TILT_X0[1]) * TILT_N[1]
+            + (z[None, None, :] - TILT_X0[2]) * TILT_N[2])
+
+
+def _node_distances(o):
+    nx, ny, nz = o.n
+    full = lambda m: slice(0, m + 2)
+    stag = lambda m: slice(1, m + 3)
+    return {"u": _dist(o, "xu", "yu", "zu", (stag(nx), full(ny), full(nz))),
+            "v": _dist(o, "xv", "yv", "zv", (full(nx), stag(ny), full(nz))),
+            "w": _dist(o, "xw", "yw", "zw", (full(nx), full(ny), stag(nz))),
+            "p": _dist(o, "xp", "yp", "zp", (full(nx), full(ny), full(nz)))}
+
+
+def _set(o, name, val):
+    o.array(name)[:] = np.asarray(val, dtype=np.float64).reshape(-1, order="F")
+
+
+def _tilted_oracle(tmp_path):
+    import pyoracle
+    tilted_plate_case(str(tmp_path))
+    o = pyoracle.Oracle.from_case(str(tmp_path))
+    e = o.counts()["ibElems"]
+    nrm = np.stack([o.array("cosAlpha")[:e], o.array("cosBeta")[:e], o.array("cosGamma")[:e]], axis=1)
+    np.testing.assert_allclose(nrm, np.tile(TILT_N, (e, 1)), atol=1e-12)
+    assert o.counts()["ib"] > 50
+    return o
+
+
+def test_moving_wall_ghost_velocity_is_exact_for_a_linear_field(tmp_path):
+    """u_c = U_w,c + alpha_c d with d the signed distance to the plate: the
+    quadratic fit through the wall value U_w,c, the image value and its normal
+    derivative is exact, so every ghost face keeps U_w,c + alpha_c d.  Fitting
+    through 0 instead of U_w (the fixed body's value) misses by U_w,c times a
+    factor of order one and is rejected below."""
+    o = _tilted_oracle(tmp_path)
+    dist = _node_distances(o)
+    alpha = {"u": 1.7, "v": -0.9, "w": 0.45}
+    want = {c: WALL_U[q] + alpha[c] * dist[c] for q, c in enumerate("uvw")}
+    for c in "uvw":
+        _set(o, c + "t", want[c])
+    o.set_wall_motion(WALL_U, (0.0, 0.0, 0.0))
+    o.call("velocity_forcing")
+    for c in "uvw":
+        np.testing.assert_allclose(o.field(c + "t"), want[c], atol=1e-12, err_msg=c)
+    # the same field with the wall at rest: the ghost faces leave the line
+    for c in "uvw":
+        _set(o, c + "t", want[c])
+    o.set_wall_motion((0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    o.call("velocity_forcing")
+    for c in "uvw":
+        assert np.max(np.abs(o.field(c + "t") - want[c])) > 0.05, c
+
+
+def test_moving_wall_ghost_pressure_uses_minus_a_dot_n(tmp_path):
+    """p = P0 + gamma d and a body acceleration a = -gamma n: the wall
+    condition dp/dn = -a.n = gamma (the sign of the reference's disabled
+    form, dpdn = -(dv/dt) cosBeta, subForcing.f90:20) agrees with the field,
+    so the fit is exact and every intercepted cell keeps P0 + gamma d.  With
+    the opposite sign, or with the Neumann condition of the fixed body, the
+    ghost pressure misses."""
+    o = _tilted_oracle(tmp_path)
+    dist = _node_distances(o)["p"]
+    p0, gamma = 0.31, -0.8
+    want = p0 + gamma * dist
+    ib = o.array("interceptedIndexPtr").reshape(3, -1)
+    sel = tuple(ib[q] - 1 for q in range(3))
+    for acc, exact in ((-gamma * TILT_N, True), (gamma * TILT_N, False), (np.zeros(3), False)):
+        _set(o, "p", want)
+        o.set_wall_motion((0.0, 0.0, 0.0), acc)
+        o.call("pressure_forcing")
+        err = np.max(np.abs(o.field("p")[sel] - want[sel]))
+        if exact:
+            assert err < 1e-12, err
+        else:
+            assert err > 1e-3, (acc, err)
+
+
+def test_moving_wall_solid_cells_and_surface_forces(tmp_path):
+    """Solid cells take U_w (and p = 0).  Surface forces on the plate:
+    - the flow moving with the wall (u = U_w, p = P0) has no wall shear, and
+      the pressure force is -P0 A n, so with the reference's normalisation
+      (subStressCal1.f90:441-450; area_Sx = A|n_y|/2, area_Sy = A|n_x|/2)
+      Cd_p = -4 P0 sign(n_x) and Cl_p = -4 P0 sign(n_y);
+    - a linear shear relative to the wall, u = U_w + alpha d, has the wall
+      shear (alpha - (alpha.n) n)/Re, the tangential part of du/dn, so
+      Cd_v = 4 (alpha_x - (alpha.n) n_x) / (Re |n_y|) and
+      Cl_v = 4 (alpha_y - (alpha.n) n_y) / (Re |n_x|); with
+      p = P0 + gamma d and a = -gamma n the wall pressure is still P0."""
+    o = _tilted_oracle(tmp_path)
+    dist = _node_distances(o)
+    re = o.inp["re"]
+    o.set_wall_motion(WALL_U, (0.0, 0.0, 0.0))
+    o.call("solid_cell_bc")
+    sol = o.array("solidIndexPtr").reshape(3, -1)
+    assert sol.shape[1] > 100
+    sel = tuple(sol[q] - 1 for q in range(3))
+    for q, c in enumerate("uvw"):
+        assert np.all(o.field(c + "t")[sel] == WALL_U[q]), c
+    assert np.all(o.field("p")[sel] == 0.0)
+
+    p0 = 0.27
+    sx, sy = np.sign(TILT_N[0]), np.sign(TILT_N[1])
+    for q, c in enumerate("uvw"):
+        _set(o, c + "t", np.full(dist[c].shape, WALL_U[q]))
+    _set(o, "p", np.full(dist["p"].shape, p0))
+    cd_v, cd_p, cl_v, cl_p = o.stress(True)
+    assert abs(cd_v) < 1e-12 and abs(cl_v) < 1e-12, (cd_v, cl_v)
+    np.testing.assert_allclose([cd_p, cl_p], [-4 * p0 * sx, -4 * p0 * sy], rtol=1e-12)
+
+    alpha = np.array([1.3, -0.7, 0.4])
+    gamma = 0.6
+    for q, c in enumerate("uvw"):
+        _set(o, c + "t", WALL_U[q] + alpha[q] * dist[c])
+    _set(o, "p", p0 + gamma * dist["p"])
+    o.set_wall_motion(WALL_U, -gamma * TILT_N)
+    cd_v, cd_p, cl_v, cl_p = o.stress(False)
+    tang = alpha - np.dot(alpha, TILT_N) * TILT_N
+    np.testing.assert_allclose([cd_v, cl_v], [4 * tang[0] / (re * abs(TILT_N[1])),
+                                              4 * tang[1] / (re * abs(TILT_N[0]))], rtol=1e-10)
+    np.testing.assert_allclose([cd_p, cl_p], [-4 * p0 * sx, -4 * p0 * sy], rtol=1e-10)
+
+
+def test_fresh_cells_take_the_wall_velocity(tmp_path):
+    """Moving the plate by -0.3 along z uncovers solid cells: every cell that
+    was solid and is no longer takes U_w (the oracle's fresh-cell rule; the
+    reference's body never moves), every other cell keeps its velocity, and
+    the re-tagged cells agree with the moved plane's corner test."""
+    o = _tilted_oracle(tmp_path)
+    nx, ny, nz = o.n
+    rng = np.random.default_rng(5)
+    for c in "uvw":
+        _set(o, c, rng.standard_normal((nx + 2) * (ny + 2) * (nz + 2)))
+    before = {c: o.field(c).copy() for c in "uvw"}
+    cell0 = o.field("cell").copy()
+    o.move_body((0.0, 0.0, -0.3), vel=WALL_U)
+    cell1 = o.field("cell")
+    fresh = (cell0 == 1) & (cell1 != 1)
+    assert fresh.sum() > 20
+    # cells that turn solid lie outside the first tagging's search window,
+    # which tagging never looked at and the re-tagged blocks now reach
+    new_solid = np.argwhere((cell0 != 1) & (cell1 == 1))
+    inside = np.all((new_solid >= 4) & (new_solid <= np.array(o.n) - 3), axis=1)
+    assert not np.any(inside)
+    for q, c in enumerate("uvw"):
+        f = o.field(c)
+        assert np.all(f[fresh] == WALL_U[q]), c
+        assert np.array_equal(f[~fresh], before[c][~fresh]), c
+    # corner test of the moved plane on the re-tagged cells: solid iff all 8
+    # corners lie below it
+    x1, y1, z1 = o.array("x1"), o.array("y1"), o.array("z1")
+    x0 = TILT_X0 + np.array([0.0, 0.0, -0.3])
+    for i, j, k in zip(*np.nonzero(fresh)):
+        cx = np.array([x1[i], x1[i + 1]]) - x0[0]
+        cy = np.array([y1[j], y1[j + 1]]) - x0[1]
+        cz = np.array([z1[k], z1[k + 1]]) - x0[2]
+        d = cx[:, None, None] * TILT_N[0] + cy[None, :, None] * TILT_N[1] + cz[None, None, :] * TILT_N[2]
+        assert d.max() > 0

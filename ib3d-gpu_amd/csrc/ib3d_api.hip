@@ -584,9 +584,23 @@ int ib3d_upload(const char *name, const void *host, int64_t nbytes) {
     IB3D_REQUIRE(nbytes == e.count * 8, std::string("ib3d_upload: size mismatch for ") + name);
     IB3D_CHECK(hipStreamSynchronize(c.stream));
     switch (e.kind) {
-        case F64_FIELD:
+        case F64_FIELD: {
             IB3D_CHECK(h2d((double *)e.dev + e.offset, host, nbytes));
+            // surface arrays: the normal distances, the forcing plans and the
+            // stress stencils were found on the old surface; new nodes also
+            // leave the normals stale
+            const double *surf[] = {c.xcent, c.ycent, c.zcent, c.cosA, c.cosB, c.cosG, c.alpha3, c.beta3, c.gamma3};
+            const double *nodes[] = {c.xnode, c.ynode, c.znode, c.xnode1, c.ynode1, c.znode1};
+            bool is_surf = false, is_node = false;
+            for (auto a : surf) is_surf |= (e.dev == a);
+            for (auto a : nodes) is_node |= (e.dev == a);
+            if (is_surf || is_node) {
+                c.normdist_ready = c.plans_ready = c.plans_pending = false;
+                c.stress_ready = false;
+            }
+            if (is_node) c.norms_ready = false;
             break;
+        }
         case PC_FIELD: {
             double *tmp = nullptr;
             IB3D_CHECK(hipMalloc(&tmp, c.P * sizeof(double)));
@@ -614,6 +628,11 @@ int ib3d_upload(const char *name, const void *host, int64_t nbytes) {
             std::memcpy(e.host->data() + 1, host, nbytes);
             IB3D_CHECK(h2d(e.dev, e.host->data(), e.host->size() * sizeof(double)));
             c.mom_ready = false;   // the momentum tables read the spacings
+            // image points, stress probes and the multigrid levels (built
+            // from the grid lines) belong to the old grid
+            c.normdist_ready = c.plans_ready = c.plans_pending = false;
+            c.stress_ready = false;
+            sor_release_tables();
             break;
         }
         case HOST_VEC: {

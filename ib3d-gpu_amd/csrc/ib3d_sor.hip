@@ -2088,6 +2088,12 @@ int sor_solve(Ctx &c, double bnorm2, double eps1) {
     c.last_outer = out;
     c.last_relres = rel;
     c.last_r0 = rel0;
+    // the sweep cap ends the solve like the reference rule's: as an error
+    if (!(rel <= c.tol)) {
+        set_error("poissonSolver: ||b/dt - A pc|| / ||b/dt|| = " + std::to_string(rel) + " above the tolerance " +
+                  std::to_string(c.tol) + " after " + std::to_string(sw) + " sweeps (the sweep cap)");
+        return -1;
+    }
     return 0;
 }
 

@@ -2,6 +2,10 @@
 MODULE ib3d_cabi
   USE, INTRINSIC :: iso_c_binding
   IMPLICIT NONE
+  ! set once the device module holds the grid / the surface mesh (by the
+  ! drop-in driver's readers, or by allocateArrays from the module arrays
+  ! that the reference's own readers filled)
+  LOGICAL, SAVE :: ib3d_grid_on_device = .FALSE., ib3d_mesh_on_device = .FALSE.
   INTERFACE
     INTEGER(c_int) FUNCTION ib3d_init(nx, ny, nz) BIND(C, name='ib3d_init')
       IMPORT :: c_int, c_int64_t

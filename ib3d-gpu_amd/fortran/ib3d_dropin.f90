@@ -94,110 +94,178 @@ SUBROUTINE array_address(name, ptr, nb)
   CASE ('v2NormDis'); ptr = c_loc(v2NormDis); nb = 8_8 * SIZE(v2NormDis, KIND=8)
   CASE ('w1NormDis'); ptr = c_loc(w1NormDis); nb = 8_8 * SIZE(w1NormDis, KIND=8)
   CASE ('w2NormDis'); ptr = c_loc(w2NormDis); nb = 8_8 * SIZE(w2NormDis, KIND=8)
+  CASE ('nelu1'); ptr = c_loc(nelu1); nb = 8_8 * SIZE(nelu1, KIND=8)
+  CASE ('nelu2'); ptr = c_loc(nelu2); nb = 8_8 * SIZE(nelu2, KIND=8)
+  CASE ('nelv1'); ptr = c_loc(nelv1); nb = 8_8 * SIZE(nelv1, KIND=8)
+  CASE ('nelv2'); ptr = c_loc(nelv2); nb = 8_8 * SIZE(nelv2, KIND=8)
+  CASE ('nelw1'); ptr = c_loc(nelw1); nb = 8_8 * SIZE(nelw1, KIND=8)
+  CASE ('nelw2'); ptr = c_loc(nelw2); nb = 8_8 * SIZE(nelw2, KIND=8)
+  CASE ('minElemcell'); ptr = c_loc(minElemcell); nb = 8_8 * SIZE(minElemcell, KIND=8)
+  CASE ('xnode'); ptr = c_loc(xnode); nb = 8_8 * SIZE(xnode, KIND=8)
+  CASE ('ynode'); ptr = c_loc(ynode); nb = 8_8 * SIZE(ynode, KIND=8)
+  CASE ('znode'); ptr = c_loc(znode); nb = 8_8 * SIZE(znode, KIND=8)
+  CASE ('alpha3'); ptr = c_loc(alpha3); nb = 8_8 * SIZE(alpha3, KIND=8)
+  CASE ('beta3'); ptr = c_loc(beta3); nb = 8_8 * SIZE(beta3, KIND=8)
+  CASE ('gamma3'); ptr = c_loc(gamma3); nb = 8_8 * SIZE(gamma3, KIND=8)
+  CASE ('Elemcell'); ptr = c_loc(Elemcell); nb = 8_8 * SIZE(Elemcell, KIND=8)
+  CASE ('ucell'); ptr = c_loc(ucell); nb = 8_8 * SIZE(ucell, KIND=8)
+  CASE ('vcell'); ptr = c_loc(vcell); nb = 8_8 * SIZE(vcell, KIND=8)
+  CASE ('wcell'); ptr = c_loc(wcell); nb = 8_8 * SIZE(wcell, KIND=8)
+  CASE ('pcell'); ptr = c_loc(pcell); nb = 8_8 * SIZE(pcell, KIND=8)
+  CASE ('ca1_uu'); ptr = c_loc(ca1_uu); nb = 8_8 * SIZE(ca1_uu, KIND=8)
+  CASE ('ca2_uu'); ptr = c_loc(ca2_uu); nb = 8_8 * SIZE(ca2_uu, KIND=8)
+  CASE ('ca3_uu'); ptr = c_loc(ca3_uu); nb = 8_8 * SIZE(ca3_uu, KIND=8)
+  CASE ('ca4_uu'); ptr = c_loc(ca4_uu); nb = 8_8 * SIZE(ca4_uu, KIND=8)
+  CASE ('ca5_uu'); ptr = c_loc(ca5_uu); nb = 8_8 * SIZE(ca5_uu, KIND=8)
+  CASE ('ca6_uu'); ptr = c_loc(ca6_uu); nb = 8_8 * SIZE(ca6_uu, KIND=8)
+  CASE ('ck1_uu'); ptr = c_loc(ck1_uu); nb = 8_8 * SIZE(ck1_uu, KIND=8)
+  CASE ('ck2_uu'); ptr = c_loc(ck2_uu); nb = 8_8 * SIZE(ck2_uu, KIND=8)
+  CASE ('ck3_uu'); ptr = c_loc(ck3_uu); nb = 8_8 * SIZE(ck3_uu, KIND=8)
+  CASE ('ck4_uu'); ptr = c_loc(ck4_uu); nb = 8_8 * SIZE(ck4_uu, KIND=8)
+  CASE ('ck5_uu'); ptr = c_loc(ck5_uu); nb = 8_8 * SIZE(ck5_uu, KIND=8)
+  CASE ('ck6_uu'); ptr = c_loc(ck6_uu); nb = 8_8 * SIZE(ck6_uu, KIND=8)
+  CASE ('ca1_vv'); ptr = c_loc(ca1_vv); nb = 8_8 * SIZE(ca1_vv, KIND=8)
+  CASE ('ca2_vv'); ptr = c_loc(ca2_vv); nb = 8_8 * SIZE(ca2_vv, KIND=8)
+  CASE ('ca3_vv'); ptr = c_loc(ca3_vv); nb = 8_8 * SIZE(ca3_vv, KIND=8)
+  CASE ('ca4_vv'); ptr = c_loc(ca4_vv); nb = 8_8 * SIZE(ca4_vv, KIND=8)
+  CASE ('ca5_vv'); ptr = c_loc(ca5_vv); nb = 8_8 * SIZE(ca5_vv, KIND=8)
+  CASE ('ca6_vv'); ptr = c_loc(ca6_vv); nb = 8_8 * SIZE(ca6_vv, KIND=8)
+  CASE ('ck1_vv'); ptr = c_loc(ck1_vv); nb = 8_8 * SIZE(ck1_vv, KIND=8)
+  CASE ('ck2_vv'); ptr = c_loc(ck2_vv); nb = 8_8 * SIZE(ck2_vv, KIND=8)
+  CASE ('ck3_vv'); ptr = c_loc(ck3_vv); nb = 8_8 * SIZE(ck3_vv, KIND=8)
+  CASE ('ck4_vv'); ptr = c_loc(ck4_vv); nb = 8_8 * SIZE(ck4_vv, KIND=8)
+  CASE ('ck5_vv'); ptr = c_loc(ck5_vv); nb = 8_8 * SIZE(ck5_vv, KIND=8)
+  CASE ('ck6_vv'); ptr = c_loc(ck6_vv); nb = 8_8 * SIZE(ck6_vv, KIND=8)
+  CASE ('ca1_ww'); ptr = c_loc(ca1_ww); nb = 8_8 * SIZE(ca1_ww, KIND=8)
+  CASE ('ca2_ww'); ptr = c_loc(ca2_ww); nb = 8_8 * SIZE(ca2_ww, KIND=8)
+  CASE ('ca3_ww'); ptr = c_loc(ca3_ww); nb = 8_8 * SIZE(ca3_ww, KIND=8)
+  CASE ('ca4_ww'); ptr = c_loc(ca4_ww); nb = 8_8 * SIZE(ca4_ww, KIND=8)
+  CASE ('ca5_ww'); ptr = c_loc(ca5_ww); nb = 8_8 * SIZE(ca5_ww, KIND=8)
+  CASE ('ca6_ww'); ptr = c_loc(ca6_ww); nb = 8_8 * SIZE(ca6_ww, KIND=8)
+  CASE ('ck1_ww'); ptr = c_loc(ck1_ww); nb = 8_8 * SIZE(ck1_ww, KIND=8)
+  CASE ('ck2_ww'); ptr = c_loc(ck2_ww); nb = 8_8 * SIZE(ck2_ww, KIND=8)
+  CASE ('ck3_ww'); ptr = c_loc(ck3_ww); nb = 8_8 * SIZE(ck3_ww, KIND=8)
+  CASE ('ck4_ww'); ptr = c_loc(ck4_ww); nb = 8_8 * SIZE(ck4_ww, KIND=8)
+  CASE ('ck5_ww'); ptr = c_loc(ck5_ww); nb = 8_8 * SIZE(ck5_ww, KIND=8)
+  CASE ('ck6_ww'); ptr = c_loc(ck6_ww); nb = 8_8 * SIZE(ck6_ww, KIND=8)
+  CASE ('ca1_uv'); ptr = c_loc(ca1_uv); nb = 8_8 * SIZE(ca1_uv, KIND=8)
+  CASE ('ca2_uv'); ptr = c_loc(ca2_uv); nb = 8_8 * SIZE(ca2_uv, KIND=8)
+  CASE ('ca3_uv'); ptr = c_loc(ca3_uv); nb = 8_8 * SIZE(ca3_uv, KIND=8)
+  CASE ('ca4_uv'); ptr = c_loc(ca4_uv); nb = 8_8 * SIZE(ca4_uv, KIND=8)
+  CASE ('ca5_uv'); ptr = c_loc(ca5_uv); nb = 8_8 * SIZE(ca5_uv, KIND=8)
+  CASE ('ca6_uv'); ptr = c_loc(ca6_uv); nb = 8_8 * SIZE(ca6_uv, KIND=8)
+  CASE ('ck1_uv'); ptr = c_loc(ck1_uv); nb = 8_8 * SIZE(ck1_uv, KIND=8)
+  CASE ('ck2_uv'); ptr = c_loc(ck2_uv); nb = 8_8 * SIZE(ck2_uv, KIND=8)
+  CASE ('ck3_uv'); ptr = c_loc(ck3_uv); nb = 8_8 * SIZE(ck3_uv, KIND=8)
+  CASE ('ck4_uv'); ptr = c_loc(ck4_uv); nb = 8_8 * SIZE(ck4_uv, KIND=8)
+  CASE ('ck5_uv'); ptr = c_loc(ck5_uv); nb = 8_8 * SIZE(ck5_uv, KIND=8)
+  CASE ('ck6_uv'); ptr = c_loc(ck6_uv); nb = 8_8 * SIZE(ck6_uv, KIND=8)
+  CASE ('ca1_uw'); ptr = c_loc(ca1_uw); nb = 8_8 * SIZE(ca1_uw, KIND=8)
+  CASE ('ca2_uw'); ptr = c_loc(ca2_uw); nb = 8_8 * SIZE(ca2_uw, KIND=8)
+  CASE ('ca3_uw'); ptr = c_loc(ca3_uw); nb = 8_8 * SIZE(ca3_uw, KIND=8)
+  CASE ('ca4_uw'); ptr = c_loc(ca4_uw); nb = 8_8 * SIZE(ca4_uw, KIND=8)
+  CASE ('ca5_uw'); ptr = c_loc(ca5_uw); nb = 8_8 * SIZE(ca5_uw, KIND=8)
+  CASE ('ca6_uw'); ptr = c_loc(ca6_uw); nb = 8_8 * SIZE(ca6_uw, KIND=8)
+  CASE ('ck1_uw'); ptr = c_loc(ck1_uw); nb = 8_8 * SIZE(ck1_uw, KIND=8)
+  CASE ('ck2_uw'); ptr = c_loc(ck2_uw); nb = 8_8 * SIZE(ck2_uw, KIND=8)
+  CASE ('ck3_uw'); ptr = c_loc(ck3_uw); nb = 8_8 * SIZE(ck3_uw, KIND=8)
+  CASE ('ck4_uw'); ptr = c_loc(ck4_uw); nb = 8_8 * SIZE(ck4_uw, KIND=8)
+  CASE ('ck5_uw'); ptr = c_loc(ck5_uw); nb = 8_8 * SIZE(ck5_uw, KIND=8)
+  CASE ('ck6_uw'); ptr = c_loc(ck6_uw); nb = 8_8 * SIZE(ck6_uw, KIND=8)
+  CASE ('ca1_vu'); ptr = c_loc(ca1_vu); nb = 8_8 * SIZE(ca1_vu, KIND=8)
+  CASE ('ca2_vu'); ptr = c_loc(ca2_vu); nb = 8_8 * SIZE(ca2_vu, KIND=8)
+  CASE ('ca3_vu'); ptr = c_loc(ca3_vu); nb = 8_8 * SIZE(ca3_vu, KIND=8)
+  CASE ('ca4_vu'); ptr = c_loc(ca4_vu); nb = 8_8 * SIZE(ca4_vu, KIND=8)
+  CASE ('ca5_vu'); ptr = c_loc(ca5_vu); nb = 8_8 * SIZE(ca5_vu, KIND=8)
+  CASE ('ca6_vu'); ptr = c_loc(ca6_vu); nb = 8_8 * SIZE(ca6_vu, KIND=8)
+  CASE ('ck1_vu'); ptr = c_loc(ck1_vu); nb = 8_8 * SIZE(ck1_vu, KIND=8)
+  CASE ('ck2_vu'); ptr = c_loc(ck2_vu); nb = 8_8 * SIZE(ck2_vu, KIND=8)
+  CASE ('ck3_vu'); ptr = c_loc(ck3_vu); nb = 8_8 * SIZE(ck3_vu, KIND=8)
+  CASE ('ck4_vu'); ptr = c_loc(ck4_vu); nb = 8_8 * SIZE(ck4_vu, KIND=8)
+  CASE ('ck5_vu'); ptr = c_loc(ck5_vu); nb = 8_8 * SIZE(ck5_vu, KIND=8)
+  CASE ('ck6_vu'); ptr = c_loc(ck6_vu); nb = 8_8 * SIZE(ck6_vu, KIND=8)
+  CASE ('ca1_vw'); ptr = c_loc(ca1_vw); nb = 8_8 * SIZE(ca1_vw, KIND=8)
+  CASE ('ca2_vw'); ptr = c_loc(ca2_vw); nb = 8_8 * SIZE(ca2_vw, KIND=8)
+  CASE ('ca3_vw'); ptr = c_loc(ca3_vw); nb = 8_8 * SIZE(ca3_vw, KIND=8)
+  CASE ('ca4_vw'); ptr = c_loc(ca4_vw); nb = 8_8 * SIZE(ca4_vw, KIND=8)
+  CASE ('ca5_vw'); ptr = c_loc(ca5_vw); nb = 8_8 * SIZE(ca5_vw, KIND=8)
+  CASE ('ca6_vw'); ptr = c_loc(ca6_vw); nb = 8_8 * SIZE(ca6_vw, KIND=8)
+  CASE ('ck1_vw'); ptr = c_loc(ck1_vw); nb = 8_8 * SIZE(ck1_vw, KIND=8)
+  CASE ('ck2_vw'); ptr = c_loc(ck2_vw); nb = 8_8 * SIZE(ck2_vw, KIND=8)
+  CASE ('ck3_vw'); ptr = c_loc(ck3_vw); nb = 8_8 * SIZE(ck3_vw, KIND=8)
+  CASE ('ck4_vw'); ptr = c_loc(ck4_vw); nb = 8_8 * SIZE(ck4_vw, KIND=8)
+  CASE ('ck5_vw'); ptr = c_loc(ck5_vw); nb = 8_8 * SIZE(ck5_vw, KIND=8)
+  CASE ('ck6_vw'); ptr = c_loc(ck6_vw); nb = 8_8 * SIZE(ck6_vw, KIND=8)
+  CASE ('ca1_wu'); ptr = c_loc(ca1_wu); nb = 8_8 * SIZE(ca1_wu, KIND=8)
+  CASE ('ca2_wu'); ptr = c_loc(ca2_wu); nb = 8_8 * SIZE(ca2_wu, KIND=8)
+  CASE ('ca3_wu'); ptr = c_loc(ca3_wu); nb = 8_8 * SIZE(ca3_wu, KIND=8)
+  CASE ('ca4_wu'); ptr = c_loc(ca4_wu); nb = 8_8 * SIZE(ca4_wu, KIND=8)
+  CASE ('ca5_wu'); ptr = c_loc(ca5_wu); nb = 8_8 * SIZE(ca5_wu, KIND=8)
+  CASE ('ca6_wu'); ptr = c_loc(ca6_wu); nb = 8_8 * SIZE(ca6_wu, KIND=8)
+  CASE ('ck1_wu'); ptr = c_loc(ck1_wu); nb = 8_8 * SIZE(ck1_wu, KIND=8)
+  CASE ('ck2_wu'); ptr = c_loc(ck2_wu); nb = 8_8 * SIZE(ck2_wu, KIND=8)
+  CASE ('ck3_wu'); ptr = c_loc(ck3_wu); nb = 8_8 * SIZE(ck3_wu, KIND=8)
+  CASE ('ck4_wu'); ptr = c_loc(ck4_wu); nb = 8_8 * SIZE(ck4_wu, KIND=8)
+  CASE ('ck5_wu'); ptr = c_loc(ck5_wu); nb = 8_8 * SIZE(ck5_wu, KIND=8)
+  CASE ('ck6_wu'); ptr = c_loc(ck6_wu); nb = 8_8 * SIZE(ck6_wu, KIND=8)
+  CASE ('ca1_wv'); ptr = c_loc(ca1_wv); nb = 8_8 * SIZE(ca1_wv, KIND=8)
+  CASE ('ca2_wv'); ptr = c_loc(ca2_wv); nb = 8_8 * SIZE(ca2_wv, KIND=8)
+  CASE ('ca3_wv'); ptr = c_loc(ca3_wv); nb = 8_8 * SIZE(ca3_wv, KIND=8)
+  CASE ('ca4_wv'); ptr = c_loc(ca4_wv); nb = 8_8 * SIZE(ca4_wv, KIND=8)
+  CASE ('ca5_wv'); ptr = c_loc(ca5_wv); nb = 8_8 * SIZE(ca5_wv, KIND=8)
+  CASE ('ca6_wv'); ptr = c_loc(ca6_wv); nb = 8_8 * SIZE(ca6_wv, KIND=8)
+  CASE ('ck1_wv'); ptr = c_loc(ck1_wv); nb = 8_8 * SIZE(ck1_wv, KIND=8)
+  CASE ('ck2_wv'); ptr = c_loc(ck2_wv); nb = 8_8 * SIZE(ck2_wv, KIND=8)
+  CASE ('ck3_wv'); ptr = c_loc(ck3_wv); nb = 8_8 * SIZE(ck3_wv, KIND=8)
+  CASE ('ck4_wv'); ptr = c_loc(ck4_wv); nb = 8_8 * SIZE(ck4_wv, KIND=8)
+  CASE ('ck5_wv'); ptr = c_loc(ck5_wv); nb = 8_8 * SIZE(ck5_wv, KIND=8)
+  CASE ('ck6_wv'); ptr = c_loc(ck6_wv); nb = 8_8 * SIZE(ck6_wv, KIND=8)
   CASE DEFAULT
     WRITE(*, '(A,A)') 'ib3d: no host copy of array ', TRIM(name)
     ERROR STOP 1
   END SELECT
 END SUBROUTINE array_address
 
-! ---------------------------------------------------------------- input
-! subReadInput.f90:2-133: 27 list-directed values, grid nodes, then the
-! staggered coordinates.  The coordinates are built by the library from the
-! node values (the same expressions) and copied back to the module.
-SUBROUTINE readInput
-  USE global
-  USE ib3d_cabi
-  IMPLICIT NONE
-  REAL(KIND=8), ALLOCATABLE :: gx(:), gy(:), gz(:)
-  INTEGER(KIND=8) :: i
-  INTEGER :: unit
-  OPEN(NEWUNIT=unit, FILE='inputdata', FORM='formatted', STATUS='old')
-  READ(unit, *) nx, ny, nz, re, itamax, eps1, pcItaMax, u0, v0, w0, surGeoPoints, a0, freq, aoa, piv_pt, &
-                xShift, yShift, zShift, istart, dt_order, inor, i_startSearch, i_endSearch, &
-                j_startSearch, j_endSearch, k_startSearch, k_endSearch
-  CLOSE(unit)
-  rev = 1.0d0 / re
-  pi = 4.0d0 * ATAN(1.0d0)
-  deltat = dt_order * 1.0d-3
-  ita = 0
-  ita1 = 0
-  totime = 0.0d0
-  PRINT *, 'dt =', deltat, 'ita = ', ita, 'totime = ', totime
-  PRINT *, 'total cells =', nx*ny*nz
-  ALLOCATE(gx(nx+1), gy(ny+1), gz(nz+1))
-  OPEN(NEWUNIT=unit, FILE='gridx.dat', FORM='formatted', STATUS='old')
-  DO i = 1, nx+1
-    READ(unit, *) gx(i)
-  END DO
-  CLOSE(unit)
-  OPEN(NEWUNIT=unit, FILE='gridy.dat', FORM='formatted', STATUS='old')
-  DO i = 1, ny+1
-    READ(unit, *) gy(i)
-  END DO
-  CLOSE(unit)
-  OPEN(NEWUNIT=unit, FILE='gridz.dat', FORM='formatted', STATUS='old')
-  DO i = 1, nz+1
-    READ(unit, *) gz(i)
-  END DO
-  CLOSE(unit)
-  ALLOCATE(x1(nx+3), y1(ny+3), z1(nz+3), deltax(nx+2), deltay(ny+2), deltaz(nz+2))
-  ALLOCATE(xu(nx+3), yu(ny+2), zu(nz+2), xv(nx+2), yv(ny+3), zv(nz+2), xw(nx+2), yw(ny+2), zw(nz+3), &
-           xp(nx+2), yp(ny+2), zp(nz+2))
-  CALL ib3d_check(ib3d_init(nx, ny, nz), 'ib3d_init')
-  CALL ib3d_check(ib3d_set_grid(gx, gy, gz), 'set_grid')
-  CALL ib3d_update_host('x1'); CALL ib3d_update_host('y1'); CALL ib3d_update_host('z1')
-  CALL ib3d_update_host('deltax'); CALL ib3d_update_host('deltay'); CALL ib3d_update_host('deltaz')
-  CALL ib3d_update_host('xu'); CALL ib3d_update_host('yu'); CALL ib3d_update_host('zu')
-  CALL ib3d_update_host('xv'); CALL ib3d_update_host('yv'); CALL ib3d_update_host('zv')
-  CALL ib3d_update_host('xw'); CALL ib3d_update_host('yw'); CALL ib3d_update_host('zw')
-  CALL ib3d_update_host('xp'); CALL ib3d_update_host('yp'); CALL ib3d_update_host('zp')
-END SUBROUTINE readInput
 
-! subReadInput.f90:137-176: Gmsh 2.2 text with the reference's line accounting
-SUBROUTINE readSurfaceMeshGmsh
-  USE global
-  USE ib3d_cabi
-  IMPLICIT NONE
-  CHARACTER(LEN=128) :: word
-  INTEGER(KIND=8) :: n, i1, i2, i3, i4, i5
-  INTEGER :: unit
-  OPEN(NEWUNIT=unit, FILE='geometries/sphere.msh', FORM='formatted', STATUS='old')
-  DO n = 1, 4
-    READ(unit, *) word
-  END DO
-  READ(unit, *) ibNodes
-  ALLOCATE(xnode(ibNodes), ynode(ibNodes), znode(ibNodes))
-  DO n = 1, ibNodes
-    READ(unit, *) i1, xnode(n), ynode(n), znode(n)
-  END DO
-  DO n = 1, 2
-    READ(unit, *) word
-  END DO
-  READ(unit, *) ibElems
-  DO n = 1, surGeoPoints
-    READ(unit, *) word
-  END DO
-  ibElems = ibElems - surGeoPoints
-  ALLOCATE(ibElP1(ibElems), ibElP2(ibElems), ibElP3(ibElems))
-  DO n = 1, ibElems
-    READ(unit, *) i1, i2, i3, i4, i5, ibElP1(n), ibElP2(n), ibElP3(n)
-  END DO
-  CLOSE(unit)
-  CALL ib3d_check(ib3d_set_mesh(ibNodes, xnode, ynode, znode, ibElems, ibElP1, ibElP2, ibElP3), 'set_mesh')
-  PRINT *, 'SURFACE MESH READING COMPLETE'
-  PRINT *, 'ibNodes =', ibNodes, 'ibElems =', ibElems
-END SUBROUTINE readSurfaceMeshGmsh
-
-! subAllocateArrays.f90:1-15 (the device copy was allocated by ib3d_init)
+! subAllocateArrays.f90:1-15: the module's field arrays, as the reference
+! allocates them.  The device copy of the module is created here from the
+! grid and the surface mesh unless the readers already handed them over (the
+! drop-in driver's readers do; the reference's own readInput and
+! readSurfaceMeshGmsh leave them in the module: x1(2:nx+2) holds the grid
+! nodes, xnode/ibElP1.. the mesh).
 SUBROUTINE allocateArrays
   USE global
+  USE ib3d_cabi
   IMPLICIT NONE
   ALLOCATE(u(nx+2,ny+2,nz+2), ut(nx+2,ny+2,nz+2), v(nx+2,ny+2,nz+2), vt(nx+2,ny+2,nz+2), &
-           w(nx+2,ny+2,nz+2), wt(nx+2,ny+2,nz+2), p(nx+2,ny+2,nz+2), pc(nx+2,ny+2,nz+2), &
+           w(nx+2,ny+2,nz+2), wt(nx+2,ny+2,nz+2), p(nx+2,ny+2,nz+2), &
+           u_sum(nx+2,ny+2,nz+2), v_sum(nx+2,ny+2,nz+2), w_sum(nx+2,ny+2,nz+2), p_sum(nx+2,ny+2,nz+2), &
+           u_avg(nx+2,ny+2,nz+2), v_avg(nx+2,ny+2,nz+2), w_avg(nx+2,ny+2,nz+2), p_avg(nx+2,ny+2,nz+2), &
            resi_u(nx+2,ny+2,nz+2), resi_v(nx+2,ny+2,nz+2), resi_w(nx+2,ny+2,nz+2))
   ALLOCATE(cell(nx+2,ny+2,nz+2))
   ALLOCATE(Acx(nx,3), Acy(ny,3), Acz(nz,3))
+  ALLOCATE(pc(nx+2,ny+2,nz+2), pco(nx+2,ny+2,nz+2))
+  ALLOCATE(SUMWSS(ibNodes), SIGNWSS(ibNodes))
+  IF (.NOT. ib3d_grid_on_device) THEN
+    CALL ib3d_check(ib3d_init(nx, ny, nz), 'ib3d_init')
+    CALL ib3d_check(ib3d_set_grid(x1(2:nx+2), y1(2:ny+2), z1(2:nz+2)), 'set_grid')
+    ib3d_grid_on_device = .TRUE.
+  END IF
+  IF (.NOT. ib3d_mesh_on_device) THEN
+    CALL ib3d_check(ib3d_set_mesh(ibNodes, xnode, ynode, znode, ibElems, ibElP1, ibElP2, ibElP3), 'set_mesh')
+    ib3d_mesh_on_device = .TRUE.
+  END IF
 END SUBROUTINE allocateArrays
+
+! replaces main.f90:55, `!$acc update host (u, v, w, ut, vt, wt, p, cell)`:
+! the same arrays, device -> module.  Call it where host output needs them
+! (the reference's writeOutput every 5000 steps, writeResult every 1000).
+SUBROUTINE ib3d_update_host_fields
+  IMPLICIT NONE
+  CALL ib3d_update_host('u'); CALL ib3d_update_host('v'); CALL ib3d_update_host('w')
+  CALL ib3d_update_host('ut'); CALL ib3d_update_host('vt'); CALL ib3d_update_host('wt')
+  CALL ib3d_update_host('p'); CALL ib3d_update_host('cell')
+END SUBROUTINE ib3d_update_host_fields
 
 ! ---------------------------------------------------------------- geometry
 SUBROUTINE shiftSurfaceNodesInitial          ! subSearch.f90:2-39
@@ -246,10 +314,13 @@ SUBROUTINE computeSurfaceNorm                ! subSearch.f90:80-131
   USE ib3d_cabi
   IMPLICIT NONE
   IF (.NOT. ALLOCATED(xcent)) &
-    ALLOCATE(xcent(ibElems), ycent(ibElems), zcent(ibElems), cosAlpha(ibElems), cosBeta(ibElems), cosGamma(ibElems))
+    ALLOCATE(xcent(ibElems), ycent(ibElems), zcent(ibElems), cosAlpha(ibElems), cosBeta(ibElems), cosGamma(ibElems), &
+             alpha3(ibElems), beta3(ibElems), gamma3(ibElems))
   CALL ib3d_check(ib3d_computeSurfaceNorm(inor), 'computeSurfaceNorm')
+  ! subSearch.f90:128 copies the nine element arrays to the host
   CALL ib3d_update_host('xcent'); CALL ib3d_update_host('ycent'); CALL ib3d_update_host('zcent')
   CALL ib3d_update_host('cosAlpha'); CALL ib3d_update_host('cosBeta'); CALL ib3d_update_host('cosGamma')
+  CALL ib3d_update_host('alpha3'); CALL ib3d_update_host('beta3'); CALL ib3d_update_host('gamma3')
   PRINT *, 'SurfaceNorm done, inor =', inor
 END SUBROUTINE computeSurfaceNorm
 
@@ -318,6 +389,16 @@ SUBROUTINE initialConditions                 ! subInitialConditions.f90:2-48
   USE ib3d_cabi
   IMPLICIT NONE
   CALL ib3d_check(ib3d_initialConditions(), 'initialConditions')
+  ! the reference sets the module's fields on the host and copies them to the
+  ! device (subInitialConditions.f90:8-44); here the device sets them and the
+  ! module receives the same values
+  CALL ib3d_update_host('u'); CALL ib3d_update_host('v'); CALL ib3d_update_host('w')
+  CALL ib3d_update_host('ut'); CALL ib3d_update_host('vt'); CALL ib3d_update_host('wt')
+  CALL ib3d_update_host('p')
+  CALL ib3d_update_host('resi_u'); CALL ib3d_update_host('resi_v'); CALL ib3d_update_host('resi_w')
+  u_sum = 0.0d0; v_sum = 0.0d0; w_sum = 0.0d0; p_sum = 0.0d0
+  u_avg = 0.0d0; v_avg = 0.0d0; w_avg = 0.0d0; p_avg = 0.0d0
+  SUMWSS = 0.0d0; SIGNWSS = 0.0d0
   ita = 0
   ita1 = 0
   totime = 0.0d0
@@ -333,10 +414,139 @@ SUBROUTINE coefficientMatrix                 ! subCoefficientMatrix.f90:1-59
   PRINT *, 'Coefficient Matrix generated'
 END SUBROUTINE coefficientMatrix
 
-SUBROUTINE non_uni_coeff                     ! subNavierStokes.f90:3-619 (tables stay on the device)
+SUBROUTINE non_uni_coeff                     ! subNavierStokes.f90:3-619
+  USE global
   USE ib3d_cabi
   IMPLICIT NONE
   CALL ib3d_check(ib3d_non_uni_coeff(), 'non_uni_coeff')
+  ! the reference builds the 108 tables on the host (subNavierStokes.f90:17-33)
+  IF (.NOT. ALLOCATED(ca1_uu)) ALLOCATE( &
+           ca1_uu(nx+2), ca2_uu(nx+2), ca3_uu(nx+2), ca4_uu(nx+2), ca5_uu(nx+2), ca6_uu(nx+2), &
+           ck1_uu(nx+2), ck2_uu(nx+2), ck3_uu(nx+2), ck4_uu(nx+2), ck5_uu(nx+2), ck6_uu(nx+2), &
+           ca1_vv(ny+2), ca2_vv(ny+2), ca3_vv(ny+2), ca4_vv(ny+2), ca5_vv(ny+2), ca6_vv(ny+2), &
+           ck1_vv(ny+2), ck2_vv(ny+2), ck3_vv(ny+2), ck4_vv(ny+2), ck5_vv(ny+2), ck6_vv(ny+2), &
+           ca1_ww(nz+2), ca2_ww(nz+2), ca3_ww(nz+2), ca4_ww(nz+2), ca5_ww(nz+2), ca6_ww(nz+2), &
+           ck1_ww(nz+2), ck2_ww(nz+2), ck3_ww(nz+2), ck4_ww(nz+2), ck5_ww(nz+2), ck6_ww(nz+2), &
+           ca1_uv(nx+2), ca2_uv(nx+2), ca3_uv(nx+2), ca4_uv(nx+2), ca5_uv(nx+2), ca6_uv(nx+2), &
+           ck1_uv(nx+2), ck2_uv(nx+2), ck3_uv(nx+2), ck4_uv(nx+2), ck5_uv(nx+2), ck6_uv(nx+2), &
+           ca1_uw(nx+2), ca2_uw(nx+2), ca3_uw(nx+2), ca4_uw(nx+2), ca5_uw(nx+2), ca6_uw(nx+2), &
+           ck1_uw(nx+2), ck2_uw(nx+2), ck3_uw(nx+2), ck4_uw(nx+2), ck5_uw(nx+2), ck6_uw(nx+2), &
+           ca1_vu(ny+2), ca2_vu(ny+2), ca3_vu(ny+2), ca4_vu(ny+2), ca5_vu(ny+2), ca6_vu(ny+2), &
+           ck1_vu(ny+2), ck2_vu(ny+2), ck3_vu(ny+2), ck4_vu(ny+2), ck5_vu(ny+2), ck6_vu(ny+2), &
+           ca1_vw(ny+2), ca2_vw(ny+2), ca3_vw(ny+2), ca4_vw(ny+2), ca5_vw(ny+2), ca6_vw(ny+2), &
+           ck1_vw(ny+2), ck2_vw(ny+2), ck3_vw(ny+2), ck4_vw(ny+2), ck5_vw(ny+2), ck6_vw(ny+2), &
+           ca1_wu(nz+2), ca2_wu(nz+2), ca3_wu(nz+2), ca4_wu(nz+2), ca5_wu(nz+2), ca6_wu(nz+2), &
+           ck1_wu(nz+2), ck2_wu(nz+2), ck3_wu(nz+2), ck4_wu(nz+2), ck5_wu(nz+2), ck6_wu(nz+2), &
+           ca1_wv(nz+2), ca2_wv(nz+2), ca3_wv(nz+2), ca4_wv(nz+2), ca5_wv(nz+2), ca6_wv(nz+2), &
+           ck1_wv(nz+2), ck2_wv(nz+2), ck3_wv(nz+2), ck4_wv(nz+2), ck5_wv(nz+2), ck6_wv(nz+2))
+  CALL ib3d_update_host('ca1_uu')
+  CALL ib3d_update_host('ca2_uu')
+  CALL ib3d_update_host('ca3_uu')
+  CALL ib3d_update_host('ca4_uu')
+  CALL ib3d_update_host('ca5_uu')
+  CALL ib3d_update_host('ca6_uu')
+  CALL ib3d_update_host('ck1_uu')
+  CALL ib3d_update_host('ck2_uu')
+  CALL ib3d_update_host('ck3_uu')
+  CALL ib3d_update_host('ck4_uu')
+  CALL ib3d_update_host('ck5_uu')
+  CALL ib3d_update_host('ck6_uu')
+  CALL ib3d_update_host('ca1_vv')
+  CALL ib3d_update_host('ca2_vv')
+  CALL ib3d_update_host('ca3_vv')
+  CALL ib3d_update_host('ca4_vv')
+  CALL ib3d_update_host('ca5_vv')
+  CALL ib3d_update_host('ca6_vv')
+  CALL ib3d_update_host('ck1_vv')
+  CALL ib3d_update_host('ck2_vv')
+  CALL ib3d_update_host('ck3_vv')
+  CALL ib3d_update_host('ck4_vv')
+  CALL ib3d_update_host('ck5_vv')
+  CALL ib3d_update_host('ck6_vv')
+  CALL ib3d_update_host('ca1_ww')
+  CALL ib3d_update_host('ca2_ww')
+  CALL ib3d_update_host('ca3_ww')
+  CALL ib3d_update_host('ca4_ww')
+  CALL ib3d_update_host('ca5_ww')
+  CALL ib3d_update_host('ca6_ww')
+  CALL ib3d_update_host('ck1_ww')
+  CALL ib3d_update_host('ck2_ww')
+  CALL ib3d_update_host('ck3_ww')
+  CALL ib3d_update_host('ck4_ww')
+  CALL ib3d_update_host('ck5_ww')
+  CALL ib3d_update_host('ck6_ww')
+  CALL ib3d_update_host('ca1_uv')
+  CALL ib3d_update_host('ca2_uv')
+  CALL ib3d_update_host('ca3_uv')
+  CALL ib3d_update_host('ca4_uv')
+  CALL ib3d_update_host('ca5_uv')
+  CALL ib3d_update_host('ca6_uv')
+  CALL ib3d_update_host('ck1_uv')
+  CALL ib3d_update_host('ck2_uv')
+  CALL ib3d_update_host('ck3_uv')
+  CALL ib3d_update_host('ck4_uv')
+  CALL ib3d_update_host('ck5_uv')
+  CALL ib3d_update_host('ck6_uv')
+  CALL ib3d_update_host('ca1_uw')
+  CALL ib3d_update_host('ca2_uw')
+  CALL ib3d_update_host('ca3_uw')
+  CALL ib3d_update_host('ca4_uw')
+  CALL ib3d_update_host('ca5_uw')
+  CALL ib3d_update_host('ca6_uw')
+  CALL ib3d_update_host('ck1_uw')
+  CALL ib3d_update_host('ck2_uw')
+  CALL ib3d_update_host('ck3_uw')
+  CALL ib3d_update_host('ck4_uw')
+  CALL ib3d_update_host('ck5_uw')
+  CALL ib3d_update_host('ck6_uw')
+  CALL ib3d_update_host('ca1_vu')
+  CALL ib3d_update_host('ca2_vu')
+  CALL ib3d_update_host('ca3_vu')
+  CALL ib3d_update_host('ca4_vu')
+  CALL ib3d_update_host('ca5_vu')
+  CALL ib3d_update_host('ca6_vu')
+  CALL ib3d_update_host('ck1_vu')
+  CALL ib3d_update_host('ck2_vu')
+  CALL ib3d_update_host('ck3_vu')
+  CALL ib3d_update_host('ck4_vu')
+  CALL ib3d_update_host('ck5_vu')
+  CALL ib3d_update_host('ck6_vu')
+  CALL ib3d_update_host('ca1_vw')
+  CALL ib3d_update_host('ca2_vw')
+  CALL ib3d_update_host('ca3_vw')
+  CALL ib3d_update_host('ca4_vw')
+  CALL ib3d_update_host('ca5_vw')
+  CALL ib3d_update_host('ca6_vw')
+  CALL ib3d_update_host('ck1_vw')
+  CALL ib3d_update_host('ck2_vw')
+  CALL ib3d_update_host('ck3_vw')
+  CALL ib3d_update_host('ck4_vw')
+  CALL ib3d_update_host('ck5_vw')
+  CALL ib3d_update_host('ck6_vw')
+  CALL ib3d_update_host('ca1_wu')
+  CALL ib3d_update_host('ca2_wu')
+  CALL ib3d_update_host('ca3_wu')
+  CALL ib3d_update_host('ca4_wu')
+  CALL ib3d_update_host('ca5_wu')
+  CALL ib3d_update_host('ca6_wu')
+  CALL ib3d_update_host('ck1_wu')
+  CALL ib3d_update_host('ck2_wu')
+  CALL ib3d_update_host('ck3_wu')
+  CALL ib3d_update_host('ck4_wu')
+  CALL ib3d_update_host('ck5_wu')
+  CALL ib3d_update_host('ck6_wu')
+  CALL ib3d_update_host('ca1_wv')
+  CALL ib3d_update_host('ca2_wv')
+  CALL ib3d_update_host('ca3_wv')
+  CALL ib3d_update_host('ca4_wv')
+  CALL ib3d_update_host('ca5_wv')
+  CALL ib3d_update_host('ca6_wv')
+  CALL ib3d_update_host('ck1_wv')
+  CALL ib3d_update_host('ck2_wv')
+  CALL ib3d_update_host('ck3_wv')
+  CALL ib3d_update_host('ck4_wv')
+  CALL ib3d_update_host('ck5_wv')
+  CALL ib3d_update_host('ck6_wv')
 END SUBROUTINE non_uni_coeff
 
 ! ---------------------------------------------------------------- per step
@@ -450,50 +660,6 @@ SUBROUTINE stressCal2
   CALL stress_write(coef)
 END SUBROUTINE stressCal2
 
-! ---------------------------------------------------------------- output
-! subWriteOutput.f90:2-26: field dump every 5000 steps (needs out/)
-SUBROUTINE writeOutput
-  USE global
-  IMPLICIT NONE
-  CHARACTER(LEN=70) :: fname
-  INTEGER(KIND=8) :: i, j, k
-  INTEGER :: unit
-  IF (MOD(ita, 5000_8) /= 0) RETURN
-  CALL ib3d_update_host('u'); CALL ib3d_update_host('v'); CALL ib3d_update_host('w')
-  CALL ib3d_update_host('p'); CALL ib3d_update_host('cell')
-  WRITE(fname, '(A,I9.9,A)') 'out/fielddata.', ita, '.dat'
-  OPEN(NEWUNIT=unit, FILE=fname, STATUS='unknown')
-  WRITE(unit, *) 'variables= "x","y","z","u","v","w","p","totime","cellid"'
-  WRITE(unit, *) 'zone, ', 'i = ', nx, ' j = ', ny, ' k = ', nz
-  DO k = 2, nz+1
-    DO j = 2, ny+1
-      DO i = 2, nx+1
-        WRITE(unit, *) xp(i), yp(j), zp(k), 0.5d0*(u(i,j,k)+u(i-1,j,k)), 0.5d0*(v(i,j,k)+v(i,j-1,k)), &
-                       0.5d0*(w(i,j,k)+w(i,j,k-1)), p(i,j,k), totime, cell(i,j,k)
-      END DO
-    END DO
-  END DO
-  CLOSE(unit)
-END SUBROUTINE writeOutput
-
-! subWriteOutput.f90:58-72: restart file every 1000 steps
-SUBROUTINE writeResult
-  USE global
-  IMPLICIT NONE
-  INTEGER(KIND=8) :: i, j, k
-  INTEGER :: unit
-  IF (MOD(ita, 1000_8) /= 0) RETURN
-  CALL ib3d_update_host('u'); CALL ib3d_update_host('v'); CALL ib3d_update_host('w'); CALL ib3d_update_host('p')
-  OPEN(NEWUNIT=unit, FILE='result', FORM='formatted', STATUS='unknown')
-  DO k = 1, nz+2
-    DO j = 1, ny+2
-      DO i = 1, nx+2
-        WRITE(unit, *) u(i,j,k), v(i,j,k), w(i,j,k), p(i,j,k), totime, ita, ita1
-      END DO
-    END DO
-  END DO
-  CLOSE(unit)
-END SUBROUTINE writeResult
 
 ! subLastConditions.f90:2-42: restart from `result`, ut = u
 SUBROUTINE lastConditions
@@ -513,6 +679,9 @@ SUBROUTINE lastConditions
   ut = u
   vt = v
   wt = w
+  u_sum = 0.0d0; v_sum = 0.0d0; w_sum = 0.0d0; p_sum = 0.0d0
+  u_avg = 0.0d0; v_avg = 0.0d0; w_avg = 0.0d0; p_avg = 0.0d0
+  SUMWSS = 0.0d0; SIGNWSS = 0.0d0
   resi_u = 0.0d0
   resi_v = 0.0d0
   resi_w = 0.0d0
@@ -522,23 +691,3 @@ SUBROUTINE lastConditions
   CALL ib3d_update_device('resi_u'); CALL ib3d_update_device('resi_v'); CALL ib3d_update_device('resi_w')
 END SUBROUTINE lastConditions
 
-! subWriteOutput.f90:76-94: cell tags after tagging (needs tag/)
-SUBROUTINE writeTagging
-  USE global
-  IMPLICIT NONE
-  CHARACTER(LEN=70) :: fname
-  INTEGER(KIND=8) :: i, j, k
-  INTEGER :: unit
-  WRITE(fname, '(A,I2.2,A)') 'tag/tagdata.', ita, '.dat'
-  OPEN(NEWUNIT=unit, FILE=fname, STATUS='unknown')
-  WRITE(unit, *) 'variables= "x","y","z","cellid"'
-  WRITE(unit, *) 'zone, ', 'i = ', nx, ' j = ', ny, ' k = ', nz
-  DO k = 2, nz+1
-    DO j = 2, ny+1
-      DO i = 2, nx+1
-        WRITE(unit, *) xp(i), yp(j), zp(k), cell(i,j,k)
-      END DO
-    END DO
-  END DO
-  CLOSE(unit)
-END SUBROUTINE writeTagging

@@ -61,19 +61,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ib3d-gpu_amd"))
 
 HBM_PEAK_GBS = 8000.0
+from ib3d_amd import presets  # noqa: E402
+
+GMRES = dict(presets.GMRES_SOR)
+MG = presets.MULTIGRID
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-# FP32 GMRES inside the FP64 refinement: Arnoldi steps per refinement step,
-# FP32 SOR sweeps per preconditioner application and their omega
-GMRES = {"m": 12, "k": 20, "omega": 1.9}
-# the multigrid preconditioner: Arnoldi steps per refinement step, pre- and
-# post-smoothing sweeps per level and their omega, sweeps and omega on the
-# coarsest level
-MG = {"m": 6, "nu1": 4, "nu2": 4, "omega": 1.55, "coarse": 4, "coarse_omega": 1.5}
 # The learned guess is kept when its residual is below this fraction of the
 # previous-step guess's.  With the multigrid solver a guess that is only
 # 10-40% better in residual norm was measured to double the Krylov steps of
@@ -111,23 +108,7 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None,
     if net is not None:
         net.config(record=False, guess=mode == "mixed+learned")
         net.set_accept(LEARNED_ACCEPT if mode == "mixed+learned" else 1.0)
-    if mode == "fp64":
-        s.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=True)
-    elif mode == "fp64-cold":
-        s.set_poisson_mode(1, omega=omega, tol=tol, check=inner, warm=False)
-    elif mode == "mixed-dc":
-        s.set_poisson_mode(2, omega=omega, tol=tol, check=inner, warm=True)
-    elif mode == "gmres-sor":
-        s.set_poisson_mode(3, omega=GMRES["omega"], tol=tol, check=GMRES["k"], warm=True)
-        s.set_krylov(GMRES["m"])
-    elif mode in ("mixed", "mixed+learned"):
-        s.set_poisson_mode(4, tol=tol, warm=True)
-        s.set_krylov(MG["m"])
-        s.set_multigrid(MG["nu1"], MG["nu2"], MG["omega"], MG["coarse"], MG["coarse_omega"], 0)
-    elif mode == "reference":
-        s.set_poisson_mode(0)
-    else:
-        raise SystemExit(f"unknown mode {mode}")
+    presets.apply(s, "mixed" if mode == "mixed+learned" else mode, omega=omega, tol=tol, inner=inner, gmres=GMRES)
     for q in range(warmup):
         if motion:
             motion.advance(s)
@@ -147,6 +128,8 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None,
             geo_s += time.perf_counter() - tg
         r = s.step()
         st = s.poisson_stats()
+        if mode != "reference" and not st["relres"] <= tol:
+            raise SystemExit(f"bench.py: {mode} step {q + 1} ended at relres {st['relres']:.3e} > {tol}")
         sweeps.append(st["sweeps"])
         sor_ms.append(st["sor_ms"])
         poi_ms.append(st["poisson_ms"])
@@ -175,6 +158,52 @@ def run_mode(solver_cls, case, mode, steps, warmup, omega, tol, inner, net=None,
         st = net.stats()
         out["learned_guess"] = {"solves": st["solves"], "kept": st["kept"], "guess_ms": round(st["guess_ms"], 3)}
     return s, out
+
+
+def full_run(solver_cls, case, mode, nsteps, tol, omega, inner, net=None, moving=None):
+    """The headline mode over the configuration's whole stated run (steps
+    1..nsteps from the initial condition), timed on its own: whole-run
+    steps/s, the settled second half's Poisson ms and Krylov steps, Cd at the
+    end and the worst final residual of any solve."""
+    s = solver_cls.from_case(case)
+    motion = Motion(moving[0], moving[1], s.deltat) if moving else None
+    if net is not None:
+        net.config(record=False, guess=mode == "mixed+learned")
+        net.set_accept(LEARNED_ACCEPT if mode == "mixed+learned" else 1.0)
+    base = net.stats() if net is not None else None
+    presets.apply(s, "mixed" if mode == "mixed+learned" else mode, omega=omega, tol=tol, inner=inner, gmres=GMRES)
+    poi, sw, worst = [], [], 0.0
+    s.sync()
+    t0 = time.perf_counter()
+    for q in range(nsteps):
+        if motion:
+            motion.advance(s)
+        r = s.step()
+        st = s.poisson_stats()
+        if not st["relres"] <= tol:
+            raise SystemExit(f"bench.py: full run {mode} step {q + 1} ended at relres {st['relres']:.3e} > {tol}")
+        worst = max(worst, st["relres"])
+        poi.append(st["poisson_ms"])
+        sw.append(st["sweeps"])
+        if (q + 1) % 100 == 0:
+            log(f"[bench] full run {mode}: step {q + 1}/{nsteps}, Cd {r['coef'][0] + r['coef'][1]:.5f}")
+    s.sync()
+    t1 = time.perf_counter()
+    half = nsteps // 2
+    out = {"mode": mode, "steps": nsteps, "t_end": round(nsteps * s.deltat, 6),
+           "steps_per_s": round(nsteps / (t1 - t0), 3), "ms_per_step": round(1e3 * (t1 - t0) / nsteps, 3),
+           "poisson_ms_per_step_settled": round(float(np.mean(poi[half:])), 3),
+           "sweeps_per_step_settled": round(float(np.mean(sw[half:])), 2),
+           "settled_steps": f"{half + 1}-{nsteps}",
+           "cd_end": round(float(r["coef"][0] + r["coef"][1]), 6), "max_relres": float(worst)}
+    if net is not None:
+        if mode == "mixed+learned":
+            st = net.stats()
+            out["learned_guess"] = {"solves": st["solves"] - base["solves"], "kept": st["kept"] - base["kept"],
+                                    "guess_ms": round(st["guess_ms"], 3)}
+        net.config(False, False)
+    s.close()
+    return out
 
 
 def train_guess(cfg_name, rec_steps, epochs, batch, lr, omega, tol, inner):
@@ -222,11 +251,35 @@ def kernel_roofline(s, precision, omega, n_cells, sweeps=200):
             "bytes_per_launch": bytes_per_launch}
 
 
-def cpu_baseline(case, gpu_sweeps_per_step, nsweeps):
+def host_cpu():
+    """CPU model and the logical CPUs this process may use (lscpu, else
+    /proc/cpuinfo)."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None and os.path.exists("/proc/cpuinfo"):
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    return {"model": model, "logical_cpus_available": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(case, gpu_sweeps_per_step, nsweeps, cfg1_steps):
+    """The reference's own Fortran (oracle/_ref), one thread: a measured
+    whole-step time on the 128x64x64 sphere, and an extrapolated per-step
+    time on the benchmark's case from a timed sample of its stages."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     if not os.path.exists(exe):
         return None
+    from ib3d_amd import cases
     t0 = time.perf_counter()
     subprocess.run([exe, "sample", str(nsweeps)], cwd=case, check=True, stdout=subprocess.DEVNULL,
                    stderr=subprocess.DEVNULL, timeout=900)
@@ -239,11 +292,36 @@ def cpu_baseline(case, gpu_sweeps_per_step, nsweeps):
     per_step = (vals["nsMomentum"] + 2 * vals["velocityBC"] + vals["solidCellBC"] + vals["velocityForcing1+BC"] +
                 vals["computeDiv"] + gpu_sweeps_per_step * vals["sor_per_sweep"] + vals["correct+BC"] +
                 vals["pressureForcing1"] + vals["stressCal1"])
+    # whole steps of the 128x64x64 sphere, timed step by step
+    c1 = tempfile.mkdtemp(prefix="ib3d_cpu_cfg1_")
+    measured = None
+    try:
+        cases.write_case(cases.CONFIGS["cfg1"], c1)
+        for sub in ("out", "tag"):
+            os.makedirs(os.path.join(c1, sub), exist_ok=True)
+        subprocess.run([exe, "time", str(cfg1_steps)], cwd=c1, check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, timeout=900)
+        steps = []
+        with open(os.path.join(c1, "harness_time.txt")) as fh:
+            for line in fh:
+                w = line.split()
+                if w[0] == "step":
+                    steps.append((int(w[2]), float(w[3])))
+        measured = {"workload": "fixed sphere 128x64x64, 5120 triangles, dt=0.005, reference rule",
+                    "steps_timed": len(steps), "s_per_step": round(sum(t for _, t in steps) / len(steps), 3),
+                    "steps_per_s": round(len(steps) / sum(t for _, t in steps), 5),
+                    "nIterPcor": [n for n, _ in steps]}
+    finally:
+        shutil.rmtree(c1, ignore_errors=True)
+    cpu = host_cpu()
     return {"value": round(1.0 / per_step, 6), "unit": "steps/s", "cores": 1, "kind": "reference",
+            "extrapolated": True, "cpu_model": cpu["model"], "logical_cpus_available": cpu["logical_cpus_available"],
             "sample": (f"reference Fortran (amdflang -O3, 1 thread) on the same 256x192x192 case: setup, one "
                        f"predictor/forcing/projection pass and {nsweeps} SOR iterations "
-                       f"({vals['sor_per_sweep'] * 1e3:.1f} ms each), {wall:.1f} s wall; per-step time at "
-                       f"{gpu_sweeps_per_step:.0f} sweeps/step = {per_step:.2f} s")}
+                       f"({vals['sor_per_sweep'] * 1e3:.1f} ms each), {wall:.1f} s wall; the value is an "
+                       f"extrapolation: the stage times with the SOR part scaled to {gpu_sweeps_per_step:.0f} "
+                       f"sweeps/step = {per_step:.2f} s per step"),
+            "measured_whole_steps": measured}
 
 
 def main():
@@ -260,6 +338,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--inner", type=int, default=20)
     ap.add_argument("--cpu-sweeps", type=int, default=20)
+    ap.add_argument("--cpu-cfg1-steps", type=int, default=2, help="whole reference steps timed on 128x64x64")
+    ap.add_argument("--full-steps", type=int, default=None,
+                    help="steps of the untimed-by-contract whole run (default: the configuration's stated run)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-config", default="cfg1", help="case whose solves are recorded for training")
     ap.add_argument("--rec-steps", type=int, default=1000, help="recorded solves (two 64^3 tiles each)")
@@ -325,10 +406,18 @@ def main():
             s2.close()
             log(f"[bench] {m}: {r2}")
             cmp_all.append(r2)
+        # the configuration's whole stated run for the headline mode (and for
+        # the same solver without the network when the headline uses it)
+        nfull = a.full_steps if a.full_steps is not None else cfg.steps
+        full = []
+        if nfull > 0:
+            for m in [a.mode] + (["mixed"] if a.mode == "mixed+learned" else []):
+                full.append(full_run(Solver, case, m, nfull, a.tol, a.omega, a.inner, net, moving))
+                log(f"[bench] full run: {full[-1]}")
         gpu_wall = time.perf_counter() - t0
         by_mode = {r["mode"]: r for r in cmp_all + [main_res]}
         fp64_sweeps = by_mode.get("fp64-cold", by_mode.get("fp64", main_res))["sweeps_per_step"]
-        cpu = None if a.no_cpu else cpu_baseline(case, fp64_sweeps, a.cpu_sweeps)
+        cpu = None if a.no_cpu else cpu_baseline(case, fp64_sweeps, a.cpu_sweeps, a.cpu_cfg1_steps)
         traffic = None
         tf = os.path.join(ROOT, "profiles", "sor_traffic.json")
         if os.path.exists(tf):
@@ -382,6 +471,8 @@ def main():
             "gpu_wall_s": round(gpu_wall, 1),
             "field_error": errors,
         }
+        if full:
+            line["full_run"] = full
         for k in ("per_body_coef", "geometry_ms_per_step"):
             if k in main_res:
                 line[k] = main_res[k]

@@ -925,14 +925,32 @@ int ib3d_set_multigrid(int nu1, int nu2, double omega, int coarse_sweeps, double
     IB3D_REQUIRE(omega > 0.0 && omega < 2.0 && coarse_omega > 0.0 && coarse_omega < 2.0,
                  "set_multigrid: relaxation factors must lie in (0, 2)");
     IB3D_REQUIRE(max_levels == 0 || max_levels >= 2, "set_multigrid: max_levels must be 0 (no cap) or >= 2");
-    c.mg_nu1 = nu1;
-    c.mg_nu2 = nu2;
+    c.mg_nu1 = c.mg_nu1c = nu1;
+    c.mg_nu2 = c.mg_nu2c = nu2;
     c.mg_omega = omega;
     c.mg_coarse = coarse_sweeps;
     c.mg_coarse_omega = coarse_omega;
     c.mg_levels = max_levels;
     return 0;
 }
+
+int ib3d_set_krylov_target(double target) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(target > 0.0 && target <= 1.0, "set_krylov_target: the target must lie in (0, 1]");
+    c.krylov_target = target;
+    return 0;
+}
+
+int ib3d_set_multigrid_coarse(int nu1, int nu2) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(nu1 >= 1 && nu1 <= 64, "set_multigrid_coarse: pre-smoothing sweeps must be 1..64");
+    IB3D_REQUIRE(nu2 >= 0 && nu2 <= 64, "set_multigrid_coarse: post-smoothing sweeps must be 0..64");
+    c.mg_nu1c = nu1;
+    c.mg_nu2c = nu2;
+    return 0;
+}
+
+int ib3d_set_fused_sweeps(int on, int planes, int blocks) { return set_fused_sweeps(on, planes, blocks); }
 
 int ib3d_set_graphs(int on) {
     ctx().use_graphs = on != 0;
@@ -979,7 +997,8 @@ int ib3d_step(int first, double rev, double re, double deltat, double eps1, doub
 int ib3d_bench_sor(int precision, int sweeps, double omega, double *ms_per_sweep) {
     Ctx &c = ctx();
     IB3D_REQUIRE(c.ready && c.matrix_ready, "bench_sor: setup incomplete");
-    IB3D_REQUIRE(precision == 64 || precision == 32, "bench_sor: precision must be 64 or 32");
+    IB3D_REQUIRE(precision == 64 || precision == 32 || precision == 33,
+                 "bench_sor: precision must be 64, 32 or 33 (FP32, one launch per sweep)");
     IB3D_REQUIRE(sweeps >= 1, "bench_sor: sweeps must be >= 1");
     return bench_sor(c, precision, sweeps, omega, ms_per_sweep);
 }

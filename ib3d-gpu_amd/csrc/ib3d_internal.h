@@ -159,6 +159,7 @@ struct Ctx {
     int krylov_m = 20;             // modes 3, 4: Arnoldi steps per refinement step
     double krylov_target = 0.3;    // modes 3, 4: a cycle stops once its residual estimate is below this x tol x ||b/dt||
     int mg_nu1 = 4, mg_nu2 = 4;    // mode 4: pre- and post-smoothing sweeps per level
+    int mg_nu1c = 4, mg_nu2c = 4;  // mode 4: the same on the levels below the finest
     double mg_omega = 1.55;        // mode 4: smoother relaxation factor
     int mg_coarse = 4;             // mode 4: sweeps on the coarsest level
     double mg_coarse_omega = 1.5;  // mode 4: relaxation factor there
@@ -254,6 +255,7 @@ int ensure_partial(int64_t n);
 int fields_init(Ctx &c);
 int poisson_solve(Ctx &c, double deltat, double eps1, int64_t *niter, double *divmax, double *derr_std);
 int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2);
+int set_fused_sweeps(int on, int tz, int blocks);
 int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);
 int plans_launch(Ctx &c);
 int shift_nodes_moving(Ctx &c, const double base[3], const double disp[3]);

@@ -317,6 +317,144 @@ k_sor_pz(T *__restrict__ mine, const T *__restrict__ other, const T *__restrict_
     if (TRACK) block_max_to(d, slot);
 }
 
+// One whole FP32 red-black sweep in one launch (the finest multigrid level).
+// Red is updated from the old black values, black from the new red values,
+// exactly as two half-sweeps in place, but the result goes to a second
+// buffer: red_new/black_new are written to dst while src keeps the old values,
+// so no workgroup ever reads a value another one is writing.
+//
+// A workgroup owns S consecutive slots of a plane (a colour row is H slots)
+// and a chunk of planes [k0, k1).  It marches in z: at plane kk it updates red
+// over its slots plus H halo slots on either side (the y neighbours of its
+// black cells) and then black at plane kk-1, whose red neighbours at kk-2,
+// kk-1 and kk are now known; red of the planes k0-1 and k1 is recomputed as a
+// halo and not stored.  Red values of the current plane go through a ring of
+// three LDS planes, so one barrier per plane separates their writes from the
+// reads of two planes later.  A thread owns one aligned pair of slots.
+// FIRST: the sweep starts from zero (no old values are read).
+// Every operation matches k_sor_pz, so the result is bit for bit that of the
+// two half-sweeps.
+static constexpr int kSweepThreads = 1024;
+template <bool FIRST>
+__global__ void __launch_bounds__(kSweepThreads)
+k_sweep2_pz(const float *__restrict__ src0, const float *__restrict__ src1, float *__restrict__ dst0,
+            float *__restrict__ dst1, const float *__restrict__ rhs0, const float *__restrict__ rhs1,
+            const float *__restrict__ axc, const float *__restrict__ ayc, const float *__restrict__ azc, SorGeom g,
+            int S, int tz, float omega, float omm, const double *gate, double thr) {
+    typedef f2v V2;
+    typedef f4v V4;
+    __shared__ float ring[3][2 * kSweepThreads];
+    if (gated(gate, thr)) return;
+    int bx, by;
+    xcd_block2(bx, by);
+    const int lo = 2 * (int)threadIdx.x;         // pair offset inside the red range
+    const int R0 = bx * S - g.H;                 // first slot of the red range
+    const int s = R0 + lo;                       // my slot pair (s, s+1)
+    const bool in_plane = s >= 0 && s + 1 < g.plane && lo < S + 2 * g.H;
+    const int jj = in_plane ? (int)fdiv((uint32_t)s, g.dH) : 0;
+    const int h = s - jj * g.H;
+    const bool row_ok = in_plane && jj >= 1 && jj <= g.sy - 2;
+    const bool own = row_ok && s >= bx * S && s < bx * S + S;
+    const int k0 = 1 + by * tz;
+    const int k1 = min(k0 + tz, g.sz - 1);
+    const V2 zero = V2(0.0f);
+    V4 c1 = V4(0.0f), c2 = V4(0.0f), c3 = V4(0.0f);
+    float a1y = 0.0f, a2y = 0.0f, a3y = 0.0f;
+    if (row_ok) {
+        c1 = *reinterpret_cast<const V4 *>(axc + 2 * h);
+        c2 = *reinterpret_cast<const V4 *>(axc + g.sxp + 2 * h);
+        c3 = *reinterpret_cast<const V4 *>(axc + 2 * g.sxp + 2 * h);
+        a1y = ayc[jj]; a2y = ayc[g.sy + jj]; a3y = ayc[2 * g.sy + jj];
+    }
+    // black old at planes kk-1, kk (and kk+1 loaded per plane); red new at
+    // kk-2, kk-1 and kk
+    V2 bo_m = zero, bo_c = zero, rn_m2 = zero, rn_m1 = zero;
+    if (!FIRST && row_ok) {
+        if (k0 - 2 >= 0) bo_m = *reinterpret_cast<const V2 *>(src1 + s + (k0 - 2) * g.plane);
+        bo_c = *reinterpret_cast<const V2 *>(src1 + s + (k0 - 1) * g.plane);
+    }
+    for (int kk = k0 - 1; kk <= k1; kk++) {
+        const int f = s + kk * g.plane;
+        V2 bo_p = zero;
+        if (!FIRST && row_ok && kk + 1 <= g.sz - 1) bo_p = *reinterpret_cast<const V2 *>(src1 + f + g.plane);
+        // ---- red at plane kk
+        V2 rn_c = zero;
+        if (row_ok && kk >= 1 && kk <= g.sz - 2) {
+            const V2 r2 = *reinterpret_cast<const V2 *>(rhs0 + f);
+            const V2 po = FIRST ? zero : *reinterpret_cast<const V2 *>(src0 + f);
+            rn_c = po;
+            const bool ok0 = r2.x == r2.x, ok1 = r2.y == r2.y;
+            if (ok0 || ok1) {
+                const int q = (jj + kk) & 1;                  // colour 0
+                const float side = FIRST ? 0.0f : (q ? src1[f + 2] : src1[f - 1]);
+                const V2 ps = FIRST ? zero : *reinterpret_cast<const V2 *>(src1 + f - g.H);
+                const V2 pn = FIRST ? zero : *reinterpret_cast<const V2 *>(src1 + f + g.H);
+                const float a1z = azc[kk], a2z = azc[g.sz + kk], a3z = azc[2 * g.sz + kk];
+                const float a1x0 = q ? c1.y : c1.x, a2x0 = q ? c2.y : c2.x, a3x0 = q ? c3.y : c3.x;
+                const float a1x1 = q ? c1.w : c1.z, a2x1 = q ? c2.w : c2.z, a3x1 = q ? c3.w : c3.z;
+                if (ok0) {
+                    const float pw = q ? bo_c.x : side, pe = q ? bo_c.y : bo_c.x;
+                    const float gs = sor_div(((((((r2.x - a1x0 * pw) - a3x0 * pe) - a1y * ps.x) - a3y * pn.x) -
+                                               a1z * bo_m.x) - a3z * bo_p.x),
+                                             ((a2x0 + a2y) + a2z));
+                    rn_c.x = omm * po.x + omega * gs;
+                }
+                if (ok1) {
+                    const float pw = q ? bo_c.y : bo_c.x, pe = q ? side : bo_c.y;
+                    const float gs = sor_div(((((((r2.y - a1x1 * pw) - a3x1 * pe) - a1y * ps.y) - a3y * pn.y) -
+                                               a1z * bo_m.y) - a3z * bo_p.y),
+                                             ((a2x1 + a2y) + a2z));
+                    rn_c.y = omm * po.y + omega * gs;
+                }
+            }
+            if (own && kk >= k0 && kk < k1) *reinterpret_cast<V2 *>(dst0 + f) = rn_c;
+        }
+        float *cur = ring[(kk + 3) % 3];
+        cur[lo] = rn_c.x;
+        cur[lo + 1] = rn_c.y;
+        __syncthreads();
+        // ---- black at plane kk-1
+        const int kb = kk - 1;
+        if (own && kb >= k0) {
+            const int fb = f - g.plane;
+            const float *prev = ring[(kk + 2) % 3];
+            const V2 r2 = *reinterpret_cast<const V2 *>(rhs1 + fb);
+            const V2 po = FIRST ? zero : bo_m;
+            V2 out = po;
+            const bool ok0 = r2.x == r2.x, ok1 = r2.y == r2.y;
+            if (ok0 || ok1) {
+                const int q = (1 + jj + kb) & 1;              // colour 1
+                const float side = q ? prev[lo + 2] : prev[lo - 1];
+                const V2 cr = V2{prev[lo], prev[lo + 1]};
+                const V2 ps = V2{prev[lo - g.H], prev[lo - g.H + 1]};
+                const V2 pn = V2{prev[lo + g.H], prev[lo + g.H + 1]};
+                const float a1z = azc[kb], a2z = azc[g.sz + kb], a3z = azc[2 * g.sz + kb];
+                const float a1x0 = q ? c1.y : c1.x, a2x0 = q ? c2.y : c2.x, a3x0 = q ? c3.y : c3.x;
+                const float a1x1 = q ? c1.w : c1.z, a2x1 = q ? c2.w : c2.z, a3x1 = q ? c3.w : c3.z;
+                if (ok0) {
+                    const float pw = q ? cr.x : side, pe = q ? cr.y : cr.x;
+                    const float gs = sor_div(((((((r2.x - a1x0 * pw) - a3x0 * pe) - a1y * ps.x) - a3y * pn.x) -
+                                               a1z * rn_m2.x) - a3z * rn_c.x),
+                                             ((a2x0 + a2y) + a2z));
+                    out.x = omm * po.x + omega * gs;
+                }
+                if (ok1) {
+                    const float pw = q ? cr.y : cr.x, pe = q ? side : cr.y;
+                    const float gs = sor_div(((((((r2.y - a1x1 * pw) - a3x1 * pe) - a1y * ps.y) - a3y * pn.y) -
+                                               a1z * rn_m2.y) - a3z * rn_c.y),
+                                             ((a2x1 + a2y) + a2z));
+                    out.y = omm * po.y + omega * gs;
+                }
+            }
+            *reinterpret_cast<V2 *>(dst1 + fb) = out;
+        }
+        bo_m = bo_c;
+        bo_c = bo_p;
+        rn_m2 = rn_m1;
+        rn_m1 = rn_c;
+    }
+}
+
 // FP64 residual r = b/dt - A p over both colours.  ADD: of p + e (e is the
 // FP32 defect-correction increment not yet added).  Writes r as FP32 (NaN
 // sentinel elsewhere) when r0 != nullptr, and per-block partial sums of r^2.
@@ -1501,6 +1639,7 @@ struct MgLevel {
     float *x = nullptr, *b = nullptr;                    // correction, right-hand side (2*NC each)
     float *r = nullptr;                                  // residual (finest level only)
     MgAxis *tx = nullptr, *ty = nullptr, *tz = nullptr;   // prolongation from the next level (sx, sy, sz entries)
+    float *x2 = nullptr;                                 // finest level: second buffer of the fused sweeps
     bool owns_tables = false, owns_xb = false;
 };
 static std::vector<MgLevel> g_mg;
@@ -1528,6 +1667,7 @@ void mg_release() {
         if (L.owns_xb) { (void)hipFree(L.x); (void)hipFree(L.b); }
         if (L.tx) { (void)hipFree(L.tx); (void)hipFree(L.ty); (void)hipFree(L.tz); }
         if (L.r) (void)hipFree(L.r);
+        if (L.x2) (void)hipFree(L.x2);
     }
     g_mg.clear();
     g_mg_key[0] = g_mg_key[1] = g_mg_key[2] = g_mg_key[3] = 0;
@@ -1702,6 +1842,10 @@ static int mg_setup(Ctx &c) {
         hipLaunchKernelGGL(k_fill_nan_f32, dim3(nblocks(2 * L.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream,
                            L.b, 2 * L.NC);
     }
+    // the fused sweeps' second finest-level buffer: cells they never write
+    // (walls, and whatever lies outside the solve) stay 0, as in x
+    IB3D_CHECK(hipMalloc(&g_mg[0].x2, 2 * g_mg[0].NC * sizeof(float)));
+    IB3D_CHECK(dzero(g_mg[0].x2, 2 * g_mg[0].NC * sizeof(float)));
     IB3D_CHECK(hipGetLastError());
     for (int q = 0; q < 4; q++) g_mg_key[q] = key[q];
     return 0;
@@ -1713,10 +1857,53 @@ static dim3 level_grid(const MgLevel &L) {
 
 // n red-black SOR sweeps on level L (the first from zero when `first`);
 // z-marching cell pairs on the finest level, one cell per thread below
+// Launch shape of the fused sweep: S own slots per workgroup (the red range
+// adds H on either side and must fit 2 x kSweepThreads slots), planes per
+// chunk so that the grid holds about g_sweep_blocks workgroups.
+static bool g_fused = false;
+static int g_sweep_tz = 0;        // planes per chunk (0: from g_sweep_blocks)
+static int g_sweep_blocks = 480;
+
+static bool sweep2_shape(const MgLevel &L, int &S, int &tz, dim3 &grid) {
+    const int64_t plane = L.H * L.sy, smax = 2 * kSweepThreads - 2 * L.H;
+    if (smax < 64) return false;
+    const int64_t nbp = (plane + smax - 1) / smax;
+    S = (int)(((plane + nbp - 1) / nbp + 1) & ~int64_t(1));
+    const int64_t nz = L.sz - 2;
+    int64_t nch = g_sweep_tz > 0 ? (nz + g_sweep_tz - 1) / g_sweep_tz
+                                 : std::min<int64_t>(nz, std::max<int64_t>(1, (g_sweep_blocks + nbp - 1) / nbp));
+    tz = (int)((nz + nch - 1) / nch);
+    nch = (nz + tz - 1) / tz;
+    grid = dim3((unsigned)nbp, (unsigned)nch);
+    return S + 2 * L.H <= 2 * kSweepThreads;
+}
+
 static void mg_smooth(Ctx &c, const MgLevel &L, bool finest, int n, bool first, double omega, const double *gate) {
     const SorGeom g = level_geom(L);
     const float om = (float)omega, omm = (float)(1.0 - omega);
     const dim3 cg(nblocks(L.NC, kSorBlock));
+    int S = 0, tz = 0;
+    dim3 sg;
+    if (finest && g_fused && L.x2 && n >= 2 && sweep2_shape(L, S, tz, sg)) {
+        // an odd count does its first sweep in place, so that the pairs of
+        // fused sweeps end in x
+        if (n % 2) {
+            mg_smooth(c, L, finest, 1, first, omega, gate);
+            n -= 1;
+            first = false;
+        }
+        float *a = L.x, *b = L.x2;
+        for (int q = 0; q < n; q++) {
+            if (first && q == 0)
+                hipLaunchKernelGGL(k_sweep2_pz<true>, sg, dim3(kSweepThreads), 0, c.stream, a, a + L.NC, b,
+                                   b + L.NC, L.b, L.b + L.NC, L.ax, L.ay, L.az, g, S, tz, om, omm, gate, 0.5);
+            else
+                hipLaunchKernelGGL(k_sweep2_pz<false>, sg, dim3(kSweepThreads), 0, c.stream, a, a + L.NC, b,
+                                   b + L.NC, L.b, L.b + L.NC, L.ax, L.ay, L.az, g, S, tz, om, omm, gate, 0.5);
+            std::swap(a, b);
+        }
+        return;
+    }
     for (int s = 0; s < n; s++) {
         for (int col = 0; col < 2; col++) {
             float *mine = L.x + col * L.NC;
@@ -1759,7 +1946,7 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
     const int top = std::min(ns, nl - 1);  // levels [0, top) restrict through launches
     for (int l = 0; l < top; l++) {
         const MgLevel &F = g_mg[l], &C = g_mg[l + 1];
-        mg_smooth(c, F, l == 0, c.mg_nu1, true, c.mg_omega, gate);
+        mg_smooth(c, F, l == 0, l == 0 ? c.mg_nu1 : c.mg_nu1c, true, c.mg_omega, gate);
         if (l == 0) {
             hipLaunchKernelGGL(k_matvec32_pz<true>, level_grid(F), tb, 0, c.stream, F.x, F.x + F.NC, F.b,
                                F.b + F.NC, F.r, F.r + F.NC, F.ax, F.ay, F.az, level_geom(F), gate, 0.5);
@@ -1773,8 +1960,8 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
     if (ns < nl) {
         MgSmall S;
         S.n = nl - ns;
-        S.nu1 = c.mg_nu1;
-        S.nu2 = c.mg_nu2;
+        S.nu1 = c.mg_nu1c;
+        S.nu2 = c.mg_nu2c;
         S.ncoarse = c.mg_coarse;
         S.om = (float)c.mg_omega;
         S.omc = (float)c.mg_coarse_omega;
@@ -1797,7 +1984,7 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
         else
             hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(2 * F.NC, kSorBlock)), tb, 0, c.stream, level_view(F),
                                level_view(C), level_geom(F), gate, 0.5);
-        mg_smooth(c, F, l == 0, c.mg_nu2, false, c.mg_omega, gate);
+        mg_smooth(c, F, l == 0, l == 0 ? c.mg_nu2 : c.mg_nu2c, false, c.mg_omega, gate);
     }
 }
 
@@ -1968,6 +2155,11 @@ static int cycle_graph(Ctx &c, int m, int k, double *relv, double *conv) {
     key.push_back((uint64_t)c.NC);
     key.push_back((uint64_t)c.mg_nu1);
     key.push_back((uint64_t)c.mg_nu2);
+    key.push_back((uint64_t)c.mg_nu1c);
+    key.push_back((uint64_t)c.mg_nu2c);
+    key.push_back((uint64_t)g_fused);
+    key.push_back((uint64_t)g_sweep_tz);
+    key.push_back((uint64_t)g_sweep_blocks);
     key.push_back((uint64_t)c.mg_coarse);
     key.push_back((uint64_t)g_mg_small);
     key.push_back((uint64_t)g_mg_lds);
@@ -2097,6 +2289,14 @@ int sor_solve(Ctx &c, double bnorm2, double eps1) {
     return 0;
 }
 
+int set_fused_sweeps(int on, int tz, int blocks) {
+    IB3D_REQUIRE(tz >= 0 && tz <= 4096 && blocks >= 1 && blocks <= 1 << 20, "set_fused_sweeps: bad launch shape");
+    g_fused = on != 0;
+    g_sweep_tz = tz;
+    g_sweep_blocks = blocks;
+    return 0;
+}
+
 int residual_norm(Ctx &c, double *rnorm, double *bnorm) {
     const unsigned nb = nblocks(2 * c.NC, kSorBlock);
     if (ensure_partial(nb)) return -1;
@@ -2140,7 +2340,7 @@ int bench_sor(Ctx &c, int precision, int sweeps, double omega, double *ms) {
     hipEvent_t a, b;
     IB3D_CHECK(hipEventCreate(&a));
     IB3D_CHECK(hipEventCreate(&b));
-    if (precision == 32) {
+    if (precision == 32 || precision == 33) {
         if (sor_float_tables(c)) return -1;
         const unsigned nb = nblocks(2 * c.NC, kSorBlock);
         if (ensure_partial(nb)) return -1;
@@ -2148,12 +2348,33 @@ int bench_sor(Ctx &c, int precision, int sweeps, double omega, double *ms) {
                            c.e32[1], c.bdt[0], c.bdt[1], c.r32[0], c.r32[1], c.ax, c.ay, c.az, geom(c), c.partial,
                            nullptr, 0.0);
     }
+    // 33: the fused FP32 sweep (one launch per red-black sweep) between e32
+    // and a second buffer, as on the finest multigrid level
+    MgLevel L;
+    float *x2 = nullptr;
+    int S = 0, tz = 0;
+    dim3 sg;
+    if (precision == 33) {
+        if (sor_float_tables(c)) return -1;
+        L.sx = c.sx; L.sy = c.sy; L.sz = c.sz; L.H = c.H; L.NC = c.NC; L.sxp = c.sxp;
+        IB3D_REQUIRE(sweep2_shape(L, S, tz, sg), "bench_sor: the fused sweep does not fit this grid");
+        IB3D_CHECK(hipMalloc(&x2, 2 * c.NC * sizeof(float)));
+        IB3D_CHECK(dzero(x2, 2 * c.NC * sizeof(float)));
+    }
+    const SorGeom g = geom(c);
+    const float om = (float)omega, omm = (float)(1.0 - omega);
+    float *xa = c.e32[0], *xb = x2;
     IB3D_CHECK(hipEventRecord(a, c.stream));
     for (int q = 0; q < sweeps; q++) {
-        if (precision == 32)
+        if (precision == 33) {
+            hipLaunchKernelGGL(k_sweep2_pz<false>, sg, dim3(kSweepThreads), 0, c.stream, xa, xa + c.NC, xb,
+                               xb + c.NC, c.r32[0], c.r32[1], g_axf, g_ayf, g_azf, g, S, tz, om, omm, nullptr, 0.0);
+            std::swap(xa, xb);
+        } else if (precision == 32) {
             sweep32(c, omega, false, false, nullptr, 0.0);
-        else
+        } else {
             sweep64(c, omega, false, nullptr, nullptr, 0.0);
+        }
     }
     IB3D_CHECK(hipEventRecord(b, c.stream));
     IB3D_CHECK(hipEventSynchronize(b));
@@ -2162,6 +2383,12 @@ int bench_sor(Ctx &c, int precision, int sweeps, double omega, double *ms) {
     *ms = (double)t / sweeps;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    if (x2) {
+        if (xa == x2) IB3D_CHECK(hipMemcpyAsync(c.e32[0], x2, 2 * c.NC * sizeof(float), hipMemcpyDeviceToDevice,
+                                                c.stream));
+        IB3D_CHECK(hipStreamSynchronize(c.stream));
+        IB3D_CHECK(hipFree(x2));
+    }
     return 0;
 }
 

@@ -65,6 +65,9 @@ def lib():
         "ib3d_set_poisson_mode": [I, D, D, I, I, I64],
         "ib3d_set_krylov": [I],
         "ib3d_set_multigrid": [I, I, D, I, D, I],
+        "ib3d_set_multigrid_coarse": [I, I],
+        "ib3d_set_krylov_target": [D],
+        "ib3d_set_fused_sweeps": [I, I, I],
         "ib3d_set_graphs": [I],
         "ib3d_poisson_stats": [PI64, PI64, PD, PD, PD, PD],
         "ib3d_residual_norm": [PD, PD],
@@ -274,6 +277,19 @@ class Solver:
     def set_multigrid(self, nu1=4, nu2=4, omega=1.55, coarse_sweeps=4, coarse_omega=1.5, max_levels=0):
         _ok(self.L.ib3d_set_multigrid(int(nu1), int(nu2), float(omega), int(coarse_sweeps), float(coarse_omega),
                                       int(max_levels)))
+
+    def set_fused_sweeps(self, on=True, planes=0, blocks=480):
+        """Finest-level multigrid smoothing with one launch per red-black
+        sweep (on) or two half-sweep launches (off); identical results."""
+        _ok(self.L.ib3d_set_fused_sweeps(int(bool(on)), int(planes), int(blocks)))
+
+    def set_krylov_target(self, target):
+        """Early exit of a Krylov cycle at target x tol x ||b/dt|| (default 0.3)."""
+        _ok(self.L.ib3d_set_krylov_target(float(target)))
+
+    def set_multigrid_coarse(self, nu1, nu2):
+        """Smoothing sweeps on the levels below the finest (after set_multigrid)."""
+        _ok(self.L.ib3d_set_multigrid_coarse(int(nu1), int(nu2)))
 
     def set_graphs(self, on=True):
         """Replay each GMRES-IR iteration as a captured HIP graph or launch its

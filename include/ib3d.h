@@ -162,6 +162,20 @@ int ib3d_set_krylov(int restart);
  * 4, 1.5, 0. */
 int ib3d_set_multigrid(int nu1, int nu2, double omega, int coarse_sweeps, double coarse_omega,
                        int max_levels);
+/* modes 3 and 4: a Krylov cycle stops early once its least-squares residual
+ * estimate is below target x tol x ||b/dt|| (default 0.3); the FP64 residual
+ * of the refinement then decides whether another cycle runs. */
+int ib3d_set_krylov_target(double target);
+/* mode 4: pre- and post-smoothing sweeps on the levels below the finest
+ * (ib3d_set_multigrid sets both the finest and these; call this after it).
+ * Those levels are small enough that a sweep costs about a launch, whatever
+ * the level's size. */
+int ib3d_set_multigrid_coarse(int nu1, int nu2);
+/* mode 4: smooth the finest level with whole red-black sweeps in one launch
+ * each (on != 0) or with two half-sweep launches (on == 0, the default).
+ * Both give identical results.  planes: z planes per workgroup (0: chosen so
+ * that the grid holds about `blocks` workgroups; default 0 and 480). */
+int ib3d_set_fused_sweeps(int on, int planes, int blocks);
 /* modes 3 and 4: replay each refinement iteration (Krylov cycle, FP64
  * residual and its norm) as one captured HIP graph (on != 0) or launch its
  * kernels one by one (on == 0, the default).  Both run the same kernels in
@@ -185,9 +199,10 @@ int ib3d_residual_norm(double *rnorm, double *bnorm);
 int ib3d_step(int first, double rev, double re, double deltat, double eps1, double out[8]);
 
 /* Kernel micro-benchmark used for the roofline figure: `sweeps` RB-SOR
- * sweeps (precision 64 or 32) on the current right-hand side, timed with
- * HIP events on the library stream.  ms_per_sweep = average duration of
- * one red+black sweep. */
+ * sweeps (precision 64 or 32: two half-sweep launches each; 33: FP32, one
+ * launch per sweep, as on the finest multigrid level) on the current
+ * right-hand side, timed with HIP events on the library stream.
+ * ms_per_sweep = average duration of one red+black sweep. */
 int ib3d_bench_sor(int precision, int sweeps, double omega, double *ms_per_sweep);
 /* Diagnostics (no reference counterpart): one streaming device copy of
  * `nbytes` (read nbytes + write nbytes), the calibration point for turning

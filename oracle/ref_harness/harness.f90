@@ -9,6 +9,9 @@
 !   ref_harness run    NSTEPS DUMPDIR [STEP ...]   full steps, stage dumps at STEPs
 !   ref_harness sample NSWEEPS                     time the stages of step 1 and
 !                                                  NSWEEPS single SOR iterations
+!   ref_harness time   NSTEPS                      time NSTEPS whole steps (setup
+!                                                  and each step into
+!                                                  harness_time.txt)
 !   ref_harness move   NSTEPS DUMPDIR DY           rigid body shifted by s*DY in y
 !                                                  at step s, then the reference's
 !                                                  computeSurfaceNorm,
@@ -56,12 +59,12 @@ program ref_harness
   integer(8) :: c0, c1, crate
   real(8) :: t_setup, t_mom, t_bc, t_sol, t_frc, t_div, t_sor, t_corr, t_std, t_prf, t_str
   real(8) :: derr1, derr2, huge_eps
-  integer :: ustep
+  integer :: ustep, utime
   logical :: dumping
 
   nargs = command_argument_count()
   if (nargs < 2) then
-    print *, 'usage: ref_harness run NSTEPS DUMPDIR [STEP ...] | sample NSWEEPS'
+    print *, 'usage: ref_harness run NSTEPS DUMPDIR [STEP ...] | sample NSWEEPS | time NSTEPS'
     stop 2
   end if
   call get_command_argument(1, mode)
@@ -181,8 +184,14 @@ program ref_harness
     stop
   end if
 
+  if (trim(mode) == 'time') then
+    open(newunit=utime, file='harness_time.txt', status='replace')
+    write(utime, '(A,I12)') 'cells ', nx*ny*nz
+    write(utime, '(A,ES25.17)') 'setup ', t_setup
+  end if
   open(newunit=ustep, file=trim(dumpdir)//'/steps.txt', status='replace')
   do s = 1, nsteps
+    call system_clock(c0)
     if (s > 1) then
       ita = ita + 1
       ita1 = ita1 + 1
@@ -211,8 +220,14 @@ program ref_harness
     end if
     write(ustep, '(I8,I12,2ES25.17)') ita, nIterPcor, divmax, totime
     flush(ustep)
+    if (trim(mode) == 'time') then
+      call system_clock(c1)
+      write(utime, '(A,I8,I12,ES25.17)') 'step ', s, nIterPcor, real(c1 - c0, 8) / real(crate, 8)
+      flush(utime)
+    end if
   end do
   close(ustep)
+  if (trim(mode) == 'time') close(utime)
 
 contains
 

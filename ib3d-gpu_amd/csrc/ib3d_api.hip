@@ -426,6 +426,7 @@ int ib3d_init(int64_t nx, int64_t ny, int64_t nz) {
     IB3D_CHECK(hipGetDeviceCount(&ndev));
     IB3D_REQUIRE(ndev > 0, "ib3d_init: no GPU visible");
     IB3D_CHECK(hipGetDevice(&c.device));
+    if (noalloc_setup(c.device)) return -1;
     c.nx = nx; c.ny = ny; c.nz = nz;
     c.sx = nx + 2; c.sy = ny + 2; c.sz = nz + 2;
     c.P = c.sx * c.sy * c.sz;
@@ -832,6 +833,12 @@ int ib3d_computeDiv(double deltat, double *divmax) {
     Ctx &c = ctx();
     IB3D_REQUIRE(c.ready && c.lists_ready, "computeDiv: cellCount has not run");
     return compute_div(c, deltat, divmax, nullptr);
+}
+
+int ib3d_computeDiv_noalloc(double deltat, double *divmax) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(c.ready && c.lists_ready, "computeDiv_noalloc: cellCount has not run");
+    return compute_div_noalloc(c, deltat, divmax);
 }
 
 int ib3d_REDBLACKSOR(double epsi, int64_t *isum, double *derr, double *derr2) {

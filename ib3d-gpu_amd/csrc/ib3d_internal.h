@@ -255,6 +255,8 @@ int ensure_partial(int64_t n);
 int fields_init(Ctx &c);
 int poisson_solve(Ctx &c, double deltat, double eps1, int64_t *niter, double *divmax, double *derr_std);
 int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2);
+int compute_div_noalloc(Ctx &c, double deltat, double *divmax);
+int noalloc_setup(int device);
 int set_fused_sweeps(int on, int tz, int blocks);
 int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);
 int plans_launch(Ctx &c);

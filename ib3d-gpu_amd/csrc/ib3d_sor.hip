@@ -1080,6 +1080,54 @@ void sor_release_tables() {
     g_f_sx = g_f_sy = g_f_sz = 0;
 }
 
+// Allocation-free computeDiv: the same kernels, with the per-block partial
+// maxima and sums of squares and the two reduced values in a static device
+// array instead of memory the library allocates on demand.  Each GPU holds
+// its own copy of `scratch` for the life of the process; its address comes
+// from hipGetSymbolAddress once per device (noalloc_setup, called by
+// ib3d_init), because in host code the array's name is not a device address.
+// Every call on a GPU shares that GPU's array and leaves its contents behind:
+// the kernels overwrite every entry they read, and calls that overlap on one
+// GPU race.
+static constexpr int64_t kScratchDoubles = (int64_t)1 << 19;  // 4 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static constexpr int kMaxDevices = 64;
+static double *g_scratch[kMaxDevices] = {};
+
+int noalloc_setup(int device) {
+    IB3D_REQUIRE(device >= 0 && device < kMaxDevices, "noalloc_setup: device index out of range");
+    if (g_scratch[device]) return 0;
+    void *ptr = nullptr;
+    IB3D_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[device] = static_cast<double *>(ptr);
+    return 0;
+}
+
+int compute_div_noalloc(Ctx &c, double deltat, double *divmax) {
+    const int64_t nb = nblocks(c.P, kSorBlock);
+    // partial maxima [0, nb), partial sums [nb, 2nb), results [2nb, 2nb+2)
+    if (2 * nb + 2 > kScratchDoubles) {
+        set_error("computeDiv_noalloc: this grid needs " + std::to_string(2 * nb + 2) +
+                  " scratch doubles; the static scratch holds " + std::to_string(kScratchDoubles) +
+                  " (at most about 67 million padded cells)");
+        return -1;
+    }
+    IB3D_REQUIRE(c.device >= 0 && c.device < kMaxDevices && g_scratch[c.device],
+                 "computeDiv_noalloc: the scratch address of this device has not been looked up (ib3d_init)");
+    double *sc = g_scratch[c.device];
+    hipLaunchKernelGGL(k_div, dim3((unsigned)nb), dim3(kSorBlock), 0, c.stream, c.ut, c.vt, c.wt, c.tag, c.dx,
+                       c.dy, c.dz, c.bdt[0], c.bdt[1], (int)c.sx, (int)c.sy, (int)c.sz, (int)c.H, deltat, sc,
+                       sc + nb);
+    launch_max_partials(sc, nb, sc + 2 * nb);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, c.stream, sc + nb, nb, sc + 2 * nb + 1);
+    IB3D_CHECK(hipGetLastError());
+    IB3D_CHECK(hipMemcpyAsync(c.hscal, sc + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    IB3D_CHECK(hipStreamSynchronize(c.stream));
+    *divmax = c.hscal[0];
+    c.bnorm2 = c.hscal[1];
+    return 0;
+}
+
 int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2) {
     const unsigned nb = nblocks(c.P, kSorBlock);
     if (ensure_partial(std::max<int64_t>(2 * (int64_t)nb, nblocks(2 * c.NC, kSorBlock)))) return -1;

@@ -56,6 +56,7 @@ def lib():
         "ib3d_velocityForcing1": [],
         "ib3d_pressureForcing1": [],
         "ib3d_computeDiv": [D, PD],
+        "ib3d_computeDiv_noalloc": [D, PD],
         "ib3d_REDBLACKSOR": [D, PI64, PD, PD],
         "ib3d_correctPressure": [],
         "ib3d_correctVelocity": [D],
@@ -253,6 +254,12 @@ class Solver:
     def computeDiv(self):
         dm = C.c_double()
         _ok(self.L.ib3d_computeDiv(self.deltat, C.byref(dm)))
+        return dm.value
+
+    def computeDiv_noalloc(self):
+        """computeDiv with no device allocation (static scratch per GPU)."""
+        dm = C.c_double()
+        _ok(self.L.ib3d_computeDiv_noalloc(self.deltat, C.byref(dm)))
         return dm.value
 
     def REDBLACKSOR(self, epsi):

@@ -118,6 +118,14 @@ int ib3d_velocityForcing1(void);
 int ib3d_pressureForcing1(void);
 /* subPcorVcor.f90:82-101: b = div(ut) on fluid cells, divmax = max|b| */
 int ib3d_computeDiv(double deltat, double *divmax);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ib3d_computeDiv without any allocation.  Same inputs,
+ * outputs and kernels; the per-block partial maxima and sums live in a static
+ * device array of 2^19 doubles (4 MiB) per GPU, held until the process ends,
+ * so grids above about 67 million padded cells are rejected with an error.
+ * Calls that overlap on one GPU share that array and race. */
+int ib3d_computeDiv_noalloc(double deltat, double *divmax);
 /* subPcorVcor.f90:148-212: REDBLACKSOR(epsi, isum, derr, derr2) with the
  * reference rule (omega 1.99, stop when max|pc-pco| < epsi).  Continues from
  * the current pc (poissonSolver zeroes it first). */

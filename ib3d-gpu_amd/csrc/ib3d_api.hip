@@ -432,7 +432,10 @@ int ib3d_init(int64_t nx, int64_t ny, int64_t nz) {
     c.P = c.sx * c.sy * c.sz;
     c.H = (((c.sx + 1) / 2) + 1) & ~int64_t(1);
     c.NC = c.sy * c.sz * c.H;
-    c.sxp = (c.sx + 7) & ~int64_t(7);
+    // x coefficient rows: a thread reads the 4 entries 2h..2h+3 of slot pair h
+    // (h < H), so a row holds at least 2H + 2 entries; a multiple of 8 keeps
+    // those 4-vectors aligned
+    c.sxp = (std::max<int64_t>(c.sx, 2 * c.H + 2) + 7) & ~int64_t(7);
     IB3D_REQUIRE(c.P < (int64_t)1 << 31, "ib3d_init: grid too large for 32-bit cell indices");
     if (check_fastdiv((int)c.H, c.NC + 1) || check_fastdiv((int)c.sy, c.NC / c.H + 1)) {
         set_error("ib3d_init: fast division self-check failed");
@@ -957,7 +960,9 @@ int ib3d_set_multigrid_coarse(int nu1, int nu2) {
     return 0;
 }
 
-int ib3d_set_fused_sweeps(int on, int planes, int blocks) { return set_fused_sweeps(on, planes, blocks); }
+int ib3d_set_fused_sweeps(int levels, int planes, int blocks) { return set_fused_sweeps(levels, planes, blocks); }
+
+int ib3d_set_momentum_tiles(int on) { return set_momentum_tiles(on); }
 
 int ib3d_set_graphs(int on) {
     ctx().use_graphs = on != 0;

@@ -44,6 +44,8 @@ enum MomQ {
     M_NQ = M_UC + 7
 };
 
+typedef __attribute__((address_space(4))) double CDouble;
+
 struct MomTabs {
     const double *m[3];    // quantity-major tables (x axis: lanes read consecutive entries)
     int mn[3];
@@ -79,8 +81,8 @@ __device__ __forceinline__ double dv(double a, double b, double y) {
 // 3rd-order Kawamura-Kuwahara convection term (subNavierStokes.f90:812-831);
 // dA = t5*dA', dB = (2 t11)*dB' come from the axis table with their reciprocals
 // (coefficient c of the stencil at tc[c * ts])
-template <bool FAST>
-__device__ __forceinline__ double kk3(const double *tc, int ts, double vel, double f2, double f1, double f0,
+template <bool FAST, class PT>
+__device__ __forceinline__ double kk3(PT tc, int ts, double vel, double f2, double f1, double f0,
                                       double fm1, double fm2, double dA, double rA, double dB, double rB) {
 #define TC(c) tc[(c) * ts]
     const double sa = (((TC(0) * f2 + TC(1) * f1) + TC(2) * f0) + TC(3) * fm1) + TC(4) * fm2;
@@ -91,32 +93,79 @@ __device__ __forceinline__ double kk3(const double *tc, int ts, double vel, doub
 
 // non-uniform 1st-order upwind term (subNavierStokes.f90:847-866), al = 1;
 // g[0..6*n] = -1/(r(r+1)), 1-1/r, r/(r+1), 2/(r(r+1)), 2/r, 2/(r+1), 1/(2 den)
-template <bool FAST>
-__device__ __forceinline__ double up1(const double *g, int n, double vel, double den, double rden, double fm,
+template <bool FAST, class PT>
+__device__ __forceinline__ double up1(PT g, int n, double vel, double den, double rden, double fm,
                                       double f0, double fp) {
     const double t1 = ((g[0] * fm - g[n] * f0) + g[2 * n] * fp);
     const double t2 = (g[3 * n] * fm - g[4 * n] * f0) + g[5 * n] * fp;
     return dv<FAST>(vel, den, rden) * t1 - (fabs(vel) * g[6 * n]) * t2;
 }
 
+// Where the predictor reads u, v, w at offset (a, b, d) from its cell.
+// GlobalFields: straight from the padded fields in HBM.
+struct GlobalFields {
+    const double *__restrict__ u, *__restrict__ v, *__restrict__ w;
+    int64_t c, sx, sxy;
+    template <int a, int b, int d> __device__ __forceinline__ double U() const { return u[c + a + b * sx + d * sxy]; }
+    template <int a, int b, int d> __device__ __forceinline__ double V() const { return v[c + a + b * sx + d * sxy]; }
+    template <int a, int b, int d> __device__ __forceinline__ double W() const { return w[c + a + b * sx + d * sxy]; }
+    __device__ __forceinline__ void split() const {}
+};
+
+// TiledFields: the cell's own plane from an LDS tile (row stride kTileS),
+// its own column's planes k-2..k+2 from registers, and the four values
+// off both the plane and the column from HBM.
+static constexpr int kMomTX = 64, kMomTZ = 8;
+static constexpr int kTileS = kMomTX + 4;
+struct TiledFields {
+    const double *tu, *tv, *tw;          // LDS, at the cell
+    double qu[5], qv[5], qw[5];          // registers, planes k-2..k+2
+    const double *__restrict__ u, *__restrict__ v, *__restrict__ w;
+    int64_t c, sx, sxy;
+    template <int a, int b, int d> __device__ __forceinline__ double U() const {
+        if constexpr (d == 0) return tu[a + b * kTileS];
+        else if constexpr (a == 0 && b == 0) return qu[d + 2];
+        else return u[c + a + b * sx + d * sxy];
+    }
+    template <int a, int b, int d> __device__ __forceinline__ double V() const {
+        if constexpr (d == 0) return tv[a + b * kTileS];
+        else if constexpr (a == 0 && b == 0) return qv[d + 2];
+        else return v[c + a + b * sx + d * sxy];
+    }
+    template <int a, int b, int d> __device__ __forceinline__ double W() const {
+        if constexpr (d == 0) return tw[a + b * kTileS];
+        else if constexpr (a == 0 && b == 0) return qw[d + 2];
+        else return w[c + a + b * sx + d * sxy];
+    }
+    // the three components are scheduled one after the other: their values
+    // come from LDS, so loading them late costs little, and keeping the
+    // components apart keeps the register count down
+    __device__ __forceinline__ void split() const { __builtin_amdgcn_sched_barrier(0); }
+};
+
 // One fluid cell of the explicit predictor (subNavierStokes.f90:622-1122).
 // The Kawamura-Kuwahara stencil applies away from the walls and from
-// intercepted cells, the upwind stencil elsewhere.
-template <bool FAST>
+// intercepted cells, the upwind stencil elsewhere.  F supplies the velocity
+// values (GlobalFields or TiledFields: the same values, so the same bits).
+template <bool FAST, class F>
 __device__ __forceinline__ void momentum_cell(
-    int i, int j, int k, const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
-    const double *__restrict__ p, const int8_t *__restrict__ cell, double *__restrict__ ut, double *__restrict__ vt,
-    double *__restrict__ wt, double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw,
-    const Tabs &T, const MomTabs &M, int nx, int ny, int nz, double rev, double deltat) {
+    const F &A, int i, int j, int k, const double *__restrict__ p, const int8_t *__restrict__ cell,
+    double *__restrict__ ut, double *__restrict__ vt, double *__restrict__ wt, double *__restrict__ ru,
+    double *__restrict__ rv, double *__restrict__ rw, const Tabs &T, const MomTabs &M, int nx, int ny, int nz,
+    double rev, double deltat) {
     const int64_t sx = nx + 2, sxy = sx * (ny + 2);
     const int64_t c = (i - 1) + sx * ((j - 1) + (int64_t)(ny + 2) * (k - 1));
     if (cell[c] != 0) return;
-#define U(a, b, d) u[c + (a) + (b) * sx + (d) * sxy]
-#define V(a, b, d) v[c + (a) + (b) * sx + (d) * sxy]
-#define W(a, b, d) w[c + (a) + (b) * sx + (d) * sxy]
+#define U(a, b, d) A.template U<a, b, d>()
+#define V(a, b, d) A.template V<a, b, d>()
+#define W(a, b, d) A.template W<a, b, d>()
 #define P(a, b, d) p[c + (a) + (b) * sx + (d) * sxy]
 #define CL(a, b, d) cell[c + (a) + (b) * sx + (d) * sxy]
-    const double *ry = M.rec[1] + (int64_t)j * kMomRec, *rz = M.rec[2] + (int64_t)k * kMomRec;
+    // the y and z records go through the constant address space: the kernel
+    // never writes them, so uniform reads of them stay scalar loads even after
+    // the stores of an earlier cell in the same thread
+    const CDouble *ry = (const CDouble *)(M.rec[1] + (int64_t)j * kMomRec);
+    const CDouble *rz = (const CDouble *)(M.rec[2] + (int64_t)k * kMomRec);
     const double *tuu = T.t[UU] + i, *tuv = T.t[UV] + i;
     const int nuu = T.tn[UU], nuv = T.tn[UV];
 #define MX(q) M.m[0][(q) * M.mn[0] + i]
@@ -169,6 +218,7 @@ __device__ __forceinline__ void momentum_cell(
         ut[c] = U(0, 0, 0) + deltat * (res + dpdx);
         ru[c] = res;
     }
+    A.split();
     // V momentum (subNavierStokes.f90:901-1006)
     {
         const double ay = MY(M_AS);
@@ -200,6 +250,7 @@ __device__ __forceinline__ void momentum_cell(
         vt[c] = V(0, 0, 0) + deltat * (res + dpdy);
         rv[c] = res;
     }
+    A.split();
     // W momentum (subNavierStokes.f90:1010-1116)
     {
         const double az = MZ(M_AS);
@@ -255,7 +306,109 @@ k_momentum(const double *__restrict__ u, const double *__restrict__ v, const dou
     const int j = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * 4 + threadIdx.y)) + 2;
     const int k = (int)blockIdx.z + 2;
     if (j > ny + 1 || i > nx - 1) return;
-    momentum_cell<FAST>(i, j, k, u, v, w, p, cell, ut, vt, wt, ru, rv, rw, T, M, nx, ny, nz, rev, deltat);
+    const int64_t sx = nx + 2, sxy = sx * (ny + 2);
+    const GlobalFields A{u, v, w, (i - 1) + sx * ((j - 1) + (int64_t)(ny + 2) * (k - 1)), sx, sxy};
+    momentum_cell<FAST>(A, i, j, k, p, cell, ut, vt, wt, ru, rv, rw, T, M, nx, ny, nz, rev, deltat);
+}
+
+// The same interior columns, a 64 x 8 tile of (i, j) marching through a chunk
+// of planes.  Every plane's u, v, w over the tile and a two-cell ring (the
+// corners the cross terms reach included) are staged once in LDS, each
+// thread keeps its own column's planes k-2..k+2 in registers, so a value
+// leaves HBM or L2 about once per plane instead of once per stencil point.
+// Two LDS buffers alternate between planes: one barrier per plane.
+template <bool FAST, int kMomTY, int OCC>
+__global__ void __launch_bounds__(kMomTX * kMomTY, OCC)
+k_momentum_lds(const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
+               const double *__restrict__ p, const int8_t *__restrict__ cell, double *__restrict__ ut,
+               double *__restrict__ vt, double *__restrict__ wt, double *__restrict__ ru, double *__restrict__ rv,
+               double *__restrict__ rw, Tabs T, MomTabs M, int nx, int ny, int nz, double rev, double deltat) {
+    constexpr int kTileR = kMomTY + 4;
+    __shared__ double tile[2][3][kTileR * kTileS];
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, tid = ty * kMomTX + tx;
+    const int i0 = 3 + (int)blockIdx.x * kMomTX, j0 = 2 + (int)blockIdx.y * kMomTY;
+    const int i = i0 + tx;
+    const int j = __builtin_amdgcn_readfirstlane(j0 + ty);
+    const int kb = 2 + (int)blockIdx.z * kMomTZ, ke = min(kb + kMomTZ, nz + 2);
+    const int64_t sx = nx + 2, sxy = sx * (ny + 2);
+    const bool mine_in = i <= nx + 2 && j <= ny + 2;   // (i, j) inside the padded grid
+    const bool compute = i <= nx - 1 && j <= ny + 1;
+    const int64_t col = (int64_t)(i - 1) + sx * (j - 1);
+    // own column, planes k-2..k+2 (0 outside the padded range)
+    double qu[5], qv[5], qw[5];
+#pragma unroll
+    for (int d = -2; d <= 2; d++) {
+        const int kk = kb + d;
+        const bool in = mine_in && kk >= 1 && kk <= nz + 2;
+        const int64_t at = col + sxy * (kk - 1);
+        qu[d + 2] = in ? u[at] : 0.0;
+        qv[d + 2] = in ? v[at] : 0.0;
+        qw[d + 2] = in ? w[at] : 0.0;
+    }
+    // ring cells of a tile: rows j0..j0+TY-1 at x offsets -2, -1, 64, 65 (4 TY
+    // cells), rows -2, -1, TY, TY+1 over the 64 columns (256 cells), and the
+    // corners (64, -1) and (-1, TY) that V(1,-1,0) and U(-1,1,0) reach
+    constexpr int nside = 4 * kMomTY, nrow = 4 * kMomTX, nring = nside + nrow + 2;
+#pragma unroll 1
+    for (int k = kb; k < ke; k++) {
+        double *t = &tile[(k - kb) & 1][0][0];
+        const int64_t pk = sxy * (k - 1);
+        const int at = (ty + 2) * kTileS + tx + 2;
+        t[at] = qu[2];
+        t[kTileR * kTileS + at] = qv[2];
+        t[2 * kTileR * kTileS + at] = qw[2];
+#pragma unroll 1
+        for (int r = tid; r < 3 * nring; r += kMomTX * kMomTY) {
+            const int f = r / nring, e = r - f * nring;
+            const double *src = f == 0 ? u : (f == 1 ? v : w);
+            int ox, oy;
+            if (e < nside) {
+                const int s4 = e & 3;
+                ox = s4 < 2 ? s4 - 2 : kMomTX + s4 - 2;
+                oy = e >> 2;
+            } else if (e < nside + nrow) {
+                const int e2 = e - nside, s4 = e2 / kMomTX;
+                ox = e2 - s4 * kMomTX;
+                oy = s4 < 2 ? s4 - 2 : kMomTY + s4 - 2;
+            } else if (e == nside + nrow) {
+                ox = kMomTX;
+                oy = -1;
+            } else {
+                ox = -1;
+                oy = kMomTY;
+            }
+            const int gi = i0 + ox, gj = j0 + oy;
+            const bool in = gi >= 1 && gi <= nx + 2 && gj >= 1 && gj <= ny + 2;
+            t[f * kTileR * kTileS + (oy + 2) * kTileS + ox + 2] =
+                in ? src[(int64_t)(gi - 1) + sx * (gj - 1) + pk] : 0.0;
+        }
+        __syncthreads();
+        if (compute) {
+            // keep the x-axis tables' loads inside the loop: hoisted out of it,
+            // their ~40 per-lane values would occupy registers for all planes
+            MomTabs Ml = M;
+            Tabs Tl = T;
+            asm volatile("" : "+s"(Ml.m[0]));
+            asm volatile("" : "+s"(Tl.t[UU]));
+            asm volatile("" : "+s"(Tl.t[UV]));
+            const TiledFields A{t + at, t + kTileR * kTileS + at, t + 2 * kTileR * kTileS + at,
+                                {qu[0], qu[1], qu[2], qu[3], qu[4]}, {qv[0], qv[1], qv[2], qv[3], qv[4]},
+                                {qw[0], qw[1], qw[2], qw[3], qw[4]}, u, v, w, col + pk, sx, sxy};
+            momentum_cell<FAST>(A, i, j, k, p, cell, ut, vt, wt, ru, rv, rw, Tl, Ml, nx, ny, nz, rev, deltat);
+        }
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            qu[d] = qu[d + 1];
+            qv[d] = qv[d + 1];
+            qw[d] = qw[d + 1];
+        }
+        const int kk = k + 3;
+        const bool in = mine_in && kk <= nz + 2;
+        const int64_t at3 = col + sxy * (kk - 1);
+        qu[4] = in ? u[at3] : 0.0;
+        qv[4] = in ? v[at3] : 0.0;
+        qw[4] = in ? w[at3] : 0.0;
+    }
 }
 
 // The wall columns i = 2, nx, nx+1 of every (j, k) row, kept out of the
@@ -272,7 +425,9 @@ k_momentum_edge(const double *__restrict__ u, const double *__restrict__ v, cons
     if (s == 1 && nx < 3) return;   // i = nx is then column 2 again
     const int i = s == 0 ? 2 : nx + s - 1;
     const int j = r % ny + 2, k = r / ny + 2;
-    momentum_cell<FAST>(i, j, k, u, v, w, p, cell, ut, vt, wt, ru, rv, rw, T, M, nx, ny, nz, rev, deltat);
+    const int64_t sx = nx + 2, sxy = sx * (ny + 2);
+    const GlobalFields A{u, v, w, (i - 1) + sx * ((j - 1) + (int64_t)(ny + 2) * (k - 1)), sx, sxy};
+    momentum_cell<FAST>(A, i, j, k, p, cell, ut, vt, wt, ru, rv, rw, T, M, nx, ny, nz, rev, deltat);
 }
 
 // subBoundaryConditions.f90:141-156: inlet u = 1, reflected v and w; outlet
@@ -489,6 +644,14 @@ static int build_mom_tables(Ctx &c) {
     return 0;
 }
 
+// the interior predictor from LDS tiles (default) or straight from HBM
+static int g_mom_lds = 0;
+int set_momentum_tiles(int on) {
+    IB3D_REQUIRE(on >= 0 && on <= 3, "set_momentum_tiles: variant must be 0..3");
+    g_mom_lds = on;
+    return 0;
+}
+
 int ns_momentum(Ctx &c, double rev, double deltat) {
     IB3D_REQUIRE(c.coeff_ready, "nsMomentum: non_uni_coeff has not run");
     IB3D_REQUIRE(c.lists_ready, "nsMomentum: cellCount has not run");
@@ -506,7 +669,19 @@ int ns_momentum(Ctx &c, double rev, double deltat) {
     const int nedge = 3 * ny * nz;
     auto launch = [&](auto fast) {
         constexpr bool F = decltype(fast)::value;
-        if (nxi > 0)
+        if (nxi > 0 && g_mom_lds == 1)
+            hipLaunchKernelGGL((k_momentum_lds<F, 8, 4>), dim3(nblocks(nxi, kMomTX), nblocks(ny, 8), nblocks(nz, kMomTZ)),
+                               dim3(kMomTX, 8, 1), 0, c.stream, c.u, c.v, c.w, c.p, c.tag, c.ut, c.vt, c.wt,
+                               c.resi_u, c.resi_v, c.resi_w, T, M, nx, ny, nz, rev, deltat);
+        else if (nxi > 0 && g_mom_lds == 2)
+            hipLaunchKernelGGL((k_momentum_lds<F, 4, 3>), dim3(nblocks(nxi, kMomTX), nblocks(ny, 4), nblocks(nz, kMomTZ)),
+                               dim3(kMomTX, 4, 1), 0, c.stream, c.u, c.v, c.w, c.p, c.tag, c.ut, c.vt, c.wt,
+                               c.resi_u, c.resi_v, c.resi_w, T, M, nx, ny, nz, rev, deltat);
+        else if (nxi > 0 && g_mom_lds == 3)
+            hipLaunchKernelGGL((k_momentum_lds<F, 8, 3>), dim3(nblocks(nxi, kMomTX), nblocks(ny, 8), nblocks(nz, kMomTZ)),
+                               dim3(kMomTX, 8, 1), 0, c.stream, c.u, c.v, c.w, c.p, c.tag, c.ut, c.vt, c.wt,
+                               c.resi_u, c.resi_v, c.resi_w, T, M, nx, ny, nz, rev, deltat);
+        else if (nxi > 0)
             hipLaunchKernelGGL(k_momentum<F>, dim3(nblocks(nxi, 64), nblocks(ny, 4), (unsigned)nz), blk, 0,
                                c.stream, c.u, c.v, c.w, c.p, c.tag, c.ut, c.vt, c.wt, c.resi_u, c.resi_v, c.resi_w, T,
                                M, nx, ny, nz, rev, deltat);

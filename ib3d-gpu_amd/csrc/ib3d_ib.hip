@@ -18,7 +18,7 @@
 #include <algorithm>
 #include <cstring>
 #include <cmath>
-#include <thread>
+#include <rocprim/rocprim.hpp>
 #include <unordered_map>
 
 namespace ib3d {
@@ -1051,37 +1051,58 @@ static void free_plan(ForcePlan &p) {
     p = ForcePlan();
 }
 
-// Host staging for the plan builder: pinned, reused across rebuilds (a
-// moving body rebuilds the plans every step).
+// ------------------------------------------------------------------------
+// Forcing plans, built on the device.  The serial loop of velocityForcing1
+// (pressureForcing1) visits the updates of one plan in order; update q reads
+// the 8 corners of its image stencil and writes one ghost face (cell).  A
+// corner that an earlier update of the loop has written must take the value
+// of the latest such update, and the last writer of a face decides its final
+// value (subForcing.f90:137-652).  Here:
+//   1. the write keys (location*4 + component) of the plan are sorted with
+//      their serial positions (a stable radix sort keeps equal keys in serial
+//      order);
+//   2. every corner finds its latest earlier writer by binary search in that
+//      order (none: the field value);
+//   3. the level of an update is one more than the highest level among its
+//      sources, found by relaxing all updates in place until nothing changes
+//      (levels only grow and are bounded by the longest chain);
+//   4. a stable radix sort by level gives the launch order (serial order
+//      within a level, as the reference's loop visits them), and the last
+//      entry of every key segment of step 1 is that face's final writer.
+// Only the level bounds, the number of final writers and the stencil error
+// flag come back to the host, with one small copy.
+// ------------------------------------------------------------------------
 static int *g_doi = nullptr;        // device: stencil records of the 7 updates per IB cell
 static double *g_dod = nullptr;
 static int64_t g_dcap = 0;
-struct PinBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-};
-static PinBuf g_pin;     // pinned staging of the plan uploads
-static PinBuf g_pin_dl;  // pinned copy of the stencil records, read in place by the host passes
 
-static int pinned(PinBuf &b, size_t bytes) {
-    if (bytes <= b.cap) return 0;
-    if (b.p) IB3D_CHECK(hipHostFree(b.p));
-    b.p = nullptr;
-    b.cap = 0;
-    IB3D_CHECK(hipHostMalloc(&b.p, bytes, hipHostMallocDefault));
-    b.cap = bytes;
-    return 0;
+// Work arrays of the device builder (shared by both plans, grow only).
+struct PlanBuild {
+    int *wk = nullptr, *wq = nullptr;        // write keys and serial positions
+    int *sk = nullptr, *sq = nullptr;        // the same, sorted by key
+    int *src = nullptr;                      // [8][m] sources (serial positions, -1: field)
+    int *lev = nullptr, *slev = nullptr;     // levels, sorted levels
+    int *ord = nullptr;                      // serial positions in launch order
+    int *pos_of = nullptr;                   // launch position of every serial position
+    int *fflag = nullptr, *fidx = nullptr;   // final-writer flags and their exclusive scan
+    int *info = nullptr;                     // [0] error, [1] n_final, [2] n_levels, [3..] level starts,
+                                             // then per relaxation sweep a "changed" flag
+    int *hinfo = nullptr;                    // pinned mirror of info
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    int64_t cap = 0;
+};
+static PlanBuild g_pb;
+static constexpr int kMaxLevels = 256, kRelaxBatch = 16, kInfoInts = 4 + kMaxLevels + 1 + 1024;
+
+static void free_build() {
+    PlanBuild &B = g_pb;
+    for (int *q : {B.wk, B.wq, B.sk, B.sq, B.src, B.lev, B.slev, B.ord, B.pos_of, B.fflag, B.fidx, B.info})
+        if (q) (void)hipFree(q);
+    if (B.hinfo) (void)hipHostFree(B.hinfo);
+    if (B.tmp) (void)hipFree(B.tmp);
+    B = PlanBuild();
 }
-
-// Host work arrays of one plan, kept across rebuilds.  src and level are
-// indexed by update id and written for every id of the plan before they
-// are read, so they are never cleared.
-struct PlanWork {
-    std::vector<int> src, level, pos_of, last, order;
-    std::vector<std::pair<int, int>> finals;
-    std::vector<int> fcomp;
-};
-static PlanWork g_vwork, g_pwork;
 
 void release_plans(Ctx &c) {
     free_plan(c.vplan);
@@ -1091,26 +1112,157 @@ void release_plans(Ctx &c) {
     g_doi = nullptr;
     g_dod = nullptr;
     g_dcap = 0;
-    for (PinBuf *b : {&g_pin, &g_pin_dl}) {
-        if (b->p) (void)hipHostFree(b->p);
-        b->p = nullptr;
-        b->cap = 0;
-    }
-    g_vwork = PlanWork();
-    g_pwork = PlanWork();
+    free_build();
 }
 
-// Upload a plan: the updates in level order (8 stencil ints, 8 sources, 5
-// doubles each) and the final-writer table [loc | upd | comp] (3*n_final
-// ints).  Device buffers grow only; all arrays go through one pinned staging
-// area and are copied with a single wait.
-static int upload_plan(Ctx &c, ForcePlan &P, PlanWork &W, const int *oi, const double *od) {
-    const std::vector<int> &order = W.order, &src = W.src, &level = W.level, &fcomp = W.fcomp;
-    const std::vector<std::pair<int, int>> &finals = W.finals;
-    const int n = (int)order.size();
-    std::vector<int> &pos_of = W.pos_of;  // read only at ids of this plan
-    for (int q = 0; q < n; q++) pos_of[order[q]] = q;
-    const int m = std::max(1, n), nf = std::max(1, (int)finals.size());
+// serial position q of a plan -> update id (velocity: per cell u2 u1 v2 v1
+// w2 w1, ids 7n..7n+5; pressure: id 7n+6)
+__device__ __forceinline__ int plan_uid(int q, int vel) { return vel ? 7 * (q / 6) + q % 6 : 7 * q + 6; }
+
+__global__ void k_plan_keys(const int *__restrict__ oi, int m, int vel, int *wk, int *wq) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const int *o = oi + 8 * (int64_t)plan_uid(q, vel);
+    wk[q] = o[1] * 4 + o[0];
+    wq[q] = q;
+}
+
+// latest writer before q of every stencil corner, by binary search in the
+// key-sorted writes (positions ascending within a key)
+__global__ void k_plan_sources(const int *__restrict__ oi, int m, int vel, int sx, int sy, const int *__restrict__ sk,
+                               const int *__restrict__ sq, int *src, int *lev) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const int *o = oi + 8 * (int64_t)plan_uid(q, vel);
+    const int comp = o[0];
+    const int ax = o[2] - (comp == 0) - 1, ay = o[3] - (comp == 1) - 1, az = o[4] - (comp == 2) - 1;
+    for (int t = 0; t < 8; t++) {
+        const int loc = (ax + (t & 1)) + sx * ((ay + ((t >> 1) & 1)) + sy * (az + (t >> 2)));
+        const int key = loc * 4 + comp;
+        int lo = 0, hi = m;                  // first index with sk >= key
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sk[mid] < key) lo = mid + 1; else hi = mid;
+        }
+        int l2 = lo, h2 = m;                 // first index in the segment with sq >= q
+        while (l2 < h2) {
+            const int mid = (l2 + h2) >> 1;
+            if (sk[mid] == key && sq[mid] < q) l2 = mid + 1; else h2 = mid;
+        }
+        src[t * (int64_t)m + q] = (l2 > lo) ? sq[l2 - 1] : -1;
+    }
+    lev[q] = 0;
+}
+
+// one in-place relaxation sweep; changed[0] = 1 if any level grew.  Skipped
+// once the previous sweep changed nothing.
+__global__ void k_plan_relax(int m, const int *__restrict__ src, int *lev, int *changed, int first) {
+    if (!first && changed[-1] == 0) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    int l = 0;
+    for (int t = 0; t < 8; t++) {
+        const int s = src[t * (int64_t)m + q];
+        if (s >= 0) l = max(l, __hip_atomic_load(&lev[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1);
+    }
+    if (l > lev[q]) {
+        __hip_atomic_store(&lev[q], l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        changed[0] = 1;
+    }
+}
+
+__global__ void k_plan_fill(int *a, int n, int v) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) a[q] = v;
+}
+
+// level bounds from the level-sorted array: info[3 + l] = first position of
+// level l, info[2] = number of levels (levels are 0..max without gaps: a
+// level-l update has a source at level l-1)
+__global__ void k_plan_bounds(int m, const int *__restrict__ slev, const int *__restrict__ ord, int *pos_of,
+                              int *info) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    pos_of[ord[p]] = p;
+    const int l = slev[p];
+    if (l >= kMaxLevels) {
+        info[0] = 2;
+        return;
+    }
+    if (p == 0 || slev[p - 1] != l) info[3 + l] = p;
+    if (p == m - 1) {
+        info[2] = l + 1;
+        info[3 + l + 1] = m;
+    }
+}
+
+// the plan in launch order: stencil record, image point, sources as launch
+// positions; final writers: the last write of every key segment
+__global__ void k_plan_emit(const int *__restrict__ oi, const double *__restrict__ od, int m, int vel,
+                            const int *__restrict__ ord, const int *__restrict__ pos_of, const int *__restrict__ src,
+                            int *upd, int *srcs, double *pos) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    const int q = ord[p];
+    const int u = plan_uid(q, vel);
+    for (int t = 0; t < 8; t++) {
+        upd[8 * (int64_t)p + t] = oi[8 * (int64_t)u + t];
+        const int sv = src[t * (int64_t)m + q];
+        srcs[8 * (int64_t)p + t] = sv >= 0 ? pos_of[sv] : -1;
+    }
+    for (int t = 0; t < 5; t++) pos[5 * (int64_t)p + t] = od[5 * (int64_t)u + t];
+}
+
+__global__ void k_plan_final_flags(int m, const int *__restrict__ sk, int *fflag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    fflag[i] = (i == m - 1 || sk[i + 1] != sk[i]) ? 1 : 0;
+}
+
+__global__ void k_plan_finals(int m, const int *__restrict__ sk, const int *__restrict__ sq,
+                              const int *__restrict__ fflag, const int *__restrict__ fidx,
+                              const int *__restrict__ pos_of, int *fin, int nf_cap, int *info) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    if (i == m - 1) info[1] = fidx[i] + fflag[i];
+    if (!fflag[i]) return;
+    const int f = fidx[i];
+    fin[f] = sk[i] >> 2;
+    fin[nf_cap + f] = pos_of[sq[i]];
+    fin[2 * nf_cap + f] = sk[i] & 3;
+}
+
+static int build_grow(Ctx &c, int64_t m) {
+    PlanBuild &B = g_pb;
+    if (!B.info) {
+        IB3D_CHECK(hipMalloc(&B.info, kInfoInts * sizeof(int)));
+        IB3D_CHECK(hipHostMalloc((void **)&B.hinfo, kInfoInts * sizeof(int), hipHostMallocDefault));
+    }
+    if (m <= B.cap) return 0;
+    IB3D_CHECK(hipStreamSynchronize(c.stream));
+    for (int **q : {&B.wk, &B.wq, &B.sk, &B.sq, &B.src, &B.lev, &B.slev, &B.ord, &B.pos_of, &B.fflag, &B.fidx}) {
+        if (*q) IB3D_CHECK(hipFree(*q));
+        *q = nullptr;
+    }
+    const int64_t cap = m + m / 4;
+    for (int **q : {&B.wk, &B.wq, &B.sk, &B.sq, &B.lev, &B.slev, &B.ord, &B.pos_of, &B.fflag, &B.fidx})
+        IB3D_CHECK(hipMalloc(q, (size_t)cap * sizeof(int)));
+    IB3D_CHECK(hipMalloc(&B.src, 8 * (size_t)cap * sizeof(int)));
+    // radix sort and scan scratch for the largest size
+    size_t b1 = 0, b2 = 0;
+    IB3D_CHECK(rocprim::radix_sort_pairs(nullptr, b1, B.wk, B.sk, B.wq, B.sq, (unsigned)cap, 0, 32, c.stream));
+    IB3D_CHECK(rocprim::exclusive_scan(nullptr, b2, B.fflag, B.fidx, 0, (size_t)cap, rocprim::plus<int>(), c.stream));
+    const size_t need = std::max(b1, b2);
+    if (need > B.tmp_bytes) {
+        if (B.tmp) IB3D_CHECK(hipFree(B.tmp));
+        IB3D_CHECK(hipMalloc(&B.tmp, need));
+        B.tmp_bytes = need;
+    }
+    B.cap = cap;
+    return 0;
+}
+
+static int plan_capacity(ForcePlan &P, int m, int nf_cap) {
     if (m > P.cap_upd) {
         if (P.upd) { (void)hipFree(P.upd); (void)hipFree(P.srcs); (void)hipFree(P.pos); (void)hipFree(P.res); }
         const int cap = m + m / 4;
@@ -1120,178 +1272,72 @@ static int upload_plan(Ctx &c, ForcePlan &P, PlanWork &W, const int *oi, const d
         IB3D_CHECK(hipMalloc(&P.res, (size_t)cap * sizeof(double)));
         P.cap_upd = cap;
     }
-    if (3 * nf > P.cap_final) {
+    if (3 * nf_cap > P.cap_final) {
         if (P.final_loc) (void)hipFree(P.final_loc);
-        const int cap = 3 * nf + 3 * nf / 4;
-        IB3D_CHECK(hipMalloc(&P.final_loc, (size_t)cap * sizeof(int)));
-        P.cap_final = cap;
+        IB3D_CHECK(hipMalloc(&P.final_loc, 3 * (size_t)nf_cap * sizeof(int)));
+        P.cap_final = 3 * nf_cap;
     }
-    const size_t b_oi = 8 * (size_t)m * sizeof(int), b_od = 5 * (size_t)m * sizeof(double),
-                 b_fin = 3 * (size_t)nf * sizeof(int);
-    if (pinned(g_pin, b_od + 2 * b_oi + b_fin)) return -1;
-    double *sod = static_cast<double *>(g_pin.p);
-    int *soi = reinterpret_cast<int *>(sod + 5 * (size_t)m), *ssrc = soi + 8 * (size_t)m, *fin = ssrc + 8 * (size_t)m;
-    int maxl = -1;
-    for (int q = 0; q < n; q++) {
-        const int u = order[q];
-        for (int t = 0; t < 8; t++) {
-            soi[8 * q + t] = oi[8 * (size_t)u + t];
-            const int s = src[8 * (size_t)u + t];
-            ssrc[8 * q + t] = s >= 0 ? pos_of[s] : -1;
-        }
-        for (int t = 0; t < 5; t++) sod[5 * q + t] = od[5 * (size_t)u + t];
-        maxl = std::max(maxl, level[u]);
-    }
-    P.n_updates = n;
-    P.n_levels = maxl + 1;
-    P.level_start.assign(P.n_levels + 1, 0);
-    for (int q = 0; q < n; q++) P.level_start[level[order[q]] + 1]++;
-    for (int l = 0; l < P.n_levels; l++) P.level_start[l + 1] += P.level_start[l];
-    P.n_final = (int)finals.size();
-    for (int q = 0; q < P.n_final; q++) {
-        fin[q] = finals[q].first;
-        fin[P.n_final + q] = pos_of[finals[q].second];
-        fin[2 * P.n_final + q] = fcomp[q];
-    }
-    IB3D_CHECK(hipMemcpyAsync(P.upd, soi, b_oi, hipMemcpyHostToDevice, c.stream));
-    IB3D_CHECK(hipMemcpyAsync(P.srcs, ssrc, b_oi, hipMemcpyHostToDevice, c.stream));
-    IB3D_CHECK(hipMemcpyAsync(P.pos, sod, b_od, hipMemcpyHostToDevice, c.stream));
-    IB3D_CHECK(hipMemcpyAsync(P.final_loc, fin, b_fin, hipMemcpyHostToDevice, c.stream));
-    IB3D_CHECK(hipStreamSynchronize(c.stream));  // the staging area is reused by the next upload
     return 0;
 }
 
-// Dependency levels of the serial loop over `ids` (in serial order).  The
-// latest writer of each (location, component) lives in a dense table over
-// the touched box or, for a large box, an open-addressing table (linear
-// probing, keys loc*4+comp, -1 = empty).
-static void levels(const std::vector<int> &ids, const int *oi, int sx, int sy, PlanWork &W) {
-    std::vector<int> &src = W.src, &level = W.level, &fcomp = W.fcomp;
-    std::vector<std::pair<int, int>> &finals = W.finals;
-    // The last writer of every (location, component) the updates touch.  A
-    // body's updates stay within a small box of the grid: there a dense
-    // table over the box replaces the hash (same lookups, no probing), and
-    // scanning it in index order gives the final writers already sorted.
-    // (0-based grid coordinates; location = x + sx (y + sy z).)
-    int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-    for (int u : ids) {
-        const int *o = &oi[8 * (size_t)u];
-        const int comp = o[0];
-        const int ax = o[2] - (comp == 0) - 1, ay = o[3] - (comp == 1) - 1, az = o[4] - (comp == 2) - 1;
-        const int64_t w = o[1];
-        const int wx = (int)(w % sx), wy = (int)((w / sx) % sy), wz = (int)(w / ((int64_t)sx * sy));
-        const int cl[3] = {std::min(ax, wx), std::min(ay, wy), std::min(az, wz)};
-        const int ch[3] = {std::max(ax + 1, wx), std::max(ay + 1, wy), std::max(az + 1, wz)};
-        for (int d = 0; d < 3; d++) {
-            lo[d] = std::min(lo[d], cl[d]);
-            hi[d] = std::max(hi[d], ch[d]);
-        }
+// Build one plan on the device; its level bounds and final-writer count come
+// back with the copy of `info` (at *host_info once the stream reaches it).
+static int plan_device(Ctx &c, ForcePlan &P, int vel, int nb) {
+    PlanBuild &B = g_pb;
+    const int m = vel ? 6 * nb : nb;
+    // every update writes one face, so there are at most m final writers;
+    // the final table is sized m and its three columns are m apart
+    if (build_grow(c, m) || plan_capacity(P, m, m)) return -1;
+    const dim3 tb(256), gb(nblocks(m, 256));
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) <= 4 * c.P) bits++;
+    hipLaunchKernelGGL(k_plan_keys, gb, tb, 0, c.stream, g_doi, m, vel, B.wk, B.wq);
+    size_t tb_bytes = B.tmp_bytes;
+    IB3D_CHECK(rocprim::radix_sort_pairs(B.tmp, tb_bytes, B.wk, B.sk, B.wq, B.sq, (unsigned)m, 0, bits, c.stream));
+    hipLaunchKernelGGL(k_plan_sources, gb, tb, 0, c.stream, g_doi, m, vel, (int)c.sx, (int)c.sy, B.sk, B.sq, B.src,
+                       B.lev);
+    // relaxation sweeps in batches, each gated on the one before
+    int *chg = B.info + 4 + kMaxLevels + 1;
+    int done = 0;
+    for (;;) {
+        IB3D_REQUIRE(done + kRelaxBatch <= 1024, "forcing: the plan's dependency chain is too long");
+        hipLaunchKernelGGL(k_plan_fill, dim3(1), dim3(kRelaxBatch), 0, c.stream, chg + done, kRelaxBatch, 0);
+        for (int r = 0; r < kRelaxBatch; r++)
+            hipLaunchKernelGGL(k_plan_relax, gb, tb, 0, c.stream, m, B.src, B.lev, chg + done + r,
+                               (done + r == 0) ? 1 : 0);
+        done += kRelaxBatch;
+        IB3D_CHECK(hipMemcpyAsync(&B.hinfo[0], chg + done - 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        IB3D_CHECK(hipStreamSynchronize(c.stream));
+        if (!B.hinfo[0]) break;
     }
-    const int64_t bx = ids.empty() ? 0 : hi[0] - lo[0] + 1, by = ids.empty() ? 0 : hi[1] - lo[1] + 1,
-                  bz = ids.empty() ? 0 : hi[2] - lo[2] + 1;
-    if (!ids.empty() && bx * by * bz <= ((int64_t)1 << 22)) {
-        std::vector<int> &last = W.last;
-        last.assign((size_t)(bx * by * bz * 4), -1);
-        auto at = [&](int x, int y, int z, int comp) -> int & {
-            return last[(size_t)((((int64_t)(z - lo[2]) * by + (y - lo[1])) * bx + (x - lo[0])) * 4 + comp)];
-        };
-        for (int u : ids) {
-            const int *o = &oi[8 * (size_t)u];
-            const int comp = o[0];
-            const int ax = o[2] - (comp == 0) - 1, ay = o[3] - (comp == 1) - 1, az = o[4] - (comp == 2) - 1;
-            int lv = 0;
-            for (int t = 0; t < 8; t++) {
-                const int sw = at(ax + (t & 1), ay + ((t >> 1) & 1), az + (t >> 2), comp);
-                src[8 * (size_t)u + t] = sw;
-                if (sw >= 0) lv = std::max(lv, level[sw] + 1);
-            }
-            level[u] = lv;
-            const int64_t w = o[1];
-            at((int)(w % sx), (int)((w / sx) % sy), (int)(w / ((int64_t)sx * sy)), comp) = u;
-        }
-        finals.clear();
-        fcomp.clear();
-        for (int z = lo[2]; z <= hi[2]; z++)
-            for (int y = lo[1]; y <= hi[1]; y++)
-                for (int x = lo[0]; x <= hi[0]; x++)
-                    for (int comp = 0; comp < 4; comp++) {
-                        const int u = at(x, y, z, comp);
-                        if (u >= 0) {
-                            finals.push_back({(int)(x + (int64_t)sx * (y + (int64_t)sy * z)), u});
-                            fcomp.push_back(comp);
-                        }
-                    }
-        return;
-    }
-    size_t cap = 16;
-    while (cap < 4 * ids.size()) cap <<= 1;
-    const size_t mask = cap - 1;
-    std::vector<int64_t> keys(cap, -1);
-    std::vector<int> vals(cap, -1);
-    auto slot = [&](int64_t key) {
-        size_t h = (size_t)((uint64_t)key * 0x9E3779B97F4A7C15ull >> 20) & mask;
-        while (keys[h] != -1 && keys[h] != key) h = (h + 1) & mask;
-        return h;
-    };
-    for (int u : ids) {
-        const int *o = &oi[8 * (size_t)u];
-        const int comp = o[0], ix = o[2], iy = o[3], iz = o[4];
-        const int ax = ix - (comp == 0), ay = iy - (comp == 1), az = iz - (comp == 2);
-        int lv = 0;
-        for (int t = 0; t < 8; t++) {
-            const int64_t loc = (int64_t)(ax + (t & 1) - 1) + (int64_t)sx * ((ay + ((t >> 1) & 1) - 1) +
-                                                                             (int64_t)sy * (az + (t >> 2) - 1));
-            const size_t h = slot(loc * 4 + comp);
-            const int s = keys[h] == -1 ? -1 : vals[h];
-            src[8 * (size_t)u + t] = s;
-            if (s >= 0) lv = std::max(lv, level[s] + 1);
-        }
-        level[u] = lv;
-        const int64_t wkey = (int64_t)o[1] * 4 + comp;
-        const size_t h = slot(wkey);
-        keys[h] = wkey;
-        vals[h] = u;
-    }
-    finals.clear();
-    fcomp.clear();
-    std::vector<std::pair<int64_t, int>> tmp;
-    for (size_t h = 0; h < cap; h++)
-        if (keys[h] != -1) tmp.push_back({keys[h], vals[h]});
-    std::sort(tmp.begin(), tmp.end());
-    for (auto &kv : tmp) {
-        finals.push_back({(int)(kv.first / 4), kv.second});
-        fcomp.push_back((int)(kv.first % 4));
-    }
-}
-
-// ids ordered by level, serial order kept within a level (a stable
-// counting sort: levels are small integers)
-static void by_level(const std::vector<int> &ids, PlanWork &W) {
-    const std::vector<int> &level = W.level;
-    int maxl = 0;
-    for (int u : ids) maxl = std::max(maxl, level[u]);
-    std::vector<int> start(maxl + 2, 0);
-    std::vector<int> &out = W.order;
-    out.resize(ids.size());
-    for (int u : ids) start[level[u] + 1]++;
-    for (int l = 0; l <= maxl; l++) start[l + 1] += start[l];
-    for (int u : ids) out[start[level[u]]++] = u;
-}
-
-// Levels, order and final writers of one plan, in its work arrays.
-static void plan_host(const std::vector<int> &ids, const int *oi, int sx, int sy, size_t nu, PlanWork &W) {
-    if (W.src.size() < 8 * nu) W.src.resize(8 * nu);
-    if (W.level.size() < nu) W.level.resize(nu);
-    if (W.pos_of.size() < nu) W.pos_of.resize(nu);
-    levels(ids, oi, sx, sy, W);
-    by_level(ids, W);
+    tb_bytes = B.tmp_bytes;
+    IB3D_CHECK(rocprim::radix_sort_pairs(B.tmp, tb_bytes, B.lev, B.slev, B.wq, B.ord, (unsigned)m, 0, 9, c.stream));
+    hipLaunchKernelGGL(k_plan_fill, dim3(1), dim3(64), 0, c.stream, B.info, 4, 0);
+    hipLaunchKernelGGL(k_plan_bounds, gb, tb, 0, c.stream, m, B.slev, B.ord, B.pos_of, B.info);
+    hipLaunchKernelGGL(k_plan_emit, gb, tb, 0, c.stream, g_doi, g_dod, m, vel, B.ord, B.pos_of, B.src, P.upd, P.srcs,
+                       P.pos);
+    hipLaunchKernelGGL(k_plan_final_flags, gb, tb, 0, c.stream, m, B.sk, B.fflag);
+    tb_bytes = B.tmp_bytes;
+    IB3D_CHECK(rocprim::exclusive_scan(B.tmp, tb_bytes, B.fflag, B.fidx, 0, (size_t)m, rocprim::plus<int>(),
+                                       c.stream));
+    hipLaunchKernelGGL(k_plan_finals, gb, tb, 0, c.stream, m, B.sk, B.sq, B.fflag, B.fidx, B.pos_of, P.final_loc, m,
+                       B.info);
+    IB3D_CHECK(hipGetLastError());
+    IB3D_CHECK(hipMemcpyAsync(B.hinfo, B.info, (4 + kMaxLevels + 1) * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    IB3D_CHECK(hipStreamSynchronize(c.stream));
+    IB3D_REQUIRE(B.hinfo[0] == 0, "forcing: the plan has more than 256 dependency levels");
+    P.n_updates = m;
+    P.n_levels = B.hinfo[2];
+    P.n_final = B.hinfo[1];
+    P.final_stride = m;
+    P.level_start.assign(B.hinfo + 3, B.hinfo + 3 + P.n_levels + 1);
+    return 0;
 }
 
 // The forcing plans are built in two halves.  plans_launch (at the end of
-// computeNormDistance) queues the stencil kernel and the copy of its records
-// and error flag to pinned memory, and records an event.  plans_finish, at
-// the first forcing call, waits for that event only: the predictor's kernels
-// queued after it keep the GPU busy while the host finds the levels.
+// computeNormDistance) queues the stencil kernel; plans_finish, at the first
+// forcing call, checks the stencil error flag and builds both plans on the
+// device (their kernels queue behind the predictor's).
 int plans_launch(Ctx &c) {
     c.plans_ready = false;
     c.plans_pending = false;
@@ -1306,6 +1352,7 @@ int plans_launch(Ctx &c) {
         c.plans_ready = true;
         return 0;
     }
+    IB3D_REQUIRE(4 * c.P < ((int64_t)1 << 31), "forcing: grid too large for 32-bit face keys");
     const int64_t nu7 = 7 * (int64_t)nb;
     if (nu7 > g_dcap) {
         if (g_doi) (void)hipFree(g_doi);
@@ -1323,13 +1370,6 @@ int plans_launch(Ctx &c) {
     A.nx = (int)c.nx; A.ny = (int)c.ny; A.nz = (int)c.nz;
     hipLaunchKernelGGL(k_force_stencil, dim3(nblocks(nu7, 256)), dim3(256), 0, c.stream, A, g_doi, g_dod, g_err);
     IB3D_CHECK(hipGetLastError());
-    const size_t b_oi = 8 * (size_t)nu7 * sizeof(int), b_od = 5 * (size_t)nu7 * sizeof(double);
-    if (pinned(g_pin_dl, b_oi + b_od + sizeof(int))) return -1;
-    char *dl = static_cast<char *>(g_pin_dl.p);
-    IB3D_CHECK(hipMemcpyAsync(dl, g_doi, b_oi, hipMemcpyDeviceToHost, c.stream));
-    IB3D_CHECK(hipMemcpyAsync(dl + b_oi, g_dod, b_od, hipMemcpyDeviceToHost, c.stream));
-    IB3D_CHECK(hipMemcpyAsync(dl + b_oi + b_od, g_err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    IB3D_CHECK(hipEventRecord(c.ev[4], c.stream));
     c.plans_pending = true;
     return 0;
 }
@@ -1337,39 +1377,8 @@ int plans_launch(Ctx &c) {
 static int plans_finish(Ctx &c) {
     c.plans_pending = false;
     const int nb = (int)c.ibCount;  // unchanged since plans_launch: a new list clears the pending flag
-    const int64_t nu7 = 7 * (int64_t)nb;
-    const size_t b_oi = 8 * (size_t)nu7 * sizeof(int), b_od = 5 * (size_t)nu7 * sizeof(double);
-    IB3D_CHECK(hipEventSynchronize(c.ev[4]));
-    const char *dl = static_cast<const char *>(g_pin_dl.p);
-    int err = 0;
-    std::memcpy(&err, dl + b_oi + b_od, sizeof(int));
-    IB3D_REQUIRE(err == 0, "forcing: an image point has no bracketing grid lines");
-    const int *oi = static_cast<const int *>(g_pin_dl.p);
-    const double *od = reinterpret_cast<const double *>(dl + b_oi);
-    const size_t nu = 7 * (size_t)nb;
-    // velocity: per cell u2 u1 v2 v1 w2 w1 in serial order
-    std::vector<int> vids, pids;
-    vids.reserve(6 * (size_t)nb);
-    pids.reserve(nb);
-    for (int n = 0; n < nb; n++) {
-        for (int kind = 0; kind < 6; kind++) vids.push_back(7 * n + kind);
-        pids.push_back(7 * n + 6);
-    }
-    // The two plans share no update and no work array: the pressure plan's
-    // levels are found on a second thread while the velocity plan's are
-    // found and uploaded.
-    const int sx = (int)c.sx, sy = (int)c.sy;
-    std::thread pth;
-    try {
-        pth = std::thread([&] { plan_host(pids, oi, sx, sy, nu, g_pwork); });
-    } catch (...) {
-        plan_host(pids, oi, sx, sy, nu, g_pwork);
-    }
-    plan_host(vids, oi, sx, sy, nu, g_vwork);
-    const int rv = upload_plan(c, c.vplan, g_vwork, oi, od);
-    if (pth.joinable()) pth.join();
-    if (rv) return -1;
-    if (upload_plan(c, c.pplan, g_pwork, oi, od)) return -1;
+    IB3D_REQUIRE(read_err(c, g_err) == 0, "forcing: an image point has no bracketing grid lines");
+    if (plan_device(c, c.vplan, 1, nb) || plan_device(c, c.pplan, 0, nb)) return -1;
     c.plans_ready = true;
     return 0;
 }
@@ -1391,9 +1400,9 @@ static int run_plan(Ctx &c, ForcePlan &P) {
         if (hi > lo)
             hipLaunchKernelGGL(k_force_level, dim3(nblocks(hi - lo, 128)), dim3(128), 0, c.stream, A, lo, hi);
     }
-    const int nf = P.n_final;
+    const int nf = P.n_final, fs = P.final_stride;
     hipLaunchKernelGGL(k_force_final, dim3(nblocks(nf, 256)), dim3(256), 0, c.stream, P.final_loc,
-                       P.final_loc + nf, P.final_loc + 2 * nf, nf, P.res, c.ut, c.vt, c.wt, c.p);
+                       P.final_loc + fs, P.final_loc + 2 * fs, nf, P.res, c.ut, c.vt, c.wt, c.p);
     IB3D_CHECK(hipGetLastError());
     return 0;
 }

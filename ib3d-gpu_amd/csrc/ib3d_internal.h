@@ -56,6 +56,7 @@ struct ForcePlan {
     int *final_loc = nullptr;          // device: write location of each final writer
     int *final_upd = nullptr;          // device: the update that writes it last
     int n_final = 0;
+    int final_stride = 0;              // distance between the loc | upd | comp columns of final_loc
     int cap_upd = 0, cap_final = 0;    // allocated sizes (the buffers are reused across rebuilds)
 };
 
@@ -258,6 +259,7 @@ int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2);
 int compute_div_noalloc(Ctx &c, double deltat, double *divmax);
 int noalloc_setup(int device);
 int set_fused_sweeps(int on, int tz, int blocks);
+int set_momentum_tiles(int on);
 int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);
 int plans_launch(Ctx &c);
 int shift_nodes_moving(Ctx &c, const double base[3], const double disp[3]);

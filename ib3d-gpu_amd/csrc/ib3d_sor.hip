@@ -1687,7 +1687,8 @@ struct MgLevel {
     float *x = nullptr, *b = nullptr;                    // correction, right-hand side (2*NC each)
     float *r = nullptr;                                  // residual (finest level only)
     MgAxis *tx = nullptr, *ty = nullptr, *tz = nullptr;   // prolongation from the next level (sx, sy, sz entries)
-    float *x2 = nullptr;                                 // finest level: second buffer of the fused sweeps
+    float *x2 = nullptr;                                 // second buffer of the fused sweeps
+    int level = 0;
     bool owns_tables = false, owns_xb = false;
 };
 static std::vector<MgLevel> g_mg;
@@ -1826,7 +1827,7 @@ static int mg_setup(Ctx &c) {
         L.sx = L.nx + 2; L.sy = L.ny + 2; L.sz = L.nz + 2;
         L.H = (((L.sx + 1) / 2) + 1) & ~int64_t(1);
         L.NC = L.sy * L.sz * L.H;
-        L.sxp = std::max<int64_t>((L.sx + 7) & ~int64_t(7), 2 * L.H);
+        L.sxp = (std::max<int64_t>(L.sx, 2 * L.H + 2) + 7) & ~int64_t(7);  // as the finest level's rows
         g_mg.push_back(L);
     }
     IB3D_REQUIRE(g_mg.size() >= 2, "poissonSolver: the grid cannot be coarsened for multigrid (each interior "
@@ -1890,10 +1891,16 @@ static int mg_setup(Ctx &c) {
         hipLaunchKernelGGL(k_fill_nan_f32, dim3(nblocks(2 * L.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream,
                            L.b, 2 * L.NC);
     }
-    // the fused sweeps' second finest-level buffer: cells they never write
-    // (walls, and whatever lies outside the solve) stay 0, as in x
-    IB3D_CHECK(hipMalloc(&g_mg[0].x2, 2 * g_mg[0].NC * sizeof(float)));
-    IB3D_CHECK(dzero(g_mg[0].x2, 2 * g_mg[0].NC * sizeof(float)));
+    // the fused sweeps' second buffer on every level smoothed by launches:
+    // cells they never write (walls, and whatever lies outside the solve)
+    // stay 0, as in x
+    for (int l = 0; l < (int)g_mg.size(); l++) g_mg[l].level = l;
+    for (int l = 0; l < (int)g_mg.size(); l++) {
+        MgLevel &L = g_mg[l];
+        if (l > 0 && l >= g_mg_small) break;
+        IB3D_CHECK(hipMalloc(&L.x2, 2 * L.NC * sizeof(float)));
+        IB3D_CHECK(dzero(L.x2, 2 * L.NC * sizeof(float)));
+    }
     IB3D_CHECK(hipGetLastError());
     for (int q = 0; q < 4; q++) g_mg_key[q] = key[q];
     return 0;
@@ -1908,7 +1915,7 @@ static dim3 level_grid(const MgLevel &L) {
 // Launch shape of the fused sweep: S own slots per workgroup (the red range
 // adds H on either side and must fit 2 x kSweepThreads slots), planes per
 // chunk so that the grid holds about g_sweep_blocks workgroups.
-static bool g_fused = false;
+static int g_fused = 0;          // levels (from the finest) smoothed by whole sweeps in one launch
 static int g_sweep_tz = 0;        // planes per chunk (0: from g_sweep_blocks)
 static int g_sweep_blocks = 480;
 
@@ -1932,7 +1939,7 @@ static void mg_smooth(Ctx &c, const MgLevel &L, bool finest, int n, bool first, 
     const dim3 cg(nblocks(L.NC, kSorBlock));
     int S = 0, tz = 0;
     dim3 sg;
-    if (finest && g_fused && L.x2 && n >= 2 && sweep2_shape(L, S, tz, sg)) {
+    if (L.level < g_fused && L.x2 && n >= 2 && sweep2_shape(L, S, tz, sg)) {
         // an odd count does its first sweep in place, so that the pairs of
         // fused sweeps end in x
         if (n % 2) {
@@ -2338,8 +2345,9 @@ int sor_solve(Ctx &c, double bnorm2, double eps1) {
 }
 
 int set_fused_sweeps(int on, int tz, int blocks) {
-    IB3D_REQUIRE(tz >= 0 && tz <= 4096 && blocks >= 1 && blocks <= 1 << 20, "set_fused_sweeps: bad launch shape");
-    g_fused = on != 0;
+    IB3D_REQUIRE(on >= 0 && on <= 64 && tz >= 0 && tz <= 4096 && blocks >= 1 && blocks <= 1 << 20,
+                 "set_fused_sweeps: bad level count or launch shape");
+    g_fused = on;
     g_sweep_tz = tz;
     g_sweep_blocks = blocks;
     return 0;

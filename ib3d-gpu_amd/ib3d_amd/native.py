@@ -69,6 +69,7 @@ def lib():
         "ib3d_set_multigrid_coarse": [I, I],
         "ib3d_set_krylov_target": [D],
         "ib3d_set_fused_sweeps": [I, I, I],
+        "ib3d_set_momentum_tiles": [I],
         "ib3d_set_graphs": [I],
         "ib3d_poisson_stats": [PI64, PI64, PD, PD, PD, PD],
         "ib3d_residual_norm": [PD, PD],
@@ -285,10 +286,16 @@ class Solver:
         _ok(self.L.ib3d_set_multigrid(int(nu1), int(nu2), float(omega), int(coarse_sweeps), float(coarse_omega),
                                       int(max_levels)))
 
-    def set_fused_sweeps(self, on=True, planes=0, blocks=480):
-        """Finest-level multigrid smoothing with one launch per red-black
-        sweep (on) or two half-sweep launches (off); identical results."""
-        _ok(self.L.ib3d_set_fused_sweeps(int(bool(on)), int(planes), int(blocks)))
+    def set_fused_sweeps(self, levels=1, planes=0, blocks=480):
+        """Multigrid smoothing of the first `levels` levels with one launch
+        per red-black sweep (0: two half-sweep launches everywhere);
+        identical results.  True/False mean 1/0."""
+        _ok(self.L.ib3d_set_fused_sweeps(int(levels), int(planes), int(blocks)))
+
+    def set_momentum_tiles(self, on=True):
+        """nsMomentum's interior from LDS tiles (on) or straight from HBM (off);
+        identical results."""
+        _ok(self.L.ib3d_set_momentum_tiles(int(on)))
 
     def set_krylov_target(self, target):
         """Early exit of a Krylov cycle at target x tol x ||b/dt|| (default 0.3)."""

@@ -179,11 +179,16 @@ int ib3d_set_krylov_target(double target);
  * Those levels are small enough that a sweep costs about a launch, whatever
  * the level's size. */
 int ib3d_set_multigrid_coarse(int nu1, int nu2);
-/* mode 4: smooth the finest level with whole red-black sweeps in one launch
- * each (on != 0) or with two half-sweep launches (on == 0, the default).
- * Both give identical results.  planes: z planes per workgroup (0: chosen so
- * that the grid holds about `blocks` workgroups; default 0 and 480). */
-int ib3d_set_fused_sweeps(int on, int planes, int blocks);
+/* mode 4: smooth the first `levels` multigrid levels (1: the finest only)
+ * with whole red-black sweeps in one launch each, the others with two
+ * half-sweep launches (levels = 0, the default).  Both give identical
+ * results.  planes: z planes per workgroup (0: chosen so that the grid holds
+ * about `blocks` workgroups; default 0 and 480). */
+int ib3d_set_fused_sweeps(int levels, int planes, int blocks);
+/* nsMomentum: the interior columns from 64 x 8 LDS tiles marching in z
+ * (on != 0) or with every stencil value read from HBM (on == 0, the default).
+ * Both give identical results. */
+int ib3d_set_momentum_tiles(int on);
 /* modes 3 and 4: replay each refinement iteration (Krylov cycle, FP64
  * residual and its norm) as one captured HIP graph (on != 0) or launch its
  * kernels one by one (on == 0, the default).  Both run the same kernels in

@@ -76,11 +76,16 @@ def test_adam_matches_oracle(net):
     np.testing.assert_allclose(net.params(), p, rtol=1e-5, atol=1e-7)
 
 
-def run_steps(case, n_before, n_after, guess, net, record=False):
+def run_steps(case, n_before, n_after, guess, net, record=False, preset=None, accept=1.0):
+    from ib3d_amd import presets
     from ib3d_amd.native import Solver
     s = Solver.from_case(case)
-    s.set_poisson_mode(1, omega=1.99, tol=1e-6, check=20, warm=True)
+    if preset is None:
+        s.set_poisson_mode(1, omega=1.99, tol=1e-6, check=20, warm=True)
+    else:
+        presets.apply(s, preset)
     net.config(record=record, guess=False)
+    net.set_accept(accept)
     for _ in range(n_before):
         s.step()
     net.config(record=False, guess=guess)
@@ -88,20 +93,34 @@ def run_steps(case, n_before, n_after, guess, net, record=False):
     for _ in range(n_after):
         s.step()
         st = s.poisson_stats()
-        row = {"sweeps": st["sweeps"], "relres": st["relres"], "relres0": st["relres0"]}
+        row = {"sweeps": st["sweeps"], "relres": st["relres"], "relres0": st["relres0"],
+               "poisson_ms": st["poisson_ms"]}
         if guess:
             row.update(net.stats())
         out.append(row)
     net.config(False, False)
+    net.set_accept(1.0)
     s.close()
     return out
 
 
 def test_learned_guess_ablation(case_dir, net):
     """Train on pairs dumped by the first 40 solves of the 128x64x64 case, then
-    compare steps 41-60 started from the network's guess against the
-    previous-step guess.  The fallback rule must hold on every solve; the
-    gain is printed (sweeps per solve with and without the network)."""
+    run steps 41-60 from the network's guess and from the previous-step guess.
+
+    * FP64 red-black SOR (the reference's smoother, L2 rule): the guess lowers
+      the initial residual and the SOR sweeps to the same 1e-6 residual; the
+      gain is asserted and printed.
+    * The benchmark's multigrid-preconditioned solver, with the benchmark's
+      acceptance ratio (a guess is kept only when it halves the previous-step
+      guess's residual): the difference in Krylov steps and Poisson ms is
+      printed with its sign.  There the network does not pay: the V-cycle
+      removes in one Krylov step what a 4-cell receptive field can predict,
+      and even starts 5-40x closer in residual (temporal extrapolation of the
+      previous corrections) need as many or more Krylov steps.
+
+    In both the fallback rule holds on every solve and every solve meets the
+    tolerance."""
     from ib3d_amd.native import Net
     case = case_dir("cfg1")
     net.release()
@@ -114,6 +133,7 @@ def test_learned_guess_ablation(case_dir, net):
     _say("[gpu] loss per epoch: " + " ".join(f"{l:.3e}" for l in loss))
     assert np.all(np.isfinite(loss)) and loss[-1] < loss[0]
     n2.clear_pairs()
+    # FP64 SOR
     base = run_steps(case, 40, 20, False, n2)
     learned = run_steps(case, 40, 20, True, n2)
     for r in learned:
@@ -122,7 +142,24 @@ def test_learned_guess_ablation(case_dir, net):
     sb = np.array([r["sweeps"] for r in base])
     sl = np.array([r["sweeps"] for r in learned])
     kept = learned[-1]["kept"]
-    _say(f"[gpu] ablation: sweeps per solve, previous-step guess {sb.mean():.1f}, learned guess {sl.mean():.1f} "
-         f"(kept {kept}/{len(learned)}); initial residual prev {np.mean([r['rel_prev'] for r in learned]):.3e}, "
-         f"learned {np.mean([r['rel_guess'] for r in learned]):.3e}; guess cost {learned[-1]['guess_ms']:.2f} ms")
+    _say(f"[gpu] ablation, FP64 SOR: sweeps per solve, previous-step guess {sb.mean():.1f}, learned guess "
+         f"{sl.mean():.1f} ({sl.mean() - sb.mean():+.1f}, kept {kept}/{len(learned)}); initial residual prev "
+         f"{np.mean([r['rel_prev'] for r in learned]):.3e}, learned {np.mean([r['rel_guess'] for r in learned]):.3e}; "
+         f"guess cost {learned[-1]['guess_ms']:.2f} ms")
+    assert sl.mean() < sb.mean()
+    # the multigrid-preconditioned mixed solver, the benchmark's acceptance ratio
+    k0 = learned[-1]["kept"]
+    base4 = run_steps(case, 40, 20, False, n2, preset="mixed")
+    learned4 = run_steps(case, 40, 20, True, n2, preset="mixed", accept=0.5)
+    for r in base4 + learned4:
+        assert r["relres"] <= 1e-6
+    for r in learned4:
+        assert r["relres0"] <= r["rel_prev"] * (1 + 1e-12)
+    kb = np.array([r["sweeps"] for r in base4])
+    kl = np.array([r["sweeps"] for r in learned4])
+    pb = np.array([r["poisson_ms"] for r in base4])
+    pl = np.array([r["poisson_ms"] for r in learned4])
+    _say(f"[gpu] ablation, multigrid GMRES-IR (accept 0.5): Krylov steps per solve {kb.mean():.2f} -> {kl.mean():.2f} "
+         f"({kl.mean() - kb.mean():+.2f}), Poisson {pb.mean():.3f} -> {pl.mean():.3f} ms ({pl.mean() - pb.mean():+.3f}), "
+         f"kept {learned4[-1]['kept'] - k0}/{len(learned4)}")
     n2.release()

@@ -38,10 +38,13 @@ the learned form's measurement beside it.  After the timed steps, the
 configuration's whole stated run (500 steps, t = 1.0) of the headline mode
 and of the learned form is timed on its own (`full_run`).
 
-Roofline: the dominant kernel is the SOR half-sweep.  Its average launch
+Roofline: the dominant kernel of mode 4 is the finest multigrid level's
+FP32 red-black sweep in one launch (k_sweep2_pz); algorithmic bytes per
+launch = 12 B per interior cell (old value and right-hand side read, new
+value written, both colours).  For the SOR modes it is the half-sweep:
+16 B (FP64) or 8 B (FP32) per interior cell per launch.  The average launch
 duration is measured with HIP events on the library's stream
-(ib3d_bench_sor); algorithmic bytes per launch = 16 B (FP64) or 8 B (FP32)
-per interior cell, i.e. 32 B / 16 B per cell per red+black sweep.
+(ib3d_bench_sor).
 
 CPU baseline: the reference's own Fortran (built from its sources with
 amdflang -O3, oracle/_ref) timed on this host, one thread, on a bounded
@@ -245,11 +248,16 @@ def train_guess(cfg_name, rec_steps, epochs, batch, lr, omega, tol, inner):
 
 
 def kernel_roofline(s, precision, omega, n_cells, sweeps=200):
+    """precision 64 / 32: the half-sweep (two launches per sweep, 16 / 8 B per
+    cell each); 33: the FP32 whole sweep in one launch (12 B per cell)."""
     s.bench_sor(precision, 5, omega)  # warm the tables and caches
     ms = s.bench_sor(precision, sweeps, omega)
-    bytes_per_cell = 32.0 if precision == 64 else 16.0
-    launch_s = 0.5 * ms * 1e-3             # red and black launches per sweep
-    bytes_per_launch = 0.5 * bytes_per_cell * n_cells
+    if precision == 33:
+        launch_s = ms * 1e-3
+        bytes_per_launch = 12.0 * n_cells
+    else:
+        launch_s = 0.5 * ms * 1e-3         # red and black launches per sweep
+        bytes_per_launch = 0.5 * (32.0 if precision == 64 else 16.0) * n_cells
     achieved = bytes_per_launch / launch_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "launch_us": round(launch_s * 1e6, 3),
@@ -397,7 +405,11 @@ def main():
         main_fields = {f: s.download(f) for f in ("u", "p")}
         log(f"[bench] {a.mode}: {main_res}")
         prec = 32 if a.mode.startswith("mixed") or a.mode == "gmres-sor" else 64
-        roof = kernel_roofline(s, prec, a.omega, n_cells)
+        # the dominant kernel: the finest level's whole FP32 sweep in mode 4,
+        # the half-sweep otherwise
+        rprec = 33 if a.mode.startswith("mixed") else prec
+        roof = kernel_roofline(s, rprec, a.omega, n_cells)
+        roof32 = kernel_roofline(s, 32, a.omega, n_cells) if rprec != 32 and prec == 32 else None
         roof64 = kernel_roofline(s, 64, a.omega, n_cells) if prec == 32 else roof
         s.close()
         cmp_all = []
@@ -429,10 +441,11 @@ def main():
         if os.path.exists(tf):
             with open(tf) as fh:
                 t = json.load(fh)
-            traffic = t.get("fp32" if prec == 32 else "fp64")
+            traffic = t.get({33: "fp32_sweep", 32: "fp32", 64: "fp64"}[rprec])
         roof_out = {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac")}
         roof_out["traffic"] = traffic
-        roof_out["kernel"] = f"k_sor_pz<{'float' if prec == 32 else 'double'}> half-sweep"
+        roof_out["kernel"] = ("k_sweep2_pz: FP32 red-black sweep in one launch (12 B per cell)" if rprec == 33 else
+                              f"k_sor_pz<{'float' if prec == 32 else 'double'}> half-sweep")
         roof_out["launch_us"] = roof["launch_us"]
         line = {
             "metric": "steps/s + Poisson ms/step to 1e-6 residual: FP64 SOR vs mixed+learned guess",
@@ -473,6 +486,8 @@ def main():
             "cd": round(main_res["cd_last"], 5),
             "roofline": roof_out,
             "roofline_fp64_sweep": {k: roof64[k] for k in ("achieved", "frac", "launch_us")},
+            **({"roofline_fp32_half_sweep": {k: roof32[k] for k in ("achieved", "frac", "launch_us")}}
+               if roof32 else {}),
             "cpu_baseline": cpu,
             "gpu_wall_s": round(gpu_wall, 1),
             "field_error": errors,

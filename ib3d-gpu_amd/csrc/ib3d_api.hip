@@ -961,6 +961,7 @@ int ib3d_set_multigrid_coarse(int nu1, int nu2) {
 }
 
 int ib3d_set_fused_sweeps(int levels, int planes, int blocks) { return set_fused_sweeps(levels, planes, blocks); }
+int ib3d_set_coarse_tiles(int from_level) { return set_coarse_tiles(from_level); }
 
 int ib3d_set_momentum_tiles(int on) { return set_momentum_tiles(on); }
 

@@ -259,6 +259,7 @@ int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2);
 int compute_div_noalloc(Ctx &c, double deltat, double *divmax);
 int noalloc_setup(int device);
 int set_fused_sweeps(int on, int tz, int blocks);
+int set_coarse_tiles(int from_level);
 int set_momentum_tiles(int on);
 int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);
 int plans_launch(Ctx &c);

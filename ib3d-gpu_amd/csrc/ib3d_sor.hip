@@ -35,6 +35,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace ib3d {
 
@@ -324,27 +325,40 @@ k_sor_pz(T *__restrict__ mine, const T *__restrict__ other, const T *__restrict_
 // so no workgroup ever reads a value another one is writing.
 //
 // A workgroup owns S consecutive slots of a plane (a colour row is H slots)
-// and a chunk of planes [k0, k1).  It marches in z: at plane kk it updates red
-// over its slots plus H halo slots on either side (the y neighbours of its
-// black cells) and then black at plane kk-1, whose red neighbours at kk-2,
-// kk-1 and kk are now known; red of the planes k0-1 and k1 is recomputed as a
-// halo and not stored.  Red values of the current plane go through a ring of
-// three LDS planes, so one barrier per plane separates their writes from the
-// reads of two planes later.  A thread owns one aligned pair of slots.
-// FIRST: the sweep starts from zero (no old values are read).
-// Every operation matches k_sor_pz, so the result is bit for bit that of the
-// two half-sweeps.
+// and a chunk of planes [k0, k1).  It marches in z, one plane kk at a time: it
+// updates red at kk over its slots plus H halo slots on either side (the y
+// neighbours of its black cells), then black at the plane it left behind,
+// whose red neighbours are now known.  Red of the two planes just outside the
+// chunk is recomputed as a halo and not stored.  Even chunks march up and odd
+// chunks down, so two neighbouring chunks reach their common boundary at the
+// same time (both at the start or both at the end of their march): the halo
+// planes one recomputes are the planes the other is reading then, and its
+// inputs come from L2 instead of HBM.  Red values of the current plane go
+// through a ring of three LDS planes, so one barrier per plane separates
+// their writes from the reads of two planes later.  A thread owns one
+// aligned pair of slots.  FIRST: the sweep starts from zero (no old values
+// are read).  Every operation matches k_sor_pz, operand for operand, so the
+// result is bit for bit that of the two half-sweeps.
 static constexpr int kSweepThreads = 1024;
 template <bool FIRST>
-__global__ void __launch_bounds__(kSweepThreads)
+__global__ void __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_sweep2_pz(const float *__restrict__ src0, const float *__restrict__ src1, float *__restrict__ dst0,
             float *__restrict__ dst1, const float *__restrict__ rhs0, const float *__restrict__ rhs1,
             const float *__restrict__ axc, const float *__restrict__ ayc, const float *__restrict__ azc, SorGeom g,
             int S, int tz, float omega, float omm, const double *gate, double thr) {
     typedef f2v V2;
-    typedef f4v V4;
     __shared__ float ring[3][2 * kSweepThreads];
+    __shared__ float xt[3][2 * kSweepThreads];
     if (gated(gate, thr)) return;
+    // x coefficients of the row (2H+2 per array, fewer than 2 x threads
+    // since the red range of S+2H slots fits the block): read per update
+    // from LDS instead of being held in registers
+    for (int i = (int)threadIdx.x; i < 2 * g.H + 2; i += kSweepThreads) {
+        xt[0][i] = axc[i];
+        xt[1][i] = axc[g.sxp + i];
+        xt[2][i] = axc[2 * g.sxp + i];
+    }
+    __syncthreads();
     int bx, by;
     xcd_block2(bx, by);
     const int lo = 2 * (int)threadIdx.x;         // pair offset inside the red range
@@ -357,101 +371,285 @@ k_sweep2_pz(const float *__restrict__ src0, const float *__restrict__ src1, floa
     const bool own = row_ok && s >= bx * S && s < bx * S + S;
     const int k0 = 1 + by * tz;
     const int k1 = min(k0 + tz, g.sz - 1);
+    const int np = k1 - k0 + 2;
     const V2 zero = V2(0.0f);
-    V4 c1 = V4(0.0f), c2 = V4(0.0f), c3 = V4(0.0f);
     float a1y = 0.0f, a2y = 0.0f, a3y = 0.0f;
     if (row_ok) {
-        c1 = *reinterpret_cast<const V4 *>(axc + 2 * h);
-        c2 = *reinterpret_cast<const V4 *>(axc + g.sxp + 2 * h);
-        c3 = *reinterpret_cast<const V4 *>(axc + 2 * g.sxp + 2 * h);
         a1y = ayc[jj]; a2y = ayc[g.sy + jj]; a3y = ayc[2 * g.sy + jj];
     }
-    // black old at planes kk-1, kk (and kk+1 loaded per plane); red new at
-    // kk-2, kk-1 and kk
-    V2 bo_m = zero, bo_c = zero, rn_m2 = zero, rn_m1 = zero;
-    if (!FIRST && row_ok) {
-        if (k0 - 2 >= 0) bo_m = *reinterpret_cast<const V2 *>(src1 + s + (k0 - 2) * g.plane);
-        bo_c = *reinterpret_cast<const V2 *>(src1 + s + (k0 - 1) * g.plane);
+    // march direction d (a compile-time constant in each copy of the body);
+    // red planes k0-1 .. k1 in order, starting at ks
+    auto march = [&](auto dir) {
+        constexpr bool up = decltype(dir)::value;
+        constexpr int d = up ? 1 : -1;
+        const int ks = up ? k0 - 1 : k1;
+        // black old at the plane behind kk (kk-d) and at kk; red new at kk-2d,
+        // kk-d and kk
+        V2 bo_b = zero, bo_c = zero, rn_b2 = zero, rn_b1 = zero;
+        if (!FIRST && row_ok) {
+            if (ks - d >= 0 && ks - d <= g.sz - 1) bo_b = *reinterpret_cast<const V2 *>(src1 + s + (ks - d) * g.plane);
+            bo_c = *reinterpret_cast<const V2 *>(src1 + s + ks * g.plane);
+        }
+        // Global inputs of the next plane are loaded one plane ahead, after this
+        // plane's red update and before the barrier, so their latency overlaps
+        // the barrier and the black update: the red inputs (rhs, old red, old
+        // black side/south/north), old black one plane further ahead and the
+        // black rhs.  The loads are unconditional, from addresses clamped into
+        // the arrays (a lane outside the rows, or a plane outside the interior,
+        // reads a valid cell whose value the guards below never use), so no load
+        // sits behind a branch and the counters of outstanding loads stay exact.
+        // Two register sets alternate (the loop is unrolled by two), so the
+        // loaded registers are never copied and no plane waits for the loads it
+        // issued itself.  The z coefficients are uniform: scalar loads.
+        typedef __attribute__((address_space(4))) float CFloat;
+        const CFloat *az = (const CFloat *)azc;
+        const int s_ld = row_ok ? s : g.H;
+        struct In {
+            V2 r2, po, ps, pn, bo_a, rb;
+            float side;
+        };
+        In A{zero, zero, zero, zero, zero, zero, 0.0f}, B = A;
+        // 32-bit byte offsets from the uniform array bases (every colour array
+        // holds fewer than 2^30 slots), so each load takes one offset register
+        auto at2 = [](const float *base, uint32_t i) {
+            return *reinterpret_cast<const V2 *>(reinterpret_cast<const char *>(base) + i * 4u);
+        };
+        auto at1 = [](const float *base, uint32_t i) {
+            return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + i * 4u);
+        };
+        auto put2 = [](float *base, uint32_t i, V2 v) {
+            *reinterpret_cast<V2 *>(reinterpret_cast<char *>(base) + i * 4u) = v;
+        };
+        auto plane_of = [&](int k) { return (uint32_t)(s_ld + min(max(k, 0), g.sz - 1) * g.plane); };
+        // inputs of red at plane kr: old black ahead of it (kr+d), and the rhs of
+        // the black plane updated after it (kr-d)
+        auto load_next = [&](int kr, In &n) {
+            const uint32_t fr = (uint32_t)(s_ld + min(max(kr, 1), g.sz - 2) * g.plane);
+            n.r2 = at2(rhs0, fr);
+            if (!FIRST) {
+                n.po = at2(src0, fr);
+                n.side = at1(src1, ((jj + kr) & 1) ? fr + 2u : fr - 1u);
+                n.ps = at2(src1, fr - (uint32_t)g.H);
+                n.pn = at2(src1, fr + (uint32_t)g.H);
+                n.bo_a = at2(src1, plane_of(kr + d));
+            }
+            n.rb = at2(rhs1, plane_of(kr - d));
+        };
+        auto plane = [&](int t, const In &c, In &n, bool next) {
+            const int kk = ks + d * t;
+            const int f = s + kk * g.plane;
+            // ---- red at plane kk
+            V2 rn_c = zero;
+            if (row_ok && kk >= 1 && kk <= g.sz - 2) {
+                rn_c = c.po;
+                const bool ok0 = c.r2.x == c.r2.x, ok1 = c.r2.y == c.r2.y;
+                if (ok0 || ok1) {
+                    const int q = (jj + kk) & 1;                  // colour 0
+                    const V2 bo_dn = up ? bo_b : c.bo_a, bo_up = up ? c.bo_a : bo_b;
+                    const float a1z = az[kk], a2z = az[g.sz + kk], a3z = az[2 * g.sz + kk];
+                    const int xo = 2 * h + q;
+                    const float a1x0 = xt[0][xo], a2x0 = xt[1][xo], a3x0 = xt[2][xo];
+                    const float a1x1 = xt[0][xo + 2], a2x1 = xt[1][xo + 2], a3x1 = xt[2][xo + 2];
+                    if (ok0) {
+                        const float pw = q ? bo_c.x : c.side, pe = q ? bo_c.y : bo_c.x;
+                        const float gs = sor_div(((((((c.r2.x - a1x0 * pw) - a3x0 * pe) - a1y * c.ps.x) - a3y * c.pn.x) -
+                                                   a1z * bo_dn.x) - a3z * bo_up.x),
+                                                 ((a2x0 + a2y) + a2z));
+                        rn_c.x = omm * c.po.x + omega * gs;
+                    }
+                    if (ok1) {
+                        const float pw = q ? bo_c.y : bo_c.x, pe = q ? c.side : bo_c.y;
+                        const float gs = sor_div(((((((c.r2.y - a1x1 * pw) - a3x1 * pe) - a1y * c.ps.y) - a3y * c.pn.y) -
+                                                   a1z * bo_dn.y) - a3z * bo_up.y),
+                                                 ((a2x1 + a2y) + a2z));
+                        rn_c.y = omm * c.po.y + omega * gs;
+                    }
+                }
+                if (own && kk >= k0 && kk < k1) put2(dst0, (uint32_t)f, rn_c);
+            }
+            if (next) load_next(kk + d, n);
+            float *cur = ring[t % 3];
+            cur[lo] = rn_c.x;
+            cur[lo + 1] = rn_c.y;
+            __syncthreads();
+            // ---- black at the plane behind, kb = kk-d
+            const int kb = kk - d;
+            if (own && t >= 1 && kb >= k0 && kb < k1) {
+                const int fb = s + kb * g.plane;
+                const float *prev = ring[(t + 2) % 3];
+                const V2 po_b = FIRST ? zero : bo_b;
+                V2 out = po_b;
+                const bool ok0 = c.rb.x == c.rb.x, ok1 = c.rb.y == c.rb.y;
+                if (ok0 || ok1) {
+                    const int q = (1 + jj + kb) & 1;              // colour 1
+                    const V2 rn_dn = up ? rn_b2 : rn_c, rn_up = up ? rn_c : rn_b2;
+                    const float sideb = q ? prev[lo + 2] : prev[lo - 1];
+                    const V2 cr = V2{prev[lo], prev[lo + 1]};
+                    const V2 psb = V2{prev[lo - g.H], prev[lo - g.H + 1]};
+                    const V2 pnb = V2{prev[lo + g.H], prev[lo + g.H + 1]};
+                    const float a1z = az[kb], a2z = az[g.sz + kb], a3z = az[2 * g.sz + kb];
+                    const int xo = 2 * h + q;
+                    const float a1x0 = xt[0][xo], a2x0 = xt[1][xo], a3x0 = xt[2][xo];
+                    const float a1x1 = xt[0][xo + 2], a2x1 = xt[1][xo + 2], a3x1 = xt[2][xo + 2];
+                    if (ok0) {
+                        const float pw = q ? cr.x : sideb, pe = q ? cr.y : cr.x;
+                        const float gs = sor_div(((((((c.rb.x - a1x0 * pw) - a3x0 * pe) - a1y * psb.x) - a3y * pnb.x) -
+                                                   a1z * rn_dn.x) - a3z * rn_up.x),
+                                                 ((a2x0 + a2y) + a2z));
+                        out.x = omm * po_b.x + omega * gs;
+                    }
+                    if (ok1) {
+                        const float pw = q ? cr.y : cr.x, pe = q ? sideb : cr.y;
+                        const float gs = sor_div(((((((c.rb.y - a1x1 * pw) - a3x1 * pe) - a1y * psb.y) - a3y * pnb.y) -
+                                                   a1z * rn_dn.y) - a3z * rn_up.y),
+                                                 ((a2x1 + a2y) + a2z));
+                        out.y = omm * po_b.y + omega * gs;
+                    }
+                }
+                put2(dst1, (uint32_t)fb, out);
+            }
+            bo_b = bo_c;
+            bo_c = c.bo_a;
+            rn_b2 = rn_b1;
+            rn_b1 = rn_c;
+        };
+        // every plane but the last loads the next one
+        load_next(ks, A);
+        int t = 0;
+        for (; t + 2 < np; t += 2) {
+            plane(t, A, B, true);
+            plane(t + 1, B, A, true);
+        }
+        if (t + 1 < np) {
+            plane(t, A, B, true);
+            plane(t + 1, B, A, false);
+        } else {
+            plane(t, A, B, false);
+        }
+    };
+    if ((by & 1) == 0)
+        march(std::true_type{});
+    else
+        march(std::false_type{});
+}
+
+// Several red-black sweeps of a coarse multigrid level in one launch
+// (temporal blocking).  A workgroup owns an 8x8x8 cube of cells and loads it
+// into LDS with a halo of 2M cells on every side (M sweeps: each half-sweep
+// moves the dependency one cell), in natural (i, j, k) order, both colours
+// interleaved.  Half-sweep m (1..2M) updates the cells of its colour that lie
+// at most 2M-m cells outside the cube, so the values it reads were exact after
+// half-sweep m-1 and the cube is exact after the last one.  The cube's
+// interior cells (both colours: updated cells and, copied through, the cells
+// outside the solve) go to dst, never to src, which the neighbouring
+// workgroups are still reading; walls are never written (they stay as in
+// src).  Positions outside the arrays hold 0 and a NaN right-hand side, so
+// they are never updated and only wall cells read them.  FIRST: the first
+// red half-sweep takes every neighbour and the old value as 0, and the first
+// black half-sweep the old value, as k_sor does from zero.  Every update is
+// k_sor's, operand for operand: the result is bit for bit that of 2M
+// half-sweep launches.
+static constexpr int kTileCube = 8;
+static constexpr int kTileMaxSweeps = 2;
+static constexpr int kTileEdge = kTileCube + 4 * kTileMaxSweeps;   // 16
+static constexpr int kTileThreads = 1024;
+template <bool FIRST, int M>
+__global__ void __launch_bounds__(kTileThreads)
+k_smooth_tile(const float *__restrict__ src, float *__restrict__ dst, const float *__restrict__ b,
+              const float *__restrict__ axc, const float *__restrict__ ayc, const float *__restrict__ azc, SorGeom g,
+              float omega, float omm, const double *gate, double thr) {
+    __shared__ float xs[kTileEdge * kTileEdge * kTileEdge];
+    __shared__ float bs[kTileEdge * kTileEdge * kTileEdge];
+    __shared__ float cx[3][kTileEdge], cy[3][kTileEdge], cz[3][kTileEdge];
+    if (gated(gate, thr)) return;
+    static_assert(M >= 1 && M <= kTileMaxSweeps, "sweeps per launch");
+    constexpr int R = 2 * M, E = kTileCube + 2 * R, E2 = E * E, E3 = E2 * E;
+    // cube origin (first interior cell) minus the halo: tile position 0
+    const int i0 = 1 + kTileCube * (int)blockIdx.x - R;
+    const int j0 = 1 + kTileCube * (int)blockIdx.y - R;
+    const int k0 = 1 + kTileCube * (int)blockIdx.z - R;
+    const int tid = (int)threadIdx.x;
+    const float nan = __builtin_nanf("");
+    // coefficient rows of the tile's columns (clamped: positions outside
+    // the arrays are never updated, so their rows are never used)
+    const int ca = min(tid / E, 2), ct = tid - ca * E;
+    float cxv = 0.0f, cyv = 0.0f, czv = 0.0f;
+    if (tid < 3 * E) {
+        cxv = axc[ca * g.sxp + min(max(i0 + ct, 0), g.sx - 1)];
+        cyv = ayc[ca * g.sy + min(max(j0 + ct, 0), g.sy - 1)];
+        czv = azc[ca * g.sz + min(max(k0 + ct, 0), g.sz - 1)];
     }
-    for (int kk = k0 - 1; kk <= k1; kk++) {
-        const int f = s + kk * g.plane;
-        V2 bo_p = zero;
-        if (!FIRST && row_ok && kk + 1 <= g.sz - 1) bo_p = *reinterpret_cast<const V2 *>(src1 + f + g.plane);
-        // ---- red at plane kk
-        V2 rn_c = zero;
-        if (row_ok && kk >= 1 && kk <= g.sz - 2) {
-            const V2 r2 = *reinterpret_cast<const V2 *>(rhs0 + f);
-            const V2 po = FIRST ? zero : *reinterpret_cast<const V2 *>(src0 + f);
-            rn_c = po;
-            const bool ok0 = r2.x == r2.x, ok1 = r2.y == r2.y;
-            if (ok0 || ok1) {
-                const int q = (jj + kk) & 1;                  // colour 0
-                const float side = FIRST ? 0.0f : (q ? src1[f + 2] : src1[f - 1]);
-                const V2 ps = FIRST ? zero : *reinterpret_cast<const V2 *>(src1 + f - g.H);
-                const V2 pn = FIRST ? zero : *reinterpret_cast<const V2 *>(src1 + f + g.H);
-                const float a1z = azc[kk], a2z = azc[g.sz + kk], a3z = azc[2 * g.sz + kk];
-                const float a1x0 = q ? c1.y : c1.x, a2x0 = q ? c2.y : c2.x, a3x0 = q ? c3.y : c3.x;
-                const float a1x1 = q ? c1.w : c1.z, a2x1 = q ? c2.w : c2.z, a3x1 = q ? c3.w : c3.z;
-                if (ok0) {
-                    const float pw = q ? bo_c.x : side, pe = q ? bo_c.y : bo_c.x;
-                    const float gs = sor_div(((((((r2.x - a1x0 * pw) - a3x0 * pe) - a1y * ps.x) - a3y * pn.x) -
-                                               a1z * bo_m.x) - a3z * bo_p.x),
-                                             ((a2x0 + a2y) + a2z));
-                    rn_c.x = omm * po.x + omega * gs;
-                }
-                if (ok1) {
-                    const float pw = q ? bo_c.y : bo_c.x, pe = q ? side : bo_c.y;
-                    const float gs = sor_div(((((((r2.y - a1x1 * pw) - a3x1 * pe) - a1y * ps.y) - a3y * pn.y) -
-                                               a1z * bo_m.y) - a3z * bo_p.y),
-                                             ((a2x1 + a2y) + a2z));
-                    rn_c.y = omm * po.y + omega * gs;
-                }
-            }
-            if (own && kk >= k0 && kk < k1) *reinterpret_cast<V2 *>(dst0 + f) = rn_c;
+    // every thread issues all of its loads (at most kTileLoads cells each)
+    // before it stores any, so their latencies overlap; positions outside the
+    // arrays load a clamped (valid) cell and store 0 and NaN instead
+    constexpr int kTileLoads = (E3 + kTileThreads - 1) / kTileThreads;
+    float xv[kTileLoads], bv[kTileLoads];
+    bool in[kTileLoads];
+#pragma unroll
+    for (int q = 0; q < kTileLoads; q++) {
+        const int n = min(tid + q * kTileThreads, E3 - 1);
+        const int lk = n / E2, r = n - lk * E2, lj = r / E, li = r - lj * E;
+        const int ii = i0 + li, jj = j0 + lj, kk = k0 + lk;
+        in[q] = ii >= 0 && ii < g.sx && jj >= 0 && jj < g.sy && kk >= 0 && kk < g.sz;
+        const int ic = min(max(ii, 0), g.sx - 1), jc = min(max(jj, 0), g.sy - 1), kc = min(max(kk, 0), g.sz - 1);
+        const uint32_t f = (uint32_t)((kc * g.sy + jc) * g.H + (ic >> 1) + ((ic + jc + kc) & 1) * g.NC);
+        bv[q] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(b) + f * 4u);
+        xv[q] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(src) + f * 4u);
+    }
+#pragma unroll
+    for (int q = 0; q < kTileLoads; q++) {
+        const int n = tid + q * kTileThreads;
+        if (n < E3) {
+            const float bq = in[q] ? bv[q] : nan;
+            xs[n] = in[q] ? ((FIRST && bq == bq) ? 0.0f : xv[q]) : 0.0f;
+            bs[n] = bq;
         }
-        float *cur = ring[(kk + 3) % 3];
-        cur[lo] = rn_c.x;
-        cur[lo + 1] = rn_c.y;
+    }
+    if (tid < 3 * E) {
+        cx[ca][ct] = cxv;
+        cy[ca][ct] = cyv;
+        cz[ca][ct] = czv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 1; m <= 2 * M; m++) {
+        const int col = (m - 1) & 1;
+        const int W = E - 2 * m, HW = (W + 1) >> 1;  // region [m, E-m) per axis; cells of one colour per row <= HW
+#pragma unroll
+        for (int idx = tid; idx < HW * W * W; idx += kTileThreads) {
+            const int row = idx / HW, p = idx - row * HW;
+            const int lk = m + row / W, lj = m + (row - (row / W) * W);
+            const int li = m + (((col - (i0 + m + j0 + lj + k0 + lk)) & 1)) + 2 * p;
+            if (li >= E - m) continue;
+            const int n = li + E * (lj + E * lk);
+            const float r = bs[n];
+            if (!(r == r)) continue;
+            float pw, pe, ps, pn, pb, pt;
+            if (FIRST && m == 1) {
+                pw = pe = ps = pn = pb = pt = 0.0f;
+            } else {
+                pw = xs[n - 1]; pe = xs[n + 1];
+                ps = xs[n - E]; pn = xs[n + E];
+                pb = xs[n - E2]; pt = xs[n + E2];
+            }
+            const float pold = xs[n];
+            const float a1x = cx[0][li], a2x = cx[1][li], a3x = cx[2][li];
+            const float a1y = cy[0][lj], a2y = cy[1][lj], a3y = cy[2][lj];
+            const float a1z = cz[0][lk], a2z = cz[1][lk], a3z = cz[2][lk];
+            const float gs = sor_div(((((((r - a1x * pw) - a3x * pe) - a1y * ps) - a3y * pn) - a1z * pb) - a3z * pt),
+                                     ((a2x + a2y) + a2z));
+            xs[n] = omm * pold + omega * gs;
+        }
         __syncthreads();
-        // ---- black at plane kk-1
-        const int kb = kk - 1;
-        if (own && kb >= k0) {
-            const int fb = f - g.plane;
-            const float *prev = ring[(kk + 2) % 3];
-            const V2 r2 = *reinterpret_cast<const V2 *>(rhs1 + fb);
-            const V2 po = FIRST ? zero : bo_m;
-            V2 out = po;
-            const bool ok0 = r2.x == r2.x, ok1 = r2.y == r2.y;
-            if (ok0 || ok1) {
-                const int q = (1 + jj + kb) & 1;              // colour 1
-                const float side = q ? prev[lo + 2] : prev[lo - 1];
-                const V2 cr = V2{prev[lo], prev[lo + 1]};
-                const V2 ps = V2{prev[lo - g.H], prev[lo - g.H + 1]};
-                const V2 pn = V2{prev[lo + g.H], prev[lo + g.H + 1]};
-                const float a1z = azc[kb], a2z = azc[g.sz + kb], a3z = azc[2 * g.sz + kb];
-                const float a1x0 = q ? c1.y : c1.x, a2x0 = q ? c2.y : c2.x, a3x0 = q ? c3.y : c3.x;
-                const float a1x1 = q ? c1.w : c1.z, a2x1 = q ? c2.w : c2.z, a3x1 = q ? c3.w : c3.z;
-                if (ok0) {
-                    const float pw = q ? cr.x : side, pe = q ? cr.y : cr.x;
-                    const float gs = sor_div(((((((r2.x - a1x0 * pw) - a3x0 * pe) - a1y * ps.x) - a3y * pn.x) -
-                                               a1z * rn_m2.x) - a3z * rn_c.x),
-                                             ((a2x0 + a2y) + a2z));
-                    out.x = omm * po.x + omega * gs;
-                }
-                if (ok1) {
-                    const float pw = q ? cr.y : cr.x, pe = q ? side : cr.y;
-                    const float gs = sor_div(((((((r2.y - a1x1 * pw) - a3x1 * pe) - a1y * ps.y) - a3y * pn.y) -
-                                               a1z * rn_m2.y) - a3z * rn_c.y),
-                                             ((a2x1 + a2y) + a2z));
-                    out.y = omm * po.y + omega * gs;
-                }
-            }
-            *reinterpret_cast<V2 *>(dst1 + fb) = out;
-        }
-        bo_m = bo_c;
-        bo_c = bo_p;
-        rn_m2 = rn_m1;
-        rn_m1 = rn_c;
+    }
+    for (int n = tid; n < kTileCube * kTileCube * kTileCube; n += kTileThreads) {
+        const int ck = n / (kTileCube * kTileCube), r = n - ck * kTileCube * kTileCube, cj = r / kTileCube,
+                  ci = r - cj * kTileCube;
+        const int li = R + ci, lj = R + cj, lk = R + ck;
+        const int ii = i0 + li, jj = j0 + lj, kk = k0 + lk;
+        if (ii > g.sx - 2 || jj > g.sy - 2 || kk > g.sz - 2) continue;
+        const int f = (kk * g.sy + jj) * g.H + (ii >> 1) + ((ii + jj + kk) & 1) * g.NC;
+        dst[f] = xs[li + E * (lj + E * lk)];
     }
 }
 
@@ -1380,12 +1578,13 @@ struct MgView {
     const float *ax, *ay, *az;      // Laplacian rows
     const MgAxis *tx, *ty, *tz;     // prolongation from the next level
     int nx, ny, nz, H, NC, sy, sz, sxp;
+    FastDiv dH, dSy;                // division by H and sy (slot decoding)
 };
 
 __device__ __forceinline__ void view_decode(const MgView &L, int f, int c, int &ii, int &jj, int &kk) {
-    const int row = f / L.H;
+    const int row = (int)fdiv((uint32_t)f, L.dH);
     const int h = f - row * L.H;
-    kk = row / L.sy;
+    kk = (int)fdiv((uint32_t)row, L.dSy);
     jj = row - kk * L.sy;
     ii = 2 * h + ((c + jj + kk) & 1);
 }
@@ -1630,28 +1829,92 @@ __device__ void blk_prolong(const MgView &F, const MgView &C) {
     __syncthreads();
 }
 
+// The constant tables of a level as one run of 32-bit words: the Laplacian
+// rows (3 x sxp, 3 x sy, 3 x sz) and, with `tables`, the prolongation tables
+// (4 words per entry: nx+2, sy, sz entries).  Word q's global source and LDS
+// destination (false past the end).
+static constexpr int kMgGather = 8;
+__device__ __forceinline__ int mg_table_words(const MgView &G, bool tables) {
+    return 3 * (G.sxp + G.sy + G.sz) + (tables ? 4 * (G.nx + 2 + G.sy + G.sz) : 0);
+}
+__device__ __forceinline__ bool mg_table_word(const MgView &G, const MgView &V, bool tables, int q, const float *&src,
+                                              float *&dst) {
+    const int r = 3 * (G.sxp + G.sy + G.sz);
+    if (q < r) {
+        // the three row tables are contiguous in LDS (lds_view) and each is
+        // its own global array
+        const int ay = 3 * G.sxp, az = ay + 3 * G.sy;
+        src = q < ay ? G.ax + q : q < az ? G.ay + (q - ay) : G.az + (q - az);
+        dst = const_cast<float *>(V.ax) + q;
+        return true;
+    }
+    if (!tables) return false;
+    q -= r;
+    const int ty = 4 * (G.nx + 2), tz = ty + 4 * G.sy;
+    if (q >= tz + 4 * G.sz) return false;
+    src = q < ty ? reinterpret_cast<const float *>(G.tx) + q
+                 : q < tz ? reinterpret_cast<const float *>(G.ty) + (q - ty) : reinterpret_cast<const float *>(G.tz) + (q - tz);
+    dst = reinterpret_cast<float *>(const_cast<MgAxis *>(V.tx)) + q;
+    return true;
+}
+
 // V-cycle over levels S.L[0..n): x of level 0 from its b.  x and b of every
 // level live in dynamic LDS for the duration (2 x 2*NC floats per level);
 // S.L[0]'s global b is read at the start and its global x written at the end.
 __global__ void __launch_bounds__(kMgSmallThreads) k_mg_small(MgSmall S, const double *gate, double thr) {
     if (gated(gate, thr)) return;
     extern __shared__ float lds[];
+    // Everything the cycle reads from global memory (level 0's b, every
+    // level's Laplacian rows and prolongation tables) is gathered in passes
+    // in which each thread issues all of its loads before it stores any, so
+    // one pass costs one memory latency: x is zeroed (cells outside the
+    // solve hold 0) while the loads are in flight.
+    const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+    {
+        const MgView V = lds_view(S, 0, lds), G = S.L[0];
+        for (int q0 = 0; q0 < 2 * V.NC; q0 += kMgGather * nt) {
+            float v[kMgGather];
+#pragma unroll
+            for (int t = 0; t < kMgGather; t++) {
+                const int q = min(q0 + tid + t * nt, 2 * V.NC - 1);
+                v[t] = G.b[q];
+            }
+#pragma unroll
+            for (int t = 0; t < kMgGather; t++) {
+                const int q = q0 + tid + t * nt;
+                if (q < 2 * V.NC) V.b[q] = v[t];
+            }
+        }
+    }
     for (int l = 0; l < S.n; l++) {
-        const MgView V = lds_view(S, l, lds), G = S.L[l];
-        for (int q = threadIdx.x; q < 2 * V.NC; q += blockDim.x) {
-            V.x[q] = 0.0f;  // cells outside the solve hold 0
-            if (l == 0) V.b[q] = G.b[q];
+        const MgView V = lds_view(S, l, lds);
+        for (int q = tid; q < 2 * V.NC; q += nt) V.x[q] = 0.0f;
+    }
+    int words = 0;
+    for (int l = 0; l < S.n; l++) words += mg_table_words(S.L[l], l + 1 < S.n);
+    for (int w0 = 0; w0 < words; w0 += kMgGather * nt) {
+        float v[kMgGather];
+        const float *a[kMgGather];
+        float *d[kMgGather];
+        // word w of the levels' tables in order (clamped to the last): its
+        // source and destination first, then all the loads, then the stores
+#pragma unroll
+        for (int t = 0; t < kMgGather; t++) {
+            const int w = min(w0 + tid + t * nt, words - 1);
+            a[t] = S.L[0].ax;
+            d[t] = nullptr;
+            for (int l = 0, base = 0; l < S.n; l++) {
+                const int n = mg_table_words(S.L[l], l + 1 < S.n);
+                if (w >= base && w < base + n)
+                    mg_table_word(S.L[l], lds_view(S, l, lds), l + 1 < S.n, w - base, a[t], d[t]);
+                base += n;
+            }
         }
-        float *ax = const_cast<float *>(V.ax), *ay = const_cast<float *>(V.ay), *az = const_cast<float *>(V.az);
-        for (int q = threadIdx.x; q < 3 * V.sxp; q += blockDim.x) ax[q] = G.ax[q];
-        for (int q = threadIdx.x; q < 3 * V.sy; q += blockDim.x) ay[q] = G.ay[q];
-        for (int q = threadIdx.x; q < 3 * V.sz; q += blockDim.x) az[q] = G.az[q];
-        if (l + 1 < S.n) {
-            MgAxis *tx = const_cast<MgAxis *>(V.tx), *ty = const_cast<MgAxis *>(V.ty), *tz = const_cast<MgAxis *>(V.tz);
-            for (int q = threadIdx.x; q < V.nx + 2; q += blockDim.x) tx[q] = G.tx[q];
-            for (int q = threadIdx.x; q < V.sy; q += blockDim.x) ty[q] = G.ty[q];
-            for (int q = threadIdx.x; q < V.sz; q += blockDim.x) tz[q] = G.tz[q];
-        }
+#pragma unroll
+        for (int t = 0; t < kMgGather; t++) v[t] = *a[t];
+#pragma unroll
+        for (int t = 0; t < kMgGather; t++)
+            if (w0 + tid + t * nt < words) *d[t] = v[t];
     }
     __syncthreads();
     for (int l = 0; l + 1 < S.n; l++) {
@@ -1915,7 +2178,8 @@ static dim3 level_grid(const MgLevel &L) {
 // Launch shape of the fused sweep: S own slots per workgroup (the red range
 // adds H on either side and must fit 2 x kSweepThreads slots), planes per
 // chunk so that the grid holds about g_sweep_blocks workgroups.
-static int g_fused = 0;          // levels (from the finest) smoothed by whole sweeps in one launch
+static int g_fused = 1;          // levels (from the finest) smoothed by whole sweeps in one launch
+static int g_coarse_tiles = 2;   // first level smoothed by two temporally blocked launches per phase (0: none)
 static int g_sweep_tz = 0;        // planes per chunk (0: from g_sweep_blocks)
 static int g_sweep_blocks = 480;
 
@@ -1939,6 +2203,30 @@ static void mg_smooth(Ctx &c, const MgLevel &L, bool finest, int n, bool first, 
     const dim3 cg(nblocks(L.NC, kSorBlock));
     int S = 0, tz = 0;
     dim3 sg;
+    if (!finest && g_coarse_tiles > 0 && L.level >= g_coarse_tiles && L.x2 && n >= 2 &&
+        (n + 1) / 2 <= kTileMaxSweeps) {
+        // two launches of ceil(n/2) and floor(n/2) sweeps: x -> x2 -> x
+        const dim3 tg((unsigned)((L.sx - 2 + kTileCube - 1) / kTileCube),
+                      (unsigned)((L.sy - 2 + kTileCube - 1) / kTileCube),
+                      (unsigned)((L.sz - 2 + kTileCube - 1) / kTileCube));
+        auto launch = [&](int m, bool fz, const float *from, float *to) {
+            if (m == 2 && fz)
+                hipLaunchKernelGGL((k_smooth_tile<true, 2>), tg, dim3(kTileThreads), 0, c.stream, from, to, L.b, L.ax,
+                                   L.ay, L.az, g, om, omm, gate, 0.5);
+            else if (m == 2)
+                hipLaunchKernelGGL((k_smooth_tile<false, 2>), tg, dim3(kTileThreads), 0, c.stream, from, to, L.b, L.ax,
+                                   L.ay, L.az, g, om, omm, gate, 0.5);
+            else if (fz)
+                hipLaunchKernelGGL((k_smooth_tile<true, 1>), tg, dim3(kTileThreads), 0, c.stream, from, to, L.b, L.ax,
+                                   L.ay, L.az, g, om, omm, gate, 0.5);
+            else
+                hipLaunchKernelGGL((k_smooth_tile<false, 1>), tg, dim3(kTileThreads), 0, c.stream, from, to, L.b, L.ax,
+                                   L.ay, L.az, g, om, omm, gate, 0.5);
+        };
+        launch((n + 1) / 2, first, L.x, L.x2);
+        launch(n / 2, false, L.x2, L.x);
+        return;
+    }
     if (L.level < g_fused && L.x2 && n >= 2 && sweep2_shape(L, S, tz, sg)) {
         // an odd count does its first sweep in place, so that the pairs of
         // fused sweeps end in x
@@ -1988,6 +2276,8 @@ static MgView level_view(const MgLevel &L) {
     V.tx = L.tx; V.ty = L.ty; V.tz = L.tz;
     V.nx = (int)L.nx; V.ny = (int)L.ny; V.nz = (int)L.nz;
     V.H = (int)L.H; V.NC = (int)L.NC; V.sy = (int)L.sy; V.sz = (int)L.sz; V.sxp = (int)L.sxp;
+    V.dH = make_fastdiv(V.H);
+    V.dSy = make_fastdiv(V.sy);
     return V;
 }
 
@@ -2213,6 +2503,7 @@ static int cycle_graph(Ctx &c, int m, int k, double *relv, double *conv) {
     key.push_back((uint64_t)c.mg_nu1c);
     key.push_back((uint64_t)c.mg_nu2c);
     key.push_back((uint64_t)g_fused);
+    key.push_back((uint64_t)g_coarse_tiles);
     key.push_back((uint64_t)g_sweep_tz);
     key.push_back((uint64_t)g_sweep_blocks);
     key.push_back((uint64_t)c.mg_coarse);
@@ -2350,6 +2641,12 @@ int set_fused_sweeps(int on, int tz, int blocks) {
     g_fused = on;
     g_sweep_tz = tz;
     g_sweep_blocks = blocks;
+    return 0;
+}
+
+int set_coarse_tiles(int from_level) {
+    IB3D_REQUIRE(from_level >= 0 && from_level <= 64, "set_coarse_tiles: bad level");
+    g_coarse_tiles = from_level;
     return 0;
 }
 

@@ -69,6 +69,7 @@ def lib():
         "ib3d_set_multigrid_coarse": [I, I],
         "ib3d_set_krylov_target": [D],
         "ib3d_set_fused_sweeps": [I, I, I],
+        "ib3d_set_coarse_tiles": [I],
         "ib3d_set_momentum_tiles": [I],
         "ib3d_set_graphs": [I],
         "ib3d_poisson_stats": [PI64, PI64, PD, PD, PD, PD],
@@ -291,6 +292,14 @@ class Solver:
         per red-black sweep (0: two half-sweep launches everywhere);
         identical results.  True/False mean 1/0."""
         _ok(self.L.ib3d_set_fused_sweeps(int(levels), int(planes), int(blocks)))
+
+    def set_coarse_tiles(self, from_level=2):
+        """Multigrid levels from `from_level` on (0: none) smoothed by two
+        temporally blocked launches per phase instead of half-sweep
+        launches; identical results.  True/False mean 2/0."""
+        if from_level is True:
+            from_level = 2
+        _ok(self.L.ib3d_set_coarse_tiles(int(from_level)))
 
     def set_momentum_tiles(self, on=True):
         """nsMomentum's interior from LDS tiles (on) or straight from HBM (off);

@@ -23,6 +23,10 @@ GMRES_SOR = {"m": 12, "k": 20, "omega": 1.9}
 # post-smoothing sweeps per level and their omega, sweeps and omega on the
 # coarsest level
 MULTIGRID = {"m": 6, "nu1": 4, "nu2": 4, "omega": 1.55, "coarse": 4, "coarse_omega": 1.5}
+# how the smoothing is launched (identical results either way): whole sweeps
+# in one launch on the first `fused` levels, two temporally blocked launches
+# per smoothing phase from level `tiles` on
+LAUNCH = {"fused": 1, "tiles": 2}
 
 MODES = ("reference", "fp64-cold", "fp64", "mixed-dc", "gmres-sor", "mixed")
 
@@ -46,5 +50,7 @@ def apply(solver, mode, omega=1.99, tol=1e-6, inner=20, gmres=None, multigrid=No
         solver.set_poisson_mode(4, tol=tol, warm=True)
         solver.set_krylov(mg["m"])
         solver.set_multigrid(mg["nu1"], mg["nu2"], mg["omega"], mg["coarse"], mg["coarse_omega"], 0)
+        solver.set_fused_sweeps(LAUNCH["fused"])
+        solver.set_coarse_tiles(LAUNCH["tiles"])
     else:
         raise ValueError(f"unknown Poisson preset {mode!r}; known: {', '.join(MODES)}")

@@ -179,12 +179,18 @@ int ib3d_set_krylov_target(double target);
  * Those levels are small enough that a sweep costs about a launch, whatever
  * the level's size. */
 int ib3d_set_multigrid_coarse(int nu1, int nu2);
-/* mode 4: smooth the first `levels` multigrid levels (1: the finest only)
- * with whole red-black sweeps in one launch each, the others with two
- * half-sweep launches (levels = 0, the default).  Both give identical
- * results.  planes: z planes per workgroup (0: chosen so that the grid holds
+/* mode 4: smooth the first `levels` multigrid levels (1, the default: the
+ * finest only) with whole red-black sweeps in one launch each, the others
+ * with two half-sweep launches (0: none).  Both give identical results.  planes: z planes per workgroup (0: chosen so that the grid holds
  * about `blocks` workgroups; default 0 and 480). */
 int ib3d_set_fused_sweeps(int levels, int planes, int blocks);
+/* mode 4: smooth the multigrid levels from `from_level` on (2, the
+ * default; 0 for none) that are cycled by launches with two temporally
+ * blocked launches per smoothing phase: each workgroup sweeps an 8x8x8 cube
+ * plus a halo in LDS, ceil(n/2) sweeps in the first launch and floor(n/2) in
+ * the second (n = 2..4), instead of 2n half-sweep launches.  Both give
+ * identical results. */
+int ib3d_set_coarse_tiles(int from_level);
 /* nsMomentum: the interior columns from 64 x 8 LDS tiles marching in z
  * (on != 0) or with every stencil value read from HBM (on == 0, the default).
  * Both give identical results. */

@@ -22,6 +22,7 @@ def _run(case, fused, nsteps, nu=(4, 4), planes=0):
         presets.apply(s, "mixed")
         mg = presets.MULTIGRID
         s.set_multigrid(nu[0], nu[1], mg["omega"], mg["coarse"], mg["coarse_omega"], 0)
+        s.set_coarse_tiles(0)  # the fused sweep on every level it is asked for
         s.set_fused_sweeps(fused, planes=planes)
         stats = []
         for _ in range(nsteps):
@@ -30,7 +31,8 @@ def _run(case, fused, nsteps, nu=(4, 4), planes=0):
             stats.append((st["sweeps"], st["outer"], st["relres"]))
         return stats, {f: s.download(f) for f in ("u", "v", "w", "p", "pc")}
     finally:
-        s.set_fused_sweeps(0)
+        s.set_fused_sweeps()
+        s.set_coarse_tiles()
         s.close()
 
 

@@ -22,7 +22,7 @@ int ns_momentum(Ctx &c, double rev, double deltat);
 int solid_bc(Ctx &c);
 int correct(Ctx &c, double dt, int do_p, int do_v);
 int steady_copy(Ctx &c, double dt, double *derr_std);
-int sor_solve(Ctx &c, double bnorm2, double eps1);
+int sor_solve(Ctx &c, double bnorm2, double eps1, const double *dnorm = nullptr);
 int residual_norm(Ctx &c, double *rnorm, double *bnorm);
 int bench_sor(Ctx &c, int precision, int sweeps, double omega, double *ms);
 int probe_copy(Ctx &c, int64_t nbytes, double *ms);
@@ -863,33 +863,60 @@ int ib3d_correctVelocity(double deltat) {
     return correct(c, deltat, 0, 1);
 }
 
+// The Poisson stage (subPcorVcor.f90): divergence, solve, correction, wall
+// conditions and the steady-state measure.  deferred: the steady-state
+// measure and the stage times are left to arrive with the caller's next
+// synchronisation (poisson_collect reads them), so the whole step waits on
+// the host fewer times.
+static int poisson_stage(Ctx &c, double deltat, double eps1, int64_t *nIterPcor, double *divmax,
+                         double *derrStdSt, double *sor_seconds, bool deferred);
+static void poisson_collect(Ctx &c, double *divmax, double *derrStdSt, double *sor_seconds) {
+    if (divmax) {
+        *divmax = c.hscal[0];
+        c.bnorm2 = c.hscal[1];
+    }
+    float t_sor = 0, t_all = 0;
+    (void)hipEventElapsedTime(&t_sor, c.ev[1], c.ev[2]);
+    (void)hipEventElapsedTime(&t_all, c.ev[0], c.ev[3]);
+    c.last_sor_ms = t_sor;
+    c.last_poisson_ms = t_all;
+    if (sor_seconds) *sor_seconds = t_sor * 1e-3;
+    if (derrStdSt) *derrStdSt = c.hscal[3];
+}
+
 int ib3d_poissonSolver(double deltat, double eps1, int64_t *nIterPcor, double *divmax, double *derrStdSt,
                        double *sor_seconds) {
     Ctx &c = ctx();
     IB3D_REQUIRE(c.ready && c.matrix_ready && c.lists_ready, "poissonSolver: setup incomplete");
+    return poisson_stage(c, deltat, eps1, nIterPcor, divmax, derrStdSt, sor_seconds, false);
+}
+
+static int poisson_stage(Ctx &c, double deltat, double eps1, int64_t *nIterPcor, double *divmax,
+                         double *derrStdSt, double *sor_seconds, bool deferred) {
     IB3D_CHECK(hipEventRecord(c.ev[0], c.stream));
     if (c.mode == 0 || !c.warm || !c.have_prev) {
         IB3D_CHECK(hipMemsetAsync(c.pc[0], 0, c.NC * sizeof(double), c.stream));
         IB3D_CHECK(hipMemsetAsync(c.pc[1], 0, c.NC * sizeof(double), c.stream));
     }
     double bnorm2 = 0.0;
-    if (compute_div(c, deltat, divmax, &bnorm2)) return -1;
+    // the Krylov modes can take ||b/dt|| from device memory: then nothing
+    // waits for the host between the divergence and the solve
+    const bool dev_norm = deferred && (c.mode == 3 || c.mode == 4) && !c.use_graphs && !cnn_active();
+    double *dnorm = c.scalars + kNormSlot;
+    if (compute_div(c, deltat, divmax, &bnorm2, !dev_norm)) return -1;
+    if (dev_norm && norm_target_async(c, c.scalars + 1, dnorm)) return -1;
     IB3D_CHECK(hipEventRecord(c.ev[1], c.stream));
-    if (sor_solve(c, bnorm2, eps1)) return -1;
+    if (sor_solve(c, bnorm2, eps1, dev_norm ? dnorm : nullptr)) return -1;
     IB3D_CHECK(hipEventRecord(c.ev[2], c.stream));
     if (correct(c, deltat, 1, 1)) return -1;
     if (velocity_bc(c)) return -1;
-    if (steady_copy(c, deltat, derrStdSt)) return -1;
+    if (steady_copy(c, deltat, nullptr)) return -1;
     IB3D_CHECK(hipEventRecord(c.ev[3], c.stream));
-    IB3D_CHECK(hipEventSynchronize(c.ev[3]));
-    float t_sor = 0, t_all = 0;
-    IB3D_CHECK(hipEventElapsedTime(&t_sor, c.ev[1], c.ev[2]));
-    IB3D_CHECK(hipEventElapsedTime(&t_all, c.ev[0], c.ev[3]));
-    c.last_sor_ms = t_sor;
-    c.last_poisson_ms = t_all;
     *nIterPcor = c.last_sweeps;
-    if (sor_seconds) *sor_seconds = t_sor * 1e-3;
     c.have_prev = true;
+    if (deferred) return 0;
+    IB3D_CHECK(hipEventSynchronize(c.ev[3]));
+    poisson_collect(c, nullptr, derrStdSt, sor_seconds);
     return 0;
 }
 
@@ -960,7 +987,9 @@ int ib3d_set_multigrid_coarse(int nu1, int nu2) {
     return 0;
 }
 
-int ib3d_set_fused_sweeps(int levels, int planes, int blocks) { return set_fused_sweeps(levels, planes, blocks); }
+int ib3d_set_fused_sweeps(int levels, int planes, int blocks, int coarse_blocks) {
+    return set_fused_sweeps(levels, planes, blocks, coarse_blocks);
+}
 int ib3d_set_coarse_tiles(int from_level) { return set_coarse_tiles(from_level); }
 
 int ib3d_set_momentum_tiles(int on) { return set_momentum_tiles(on); }
@@ -995,10 +1024,12 @@ int ib3d_step(int first, double rev, double re, double deltat, double eps1, doub
         return -1;
     int64_t it = 0;
     double dm = 0, ds = 0, sor_s = 0;
-    if (ib3d_poissonSolver(deltat, eps1, &it, &dm, &ds, &sor_s)) return -1;
+    IB3D_REQUIRE(c.matrix_ready && c.lists_ready, "step: setup incomplete");
+    if (poisson_stage(c, deltat, eps1, &it, &dm, &ds, &sor_s, true)) return -1;
     if (pressure_forcing(c)) return -1;
     double coef[4];
-    if (stress(c, first, re, coef)) return -1;
+    if (stress(c, first, re, coef)) return -1;   // waits for the stream
+    poisson_collect(c, &dm, &ds, &sor_s);
     out[0] = (double)it;
     out[1] = dm;
     out[2] = ds;

@@ -113,7 +113,7 @@ struct Ctx {
     double *partial = nullptr;     // per-block partial sums
     int64_t n_partial = 0;
     double bnorm2 = 0.0;           // ||b/dt||^2 of the last computeDiv
-    double *scalars = nullptr;     // small device scalars
+    double *scalars = nullptr;     // small device scalars (64)
     double *hscal = nullptr;       // pinned host mirror
 
     // surface mesh
@@ -255,10 +255,14 @@ int ensure_partial(int64_t n);
 // modules
 int fields_init(Ctx &c);
 int poisson_solve(Ctx &c, double deltat, double eps1, int64_t *niter, double *divmax, double *derr_std);
-int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2);
+// device scalar slots 40, 41: ||b/dt|| and the Krylov target of a solve
+// queued before the host has read the norm
+static constexpr int kNormSlot = 40;
+int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2, bool wait = true);
+int norm_target_async(Ctx &c, const double *bnorm2, double *out);
 int compute_div_noalloc(Ctx &c, double deltat, double *divmax);
 int noalloc_setup(int device);
-int set_fused_sweeps(int on, int tz, int blocks);
+int set_fused_sweeps(int on, int tz, int blocks, int coarse_blocks);
 int set_coarse_tiles(int from_level);
 int set_momentum_tiles(int on);
 int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);

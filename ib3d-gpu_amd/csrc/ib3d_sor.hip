@@ -1326,7 +1326,10 @@ int compute_div_noalloc(Ctx &c, double deltat, double *divmax) {
     return 0;
 }
 
-int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2) {
+// wait = false: max|div| and ||b/dt||^2 are left to arrive in c.hscal[0..1]
+// at the caller's next synchronisation (the solve then reads the norm from
+// device memory)
+int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2, bool wait) {
     const unsigned nb = nblocks(c.P, kSorBlock);
     if (ensure_partial(std::max<int64_t>(2 * (int64_t)nb, nblocks(2 * c.NC, kSorBlock)))) return -1;
     IB3D_CHECK(hipMemsetAsync(c.scalars, 0, 8 * sizeof(double), c.stream));
@@ -1337,10 +1340,26 @@ int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2) {
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, c.stream, c.partial, (int64_t)nb, c.scalars + 1);
     IB3D_CHECK(hipGetLastError());
     IB3D_CHECK(hipMemcpyAsync(c.hscal, c.scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (!wait) return 0;
     IB3D_CHECK(hipStreamSynchronize(c.stream));
     *divmax = c.hscal[0];
     if (bnorm2) *bnorm2 = c.hscal[1];
     c.bnorm2 = c.hscal[1];
+    return 0;
+}
+
+// ||b/dt|| and the Krylov target kt_tol x ||b/dt|| from ||b/dt||^2 on the
+// device, for a solve queued before the host has read the norm
+__global__ void k_norm_target(const double *bnorm2, double *out, double kt_tol) {
+    if (threadIdx.x != 0) return;
+    const double bn = sqrt(*bnorm2);
+    out[0] = bn;
+    out[1] = kt_tol * bn;
+}
+
+int norm_target_async(Ctx &c, const double *bnorm2, double *out) {
+    hipLaunchKernelGGL(k_norm_target, dim3(1), dim3(64), 0, c.stream, bnorm2, out, c.krylov_target * c.tol);
+    IB3D_CHECK(hipGetLastError());
     return 0;
 }
 
@@ -2178,10 +2197,11 @@ static dim3 level_grid(const MgLevel &L) {
 // Launch shape of the fused sweep: S own slots per workgroup (the red range
 // adds H on either side and must fit 2 x kSweepThreads slots), planes per
 // chunk so that the grid holds about g_sweep_blocks workgroups.
-static int g_fused = 1;          // levels (from the finest) smoothed by whole sweeps in one launch
+static int g_fused = 2;          // levels (from the finest) smoothed by whole sweeps in one launch
 static int g_coarse_tiles = 2;   // first level smoothed by two temporally blocked launches per phase (0: none)
-static int g_sweep_tz = 0;        // planes per chunk (0: from g_sweep_blocks)
+static int g_sweep_tz = 0;        // planes per chunk on the finest level (0: from g_sweep_blocks)
 static int g_sweep_blocks = 480;
+static int g_sweep_blocks_coarse = 192;  // workgroup target below the finest level
 
 static bool sweep2_shape(const MgLevel &L, int &S, int &tz, dim3 &grid) {
     const int64_t plane = L.H * L.sy, smax = 2 * kSweepThreads - 2 * L.H;
@@ -2189,8 +2209,9 @@ static bool sweep2_shape(const MgLevel &L, int &S, int &tz, dim3 &grid) {
     const int64_t nbp = (plane + smax - 1) / smax;
     S = (int)(((plane + nbp - 1) / nbp + 1) & ~int64_t(1));
     const int64_t nz = L.sz - 2;
-    int64_t nch = g_sweep_tz > 0 ? (nz + g_sweep_tz - 1) / g_sweep_tz
-                                 : std::min<int64_t>(nz, std::max<int64_t>(1, (g_sweep_blocks + nbp - 1) / nbp));
+    const int64_t target = L.level == 0 ? g_sweep_blocks : g_sweep_blocks_coarse;
+    int64_t nch = (g_sweep_tz > 0 && L.level == 0) ? (nz + g_sweep_tz - 1) / g_sweep_tz
+                                                   : std::min<int64_t>(nz, std::max<int64_t>(1, (target + nbp - 1) / nbp));
     tz = (int)((nz + nch - 1) / nch);
     nch = (nz + tz - 1) / tz;
     grid = dim3((unsigned)nbp, (unsigned)nch);
@@ -2506,6 +2527,7 @@ static int cycle_graph(Ctx &c, int m, int k, double *relv, double *conv) {
     key.push_back((uint64_t)g_coarse_tiles);
     key.push_back((uint64_t)g_sweep_tz);
     key.push_back((uint64_t)g_sweep_blocks);
+    key.push_back((uint64_t)g_sweep_blocks_coarse);
     key.push_back((uint64_t)c.mg_coarse);
     key.push_back((uint64_t)g_mg_small);
     key.push_back((uint64_t)g_mg_lds);
@@ -2531,7 +2553,13 @@ static int cycle_graph(Ctx &c, int m, int k, double *relv, double *conv) {
     return 0;
 }
 
-static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double *rel, double *rel0) {
+// batch results that fit the pinned host mirror (c.hscal[16 .. 16 + 2 x 14 + 2])
+static constexpr int64_t kBatchMirror = 14;
+
+// dnorm (device, or null): ||b/dt|| and the Krylov target, when the host has
+// not read the norm (bn is then unused)
+static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double *rel, double *rel0,
+                     const double *dnorm) {
     const int m = c.krylov_m, k = c.mode == 4 ? 1 : c.check;
     if (sor_float_tables(c)) return -1;
     if (gmres_buffers(c, m)) return -1;
@@ -2551,14 +2579,14 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
         IB3D_CHECK(hipGetLastError());
         if (cnn_before_solve(c, c.scalars + 16, c.scalars + 18, resid_rel)) return -1;
     }
-    residual_async(c, false, true, bn, relv, conv, nullptr);
+    residual_async(c, false, true, bn, relv, conv, nullptr, dnorm);
     int64_t o = 0, seen = 0;
     // A cycle is hundreds of launches; one queued after convergence still
     // costs their launch overhead.  Queue one cycle less than the previous
     // solve needed, then one at a time (a host check costs far less than a
     // cycle).
     int batch = (int)std::min<int64_t>(16, std::max<int64_t>(1, c.last_outer - 1));
-    std::vector<double> hr, hc;
+    std::vector<double> hr;
     if (c.use_graphs) {
         if (cycle_graph(c, m, k, relv, conv)) return -1;
         hipLaunchKernelGGL(k_cycle_init, dim3(1), dim3(64), 0, c.stream, c.scalars + kCycSlot, relv, conv, bn,
@@ -2572,30 +2600,45 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
                 continue;
             }
             const double *gate = conv + (o - 1);
-            gmres_cycle(c, m, k, gate, c.krylov_target * c.tol * bn);
-            residual_async(c, false, true, bn, relv + o, conv + o, gate);
+            gmres_cycle(c, m, k, gate, c.krylov_target * c.tol * bn, dnorm ? dnorm + 1 : nullptr);
+            residual_async(c, false, true, bn, relv + o, conv + o, gate, dnorm);
         }
         IB3D_CHECK(hipGetLastError());
+        // the batch's residuals and flags and the Arnoldi step count, in one
+        // host round trip (into the pinned mirror when they fit)
         const int64_t n = o - seen + 1;
-        hr.resize(n);
-        hc.resize(n);
-        IB3D_CHECK(hipMemcpyAsync(hr.data(), relv + seen, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-        IB3D_CHECK(hipMemcpyAsync(hc.data(), conv + seen, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        double *pr, *pc, *ps;
+        if (n <= kBatchMirror) {
+            pr = c.hscal + 16;
+            pc = pr + kBatchMirror;
+            ps = pc + kBatchMirror;
+        } else {
+            hr.resize(2 * n + 2);
+            pr = hr.data();
+            pc = pr + n;
+            ps = pc + n;
+        }
+        IB3D_CHECK(hipMemcpyAsync(pr, relv + seen, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        IB3D_CHECK(hipMemcpyAsync(pc, conv + seen, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        IB3D_CHECK(hipMemcpyAsync(ps, c.scalars + 24, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        if (dnorm) IB3D_CHECK(hipMemcpyAsync(ps + 1, dnorm, sizeof(double), hipMemcpyDeviceToHost, c.stream));
         IB3D_CHECK(hipStreamSynchronize(c.stream));
-        if (seen == 0) *rel0 = hr[0];
+        if (dnorm) bn = ps[1];
+        if (seen == 0) *rel0 = pr[0];
         for (int64_t q = 0; q < n; q++) {
-            if (!(hr[q] == hr[q]) || !std::isfinite(bn)) {
+            if (!(pr[q] == pr[q]) || !std::isfinite(bn)) {
                 set_error("poissonSolver: the residual is not finite");
                 return -1;
             }
-            if (hc[q] < 0.5 || seen + q >= max_outer) {
+            if (pc[q] < 0.5 || seen + q >= max_outer) {
                 const int64_t oc = seen + q;
-                if (cnn && cnn_after_solve(c, c.scalars + 16, c.scalars + 18)) return -1;
-                double steps = 0.0;
-                IB3D_CHECK(d2h(&steps, c.scalars + 24, sizeof(double)));
+                // the steps counted so far are all of the solve's: the
+                // cycles after convergence were gated off
+                const double steps = *ps;
                 *sweeps = (int64_t)steps * k;
                 *outer = oc;
-                *rel = hr[q];
+                *rel = pr[q];
+                if (cnn && cnn_after_solve(c, c.scalars + 16, c.scalars + 18)) return -1;
                 return 0;
             }
         }
@@ -2604,8 +2647,8 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
     }
 }
 
-int sor_solve(Ctx &c, double bnorm2, double eps1) {
-    const double bn = std::sqrt(bnorm2);
+int sor_solve(Ctx &c, double bnorm2, double eps1, const double *dnorm) {
+    const double bn = dnorm ? 0.0 : std::sqrt(bnorm2);
     if (c.mode == 0) {
         int64_t isum;
         double d2;
@@ -2618,7 +2661,7 @@ int sor_solve(Ctx &c, double bnorm2, double eps1) {
     int64_t sw = 0, out = 0;
     double rel = 0, rel0 = 0;
     if (c.mode == 3 || c.mode == 4) {
-        if (sor_gmres(c, bn, &sw, &out, &rel, &rel0)) return -1;
+        if (sor_gmres(c, bn, &sw, &out, &rel, &rel0, dnorm)) return -1;
     } else if (sor_l2(c, c.mode == 2, bn, &sw, &out, &rel, &rel0)) {
         return -1;
     }
@@ -2635,12 +2678,14 @@ int sor_solve(Ctx &c, double bnorm2, double eps1) {
     return 0;
 }
 
-int set_fused_sweeps(int on, int tz, int blocks) {
-    IB3D_REQUIRE(on >= 0 && on <= 64 && tz >= 0 && tz <= 4096 && blocks >= 1 && blocks <= 1 << 20,
+int set_fused_sweeps(int on, int tz, int blocks, int coarse_blocks) {
+    IB3D_REQUIRE(on >= 0 && on <= 64 && tz >= 0 && tz <= 4096 && blocks >= 1 && blocks <= 1 << 20 &&
+                     coarse_blocks >= 1 && coarse_blocks <= 1 << 20,
                  "set_fused_sweeps: bad level count or launch shape");
     g_fused = on;
     g_sweep_tz = tz;
     g_sweep_blocks = blocks;
+    g_sweep_blocks_coarse = coarse_blocks;
     return 0;
 }
 
@@ -2703,25 +2748,42 @@ int bench_sor(Ctx &c, int precision, int sweeps, double omega, double *ms) {
     }
     // 33: the fused FP32 sweep (one launch per red-black sweep) between e32
     // and a second buffer, as on the finest multigrid level
+    // After a mode-4 solve it sweeps the solver's own buffers (the first
+    // Krylov vector z from the first basis vector v, and the multigrid's
+    // second buffer), so the probe reads and writes the memory the solver's
+    // sweeps do; the solver overwrites z and v at its next solve and keeps 0
+    // in their cells outside the solve (the sweep only copies those).
     MgLevel L;
-    float *x2 = nullptr;
+    float *x2 = nullptr, *x0 = c.e32[0];
+    const float *rhs0 = c.r32[0], *rhs1 = c.r32[1];
+    bool own_x2 = false;
     int S = 0, tz = 0;
     dim3 sg;
     if (precision == 33) {
         if (sor_float_tables(c)) return -1;
         L.sx = c.sx; L.sy = c.sy; L.sz = c.sz; L.H = c.H; L.NC = c.NC; L.sxp = c.sxp;
         IB3D_REQUIRE(sweep2_shape(L, S, tz, sg), "bench_sor: the fused sweep does not fit this grid");
-        IB3D_CHECK(hipMalloc(&x2, 2 * c.NC * sizeof(float)));
-        IB3D_CHECK(dzero(x2, 2 * c.NC * sizeof(float)));
+        if (!g_mg.empty() && g_mg[0].x2 && g_mg[0].NC == c.NC) {
+            x2 = g_mg[0].x2;
+            if (g_kv && g_kz && g_k_nc == c.NC) {
+                x0 = g_kz;
+                rhs0 = g_kv;
+                rhs1 = g_kv + c.NC;
+            }
+        } else {
+            IB3D_CHECK(hipMalloc(&x2, 2 * c.NC * sizeof(float)));
+            IB3D_CHECK(dzero(x2, 2 * c.NC * sizeof(float)));
+            own_x2 = true;
+        }
     }
     const SorGeom g = geom(c);
     const float om = (float)omega, omm = (float)(1.0 - omega);
-    float *xa = c.e32[0], *xb = x2;
+    float *xa = x0, *xb = x2;
     IB3D_CHECK(hipEventRecord(a, c.stream));
     for (int q = 0; q < sweeps; q++) {
         if (precision == 33) {
             hipLaunchKernelGGL(k_sweep2_pz<false>, sg, dim3(kSweepThreads), 0, c.stream, xa, xa + c.NC, xb,
-                               xb + c.NC, c.r32[0], c.r32[1], g_axf, g_ayf, g_azf, g, S, tz, om, omm, nullptr, 0.0);
+                               xb + c.NC, rhs0, rhs1, g_axf, g_ayf, g_azf, g, S, tz, om, omm, nullptr, 0.0);
             std::swap(xa, xb);
         } else if (precision == 32) {
             sweep32(c, omega, false, false, nullptr, 0.0);
@@ -2737,10 +2799,10 @@ int bench_sor(Ctx &c, int precision, int sweeps, double omega, double *ms) {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (x2) {
-        if (xa == x2) IB3D_CHECK(hipMemcpyAsync(c.e32[0], x2, 2 * c.NC * sizeof(float), hipMemcpyDeviceToDevice,
+        if (xa == x2) IB3D_CHECK(hipMemcpyAsync(x0, x2, 2 * c.NC * sizeof(float), hipMemcpyDeviceToDevice,
                                                 c.stream));
         IB3D_CHECK(hipStreamSynchronize(c.stream));
-        IB3D_CHECK(hipFree(x2));
+        if (own_x2) IB3D_CHECK(hipFree(x2));
     }
     return 0;
 }

@@ -68,7 +68,7 @@ def lib():
         "ib3d_set_multigrid": [I, I, D, I, D, I],
         "ib3d_set_multigrid_coarse": [I, I],
         "ib3d_set_krylov_target": [D],
-        "ib3d_set_fused_sweeps": [I, I, I],
+        "ib3d_set_fused_sweeps": [I, I, I, I],
         "ib3d_set_coarse_tiles": [I],
         "ib3d_set_momentum_tiles": [I],
         "ib3d_set_graphs": [I],
@@ -287,11 +287,11 @@ class Solver:
         _ok(self.L.ib3d_set_multigrid(int(nu1), int(nu2), float(omega), int(coarse_sweeps), float(coarse_omega),
                                       int(max_levels)))
 
-    def set_fused_sweeps(self, levels=1, planes=0, blocks=480):
+    def set_fused_sweeps(self, levels=2, planes=0, blocks=480, coarse_blocks=192):
         """Multigrid smoothing of the first `levels` levels with one launch
         per red-black sweep (0: two half-sweep launches everywhere);
         identical results.  True/False mean 1/0."""
-        _ok(self.L.ib3d_set_fused_sweeps(int(levels), int(planes), int(blocks)))
+        _ok(self.L.ib3d_set_fused_sweeps(int(levels), int(planes), int(blocks), int(coarse_blocks)))
 
     def set_coarse_tiles(self, from_level=2):
         """Multigrid levels from `from_level` on (0: none) smoothed by two

@@ -26,7 +26,7 @@ MULTIGRID = {"m": 6, "nu1": 4, "nu2": 4, "omega": 1.55, "coarse": 4, "coarse_ome
 # how the smoothing is launched (identical results either way): whole sweeps
 # in one launch on the first `fused` levels, two temporally blocked launches
 # per smoothing phase from level `tiles` on
-LAUNCH = {"fused": 1, "tiles": 2}
+LAUNCH = {"fused": 2, "tiles": 2}
 
 MODES = ("reference", "fp64-cold", "fp64", "mixed-dc", "gmres-sor", "mixed")
 

@@ -179,11 +179,14 @@ int ib3d_set_krylov_target(double target);
  * Those levels are small enough that a sweep costs about a launch, whatever
  * the level's size. */
 int ib3d_set_multigrid_coarse(int nu1, int nu2);
-/* mode 4: smooth the first `levels` multigrid levels (1, the default: the
- * finest only) with whole red-black sweeps in one launch each, the others
- * with two half-sweep launches (0: none).  Both give identical results.  planes: z planes per workgroup (0: chosen so that the grid holds
- * about `blocks` workgroups; default 0 and 480). */
-int ib3d_set_fused_sweeps(int levels, int planes, int blocks);
+/* mode 4: smooth the first `levels` multigrid levels (2, the default: the
+ * finest and the next) with whole red-black sweeps in one launch each, the
+ * others with two half-sweep launches (0: none), unless they are tiled
+ * (ib3d_set_coarse_tiles).  Both give identical results.  planes: z planes
+ * per workgroup on the finest level (0: chosen so that the grid holds about
+ * `blocks` workgroups); below it the grid holds about `coarse_blocks`
+ * (defaults 0, 480 and 192). */
+int ib3d_set_fused_sweeps(int levels, int planes, int blocks, int coarse_blocks);
 /* mode 4: smooth the multigrid levels from `from_level` on (2, the
  * default; 0 for none) that are cycled by launches with two temporally
  * blocked launches per smoothing phase: each workgroup sweeps an 8x8x8 cube

@@ -455,6 +455,12 @@ def main():
         roof_out["kernel"] = ("k_sweep2_pz: FP32 red-black sweep in one launch (12 B per cell)" if rprec == 33 else
                               f"k_sor_pz<{'float' if prec == 32 else 'double'}> half-sweep")
         roof_out["launch_us"] = roof["launch_us"]
+        # what a plain streaming copy reaches on this device (604 MB read and
+        # written), for reading `frac` against a practical ceiling
+        s_cp = Solver.from_case(case)
+        cp_ms = min(s_cp.probe_copy(603979776) for _ in range(3))
+        s_cp.close()
+        roof_out["copy_gbs"] = round(2 * 603979776 / cp_ms / 1e6, 1)
         line = {
             "metric": "steps/s + Poisson ms/step to 1e-6 residual: FP64 SOR vs mixed+learned guess",
             "value": round(main_res["steps_per_s"], 4),

@@ -361,6 +361,9 @@ def main():
     ap.add_argument("--gmres-m", type=int, default=GMRES["m"])
     ap.add_argument("--gmres-k", type=int, default=GMRES["k"])
     ap.add_argument("--gmres-omega", type=float, default=GMRES["omega"])
+    ap.add_argument("--profile-momentum", action="store_true",
+                    help="set the case up, run one step, then three nsMomentum calls with the direct kernel and three "
+                         "with the LDS-tiled one (a short program for rocprofv3 counter passes)")
     ap.add_argument("--profile-sor", nargs=2, type=int, metavar=("PRECISION", "SWEEPS"),
                     help="profiling aid: set up the case, run SWEEPS RB-SOR sweeps and one copy probe, exit")
     a = ap.parse_args()
@@ -374,6 +377,22 @@ def main():
 
     cfg = cases.CONFIGS[a.config]
     case = tempfile.mkdtemp(prefix="ib3d_bench_")
+    if a.profile_momentum:
+        try:
+            cases.write_case(cfg, case)
+            s = Solver.from_case(case)
+            s.step()
+            for tiles in (0, 1):
+                s.set_momentum_tiles(tiles)
+                for _ in range(3):
+                    s.nsMomentum()
+            s.set_momentum_tiles(0)
+            s.sync()
+            print(json.dumps({"profile": "momentum", "calls": 3, "variants": ["direct", "lds tiles 64x8"]}), flush=True)
+            s.close()
+        finally:
+            shutil.rmtree(case, ignore_errors=True)
+        return
     if a.profile_sor:
         try:
             cases.write_case(cfg, case)

@@ -534,12 +534,13 @@ k_sweep2_pz(const float *__restrict__ src0, const float *__restrict__ src1, floa
 }
 
 // Several red-black sweeps of a coarse multigrid level in one launch
-// (temporal blocking).  A workgroup owns an 8x8x8 cube of cells and loads it
-// into LDS with a halo of 2M cells on every side (M sweeps: each half-sweep
-// moves the dependency one cell), in natural (i, j, k) order, both colours
-// interleaved.  Half-sweep m (1..2M) updates the cells of its colour that lie
-// at most 2M-m cells outside the cube, so the values it reads were exact after
-// half-sweep m-1 and the cube is exact after the last one.  The cube's
+// (temporal blocking).  A workgroup owns a CX x CY x CZ block of cells (8 x 8
+// x 8 as launched) and loads it into LDS with a halo of 2M cells on every
+// side (M sweeps: each half-sweep moves the dependency one cell), in natural
+// (i, j, k) order, both colours interleaved.  Half-sweep m (1..2M) updates
+// the cells of its colour that lie at most 2M-m cells outside the block, so
+// the values it reads were exact after half-sweep m-1 and the block is exact
+// after the last one.  The block's
 // interior cells (both colours: updated cells and, copied through, the cells
 // outside the solve) go to dst, never to src, which the neighbouring
 // workgroups are still reading; walls are never written (they stay as in
@@ -549,36 +550,36 @@ k_sweep2_pz(const float *__restrict__ src0, const float *__restrict__ src1, floa
 // black half-sweep the old value, as k_sor does from zero.  Every update is
 // k_sor's, operand for operand: the result is bit for bit that of 2M
 // half-sweep launches.
-static constexpr int kTileCube = 8;
+static constexpr int kTileCube = 8;       // the cube of levels 2 and below
 static constexpr int kTileMaxSweeps = 2;
-static constexpr int kTileEdge = kTileCube + 4 * kTileMaxSweeps;   // 16
 static constexpr int kTileThreads = 1024;
-template <bool FIRST, int M>
+template <bool FIRST, int M, int CX, int CY, int CZ>
 __global__ void __launch_bounds__(kTileThreads)
 k_smooth_tile(const float *__restrict__ src, float *__restrict__ dst, const float *__restrict__ b,
               const float *__restrict__ axc, const float *__restrict__ ayc, const float *__restrict__ azc, SorGeom g,
               float omega, float omm, const double *gate, double thr) {
-    __shared__ float xs[kTileEdge * kTileEdge * kTileEdge];
-    __shared__ float bs[kTileEdge * kTileEdge * kTileEdge];
-    __shared__ float cx[3][kTileEdge], cy[3][kTileEdge], cz[3][kTileEdge];
-    if (gated(gate, thr)) return;
     static_assert(M >= 1 && M <= kTileMaxSweeps, "sweeps per launch");
-    constexpr int R = 2 * M, E = kTileCube + 2 * R, E2 = E * E, E3 = E2 * E;
-    // cube origin (first interior cell) minus the halo: tile position 0
-    const int i0 = 1 + kTileCube * (int)blockIdx.x - R;
-    const int j0 = 1 + kTileCube * (int)blockIdx.y - R;
-    const int k0 = 1 + kTileCube * (int)blockIdx.z - R;
+    constexpr int R = 2 * M, EX = CX + 2 * R, EY = CY + 2 * R, EZ = CZ + 2 * R;
+    constexpr int E2 = EX * EY, E3 = E2 * EZ;
+    __shared__ float xs[E3];
+    __shared__ float bs[E3];
+    __shared__ float cx[3][EX], cy[3][EY], cz[3][EZ];
+    if (gated(gate, thr)) return;
+    // cuboid origin (first interior cell) minus the halo: tile position 0
+    const int i0 = 1 + CX * (int)blockIdx.x - R;
+    const int j0 = 1 + CY * (int)blockIdx.y - R;
+    const int k0 = 1 + CZ * (int)blockIdx.z - R;
     const int tid = (int)threadIdx.x;
     const float nan = __builtin_nanf("");
     // coefficient rows of the tile's columns (clamped: positions outside
     // the arrays are never updated, so their rows are never used)
-    const int ca = min(tid / E, 2), ct = tid - ca * E;
+    const int ax_ = min(tid / EX, 2), tx_ = tid - ax_ * EX;
+    const int ay_ = min(tid / EY, 2), ty_ = tid - ay_ * EY;
+    const int az_ = min(tid / EZ, 2), tz_ = tid - az_ * EZ;
     float cxv = 0.0f, cyv = 0.0f, czv = 0.0f;
-    if (tid < 3 * E) {
-        cxv = axc[ca * g.sxp + min(max(i0 + ct, 0), g.sx - 1)];
-        cyv = ayc[ca * g.sy + min(max(j0 + ct, 0), g.sy - 1)];
-        czv = azc[ca * g.sz + min(max(k0 + ct, 0), g.sz - 1)];
-    }
+    if (tid < 3 * EX) cxv = axc[ax_ * g.sxp + min(max(i0 + tx_, 0), g.sx - 1)];
+    if (tid < 3 * EY) cyv = ayc[ay_ * g.sy + min(max(j0 + ty_, 0), g.sy - 1)];
+    if (tid < 3 * EZ) czv = azc[az_ * g.sz + min(max(k0 + tz_, 0), g.sz - 1)];
     // every thread issues all of its loads (at most kTileLoads cells each)
     // before it stores any, so their latencies overlap; positions outside the
     // arrays load a clamped (valid) cell and store 0 and NaN instead
@@ -588,7 +589,7 @@ k_smooth_tile(const float *__restrict__ src, float *__restrict__ dst, const floa
 #pragma unroll
     for (int q = 0; q < kTileLoads; q++) {
         const int n = min(tid + q * kTileThreads, E3 - 1);
-        const int lk = n / E2, r = n - lk * E2, lj = r / E, li = r - lj * E;
+        const int lk = n / E2, r = n - lk * E2, lj = r / EX, li = r - lj * EX;
         const int ii = i0 + li, jj = j0 + lj, kk = k0 + lk;
         in[q] = ii >= 0 && ii < g.sx && jj >= 0 && jj < g.sy && kk >= 0 && kk < g.sz;
         const int ic = min(max(ii, 0), g.sx - 1), jc = min(max(jj, 0), g.sy - 1), kc = min(max(kk, 0), g.sz - 1);
@@ -605,23 +606,22 @@ k_smooth_tile(const float *__restrict__ src, float *__restrict__ dst, const floa
             bs[n] = bq;
         }
     }
-    if (tid < 3 * E) {
-        cx[ca][ct] = cxv;
-        cy[ca][ct] = cyv;
-        cz[ca][ct] = czv;
-    }
+    if (tid < 3 * EX) cx[ax_][tx_] = cxv;
+    if (tid < 3 * EY) cy[ay_][ty_] = cyv;
+    if (tid < 3 * EZ) cz[az_][tz_] = czv;
     __syncthreads();
 #pragma unroll
     for (int m = 1; m <= 2 * M; m++) {
         const int col = (m - 1) & 1;
-        const int W = E - 2 * m, HW = (W + 1) >> 1;  // region [m, E-m) per axis; cells of one colour per row <= HW
+        // region [m, E-m) per axis; cells of one colour per row <= HW
+        const int WX = EX - 2 * m, WY = EY - 2 * m, WZ = EZ - 2 * m, HW = (WX + 1) >> 1;
 #pragma unroll
-        for (int idx = tid; idx < HW * W * W; idx += kTileThreads) {
+        for (int idx = tid; idx < HW * WY * WZ; idx += kTileThreads) {
             const int row = idx / HW, p = idx - row * HW;
-            const int lk = m + row / W, lj = m + (row - (row / W) * W);
+            const int lk = m + row / WY, lj = m + (row - (row / WY) * WY);
             const int li = m + (((col - (i0 + m + j0 + lj + k0 + lk)) & 1)) + 2 * p;
-            if (li >= E - m) continue;
-            const int n = li + E * (lj + E * lk);
+            if (li >= EX - m) continue;
+            const int n = li + EX * (lj + EY * lk);
             const float r = bs[n];
             if (!(r == r)) continue;
             float pw, pe, ps, pn, pb, pt;
@@ -629,7 +629,7 @@ k_smooth_tile(const float *__restrict__ src, float *__restrict__ dst, const floa
                 pw = pe = ps = pn = pb = pt = 0.0f;
             } else {
                 pw = xs[n - 1]; pe = xs[n + 1];
-                ps = xs[n - E]; pn = xs[n + E];
+                ps = xs[n - EX]; pn = xs[n + EX];
                 pb = xs[n - E2]; pt = xs[n + E2];
             }
             const float pold = xs[n];
@@ -642,14 +642,13 @@ k_smooth_tile(const float *__restrict__ src, float *__restrict__ dst, const floa
         }
         __syncthreads();
     }
-    for (int n = tid; n < kTileCube * kTileCube * kTileCube; n += kTileThreads) {
-        const int ck = n / (kTileCube * kTileCube), r = n - ck * kTileCube * kTileCube, cj = r / kTileCube,
-                  ci = r - cj * kTileCube;
+    for (int n = tid; n < CX * CY * CZ; n += kTileThreads) {
+        const int ck = n / (CX * CY), r = n - ck * CX * CY, cj = r / CX, ci = r - cj * CX;
         const int li = R + ci, lj = R + cj, lk = R + ck;
         const int ii = i0 + li, jj = j0 + lj, kk = k0 + lk;
         if (ii > g.sx - 2 || jj > g.sy - 2 || kk > g.sz - 2) continue;
         const int f = (kk * g.sy + jj) * g.H + (ii >> 1) + ((ii + jj + kk) & 1) * g.NC;
-        dst[f] = xs[li + E * (lj + E * lk)];
+        dst[f] = xs[li + EX * (lj + EY * lk)];
     }
 }
 
@@ -2226,23 +2225,18 @@ static void mg_smooth(Ctx &c, const MgLevel &L, bool finest, int n, bool first, 
     dim3 sg;
     if (!finest && g_coarse_tiles > 0 && L.level >= g_coarse_tiles && L.x2 && n >= 2 &&
         (n + 1) / 2 <= kTileMaxSweeps) {
-        // two launches of ceil(n/2) and floor(n/2) sweeps: x -> x2 -> x
-        const dim3 tg((unsigned)((L.sx - 2 + kTileCube - 1) / kTileCube),
-                      (unsigned)((L.sy - 2 + kTileCube - 1) / kTileCube),
-                      (unsigned)((L.sz - 2 + kTileCube - 1) / kTileCube));
+        // two launches of ceil(n/2) and floor(n/2) sweeps: x -> x2 -> x, on
+        // 8 x 8 x 8 blocks
+        constexpr int T = kTileCube;
+        const dim3 tg((unsigned)((L.sx - 2 + T - 1) / T), (unsigned)((L.sy - 2 + T - 1) / T),
+                      (unsigned)((L.sz - 2 + T - 1) / T));
         auto launch = [&](int m, bool fz, const float *from, float *to) {
-            if (m == 2 && fz)
-                hipLaunchKernelGGL((k_smooth_tile<true, 2>), tg, dim3(kTileThreads), 0, c.stream, from, to, L.b, L.ax,
-                                   L.ay, L.az, g, om, omm, gate, 0.5);
-            else if (m == 2)
-                hipLaunchKernelGGL((k_smooth_tile<false, 2>), tg, dim3(kTileThreads), 0, c.stream, from, to, L.b, L.ax,
-                                   L.ay, L.az, g, om, omm, gate, 0.5);
-            else if (fz)
-                hipLaunchKernelGGL((k_smooth_tile<true, 1>), tg, dim3(kTileThreads), 0, c.stream, from, to, L.b, L.ax,
-                                   L.ay, L.az, g, om, omm, gate, 0.5);
-            else
-                hipLaunchKernelGGL((k_smooth_tile<false, 1>), tg, dim3(kTileThreads), 0, c.stream, from, to, L.b, L.ax,
-                                   L.ay, L.az, g, om, omm, gate, 0.5);
+#define IB3D_TILE(F, MM)                                                                                            \
+    hipLaunchKernelGGL((k_smooth_tile<F, MM, T, T, T>), tg, dim3(kTileThreads), 0, c.stream, from, to, L.b, L.ax,    \
+                       L.ay, L.az, g, om, omm, gate, 0.5)
+            if (m == 2) { if (fz) IB3D_TILE(true, 2); else IB3D_TILE(false, 2); }
+            else { if (fz) IB3D_TILE(true, 1); else IB3D_TILE(false, 1); }
+#undef IB3D_TILE
         };
         launch((n + 1) / 2, first, L.x, L.x2);
         launch(n / 2, false, L.x2, L.x);

@@ -5,7 +5,9 @@ half-sweep launches per sweep: the same FP32 operations on the same operands,
 so every field, every Krylov count and every residual must be bit-identical.
 Covers the first phase from zero, odd sweep counts (launches of 2 and 1
 sweeps), one sweep per launch, counts above the kernel's limit (the
-half-sweep launches are kept), and the fused finest-level sweep beside it."""
+half-sweep launches are kept), the fused finest-level sweep beside it, and
+the 16 x 16 x 8 blocks of a level of half a million cells or more (level 1
+at 256x192x192)."""
 import sys
 
 import numpy as np
@@ -40,7 +42,7 @@ def _run(case, tiles, nsteps, nu, fused):
 @pytest.mark.parametrize("name,nsteps,nu,fused,first", [("small", 5, (4, 4), 0, 1), ("small", 4, (3, 2), 0, 1),
                                                         ("small", 3, (1, 5), 0, 1), ("cfg1", 4, (4, 4), 0, 1),
                                                         ("cfg1", 3, (3, 3), 1, 2), ("cfg2", 3, (4, 4), 1, 2),
-                                                        ("multi", 3, (4, 4), 0, 1)])
+                                                        ("multi", 3, (4, 4), 0, 1), ("cfg2", 2, (4, 4), 1, 1)])
 def test_coarse_tiles_are_bitwise_the_half_sweeps(case_dir, name, nsteps, nu, fused, first):
     """first: the first level smoothed by the tiled launches"""
     d = case_dir(name)

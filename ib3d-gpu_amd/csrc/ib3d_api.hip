@@ -428,14 +428,12 @@ int ib3d_init(int64_t nx, int64_t ny, int64_t nz) {
     IB3D_CHECK(hipGetDevice(&c.device));
     if (noalloc_setup(c.device)) return -1;
     c.nx = nx; c.ny = ny; c.nz = nz;
-    c.sx = nx + 2; c.sy = ny + 2; c.sz = nz + 2;
+    const ColourLayout lay = colour_layout(nx, ny, nz);
+    c.sx = lay.sx; c.sy = lay.sy; c.sz = lay.sz;
     c.P = c.sx * c.sy * c.sz;
-    c.H = (((c.sx + 1) / 2) + 1) & ~int64_t(1);
-    c.NC = c.sy * c.sz * c.H;
-    // x coefficient rows: a thread reads the 4 entries 2h..2h+3 of slot pair h
-    // (h < H), so a row holds at least 2H + 2 entries; a multiple of 8 keeps
-    // those 4-vectors aligned
-    c.sxp = (std::max<int64_t>(c.sx, 2 * c.H + 2) + 7) & ~int64_t(7);
+    c.H = lay.H;
+    c.NC = lay.NC;
+    c.sxp = lay.sxp;
     IB3D_REQUIRE(c.P < (int64_t)1 << 31, "ib3d_init: grid too large for 32-bit cell indices");
     if (check_fastdiv((int)c.H, c.NC + 1) || check_fastdiv((int)c.sy, c.NC / c.H + 1)) {
         set_error("ib3d_init: fast division self-check failed");
@@ -991,6 +989,12 @@ int ib3d_set_fused_sweeps(int levels, int planes, int blocks, int coarse_blocks)
     return set_fused_sweeps(levels, planes, blocks, coarse_blocks);
 }
 int ib3d_set_coarse_tiles(int from_level) { return set_coarse_tiles(from_level); }
+
+int ib3d_selfcheck_resid_guard(int64_t nx, int64_t ny, int64_t nz, int drop_interior, int64_t *outside) {
+    IB3D_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && outside, "selfcheck_resid_guard: bad grid or output");
+    *outside = resid_guard_selfcheck(nx, ny, nz, drop_interior != 0);
+    return 0;
+}
 
 int ib3d_set_momentum_tiles(int on) { return set_momentum_tiles(on); }
 

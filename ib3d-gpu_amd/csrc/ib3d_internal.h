@@ -18,6 +18,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -196,6 +197,23 @@ struct FastDiv {
     int d = 1;
 };
 
+// The colour-compacted layout of an nx x ny x nz interior grid: padded
+// extents, slots per colour row (H, even), slots per colour (NC) and the x
+// coefficient row stride (sxp).  A thread reads the 4 x-coefficient entries
+// 2h..2h+3 of slot pair h (h < H), so a row holds at least 2H + 2 entries;
+// a multiple of 8 keeps those 4-vectors aligned.
+struct ColourLayout {
+    int64_t sx, sy, sz, H, NC, sxp;
+};
+static inline ColourLayout colour_layout(int64_t nx, int64_t ny, int64_t nz) {
+    ColourLayout L;
+    L.sx = nx + 2; L.sy = ny + 2; L.sz = nz + 2;
+    L.H = (((L.sx + 1) / 2) + 1) & ~int64_t(1);
+    L.NC = L.sy * L.sz * L.H;
+    L.sxp = (std::max<int64_t>(L.sx, 2 * L.H + 2) + 7) & ~int64_t(7);
+    return L;
+}
+
 static inline FastDiv make_fastdiv(int d) {
     FastDiv f;
     int l = 0;
@@ -264,6 +282,7 @@ int compute_div_noalloc(Ctx &c, double deltat, double *divmax);
 int noalloc_setup(int device);
 int set_fused_sweeps(int on, int tz, int blocks, int coarse_blocks);
 int set_coarse_tiles(int from_level);
+int64_t resid_guard_selfcheck(int64_t nx, int64_t ny, int64_t nz, bool drop_interior);
 int set_momentum_tiles(int on);
 int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);
 int plans_launch(Ctx &c);

@@ -172,12 +172,19 @@ struct SorGeom {
     FastDiv dH, dSy;
 };
 
-__device__ __forceinline__ void decode(const SorGeom &g, int f, int c, int &ii, int &jj, int &kk) {
+__host__ __device__ __forceinline__ void decode(const SorGeom &g, int f, int c, int &ii, int &jj, int &kk) {
     const int row = (int)fdiv((uint32_t)f, g.dH);
     const int h = f - row * g.H;
     kk = (int)fdiv((uint32_t)row, g.dSy);
     jj = row - kk * g.sy;
     ii = 2 * h + ((c + jj + kk) & 1);
+}
+
+// A row of the residual kernels: an interior cell.  With the finite b/dt
+// of a live cell this is what k_resid tests; the interior test keeps every
+// neighbour access inside the arrays whatever the NaN sentinel holds.
+__host__ __device__ __forceinline__ bool interior_cell(const SorGeom &g, int ii, int jj, int kk) {
+    return ii >= 1 && ii <= g.sx - 2 && jj >= 1 && jj <= g.sy - 2 && kk >= 1 && kk <= g.sz - 2;
 }
 
 // FP64 keeps the reference's correctly rounded division; the FP32 smoother
@@ -679,7 +686,7 @@ k_resid(const double *__restrict__ p0, const double *__restrict__ p1, const floa
         decode(g, f, colour, ii, jj, kk);
         // live cells carry a finite b/dt; the explicit interior test keeps
         // every neighbour access inside the arrays whatever the sentinel holds
-        if (bd == bd && ii >= 1 && ii <= g.sx - 2 && jj >= 1 && jj <= g.sy - 2 && kk >= 1 && kk <= g.sz - 2) {
+        if (bd == bd && interior_cell(g, ii, jj, kk)) {
             const int q = (colour + jj + kk) & 1;
             const int sx = g.sxp, sy = g.sy, nzp = g.sz;
             double pw = po[f - 1 + q], pe = po[f + q], ps = po[f - g.H], pn = po[f + g.H];
@@ -1199,6 +1206,38 @@ static SorGeom geom(const Ctx &c) {
     g.dH = make_fastdiv(g.H);
     g.dSy = make_fastdiv(g.sy);
     return g;
+}
+
+// Host self-check of k_resid's guard on an nx x ny x nz grid: every slot of
+// both colours is taken as live (a finite b/dt everywhere, the case of a
+// sentinel that was overwritten); a slot the guard admits reads its six
+// neighbours and its coefficient entries, which must all lie inside the
+// arrays.  Returns the number of reads outside.  drop_interior leaves the
+// interior test out (the guard before it was added), so a caller can see
+// the check catch that form.
+int64_t resid_guard_selfcheck(int64_t nx, int64_t ny, int64_t nz, bool drop_interior) {
+    const ColourLayout lay = colour_layout(nx, ny, nz);
+    SorGeom g;
+    g.H = (int)lay.H; g.NC = (int)lay.NC; g.sy = (int)lay.sy; g.sx = (int)lay.sx; g.sz = (int)lay.sz;
+    g.plane = (int)(lay.H * lay.sy); g.sxp = (int)lay.sxp;
+    g.dH = make_fastdiv(g.H);
+    g.dSy = make_fastdiv(g.sy);
+    int64_t outside = 0;
+    auto in = [&](int64_t i, int64_t n) { return i >= 0 && i < n; };
+    for (int colour = 0; colour < 2; colour++) {
+        for (int f = 0; f < g.NC; f++) {
+            int ii, jj, kk;
+            decode(g, f, colour, ii, jj, kk);
+            if (!drop_interior && !interior_cell(g, ii, jj, kk)) continue;
+            const int q = (colour + jj + kk) & 1;
+            for (int64_t idx : {(int64_t)f - 1 + q, (int64_t)f + q, (int64_t)f - g.H, (int64_t)f + g.H,
+                                (int64_t)f - g.plane, (int64_t)f + g.plane})
+                outside += !in(idx, g.NC);
+            outside += !in(ii, g.sxp) + !in(2 * (int64_t)g.sxp + ii, 3 * (int64_t)g.sxp);
+            outside += !in(jj, g.sy) + !in(kk, g.sz);
+        }
+    }
+    return outside;
 }
 
 // host-side self check of the division helper for this grid's divisors

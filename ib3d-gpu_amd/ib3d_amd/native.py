@@ -70,6 +70,7 @@ def lib():
         "ib3d_set_krylov_target": [D],
         "ib3d_set_fused_sweeps": [I, I, I, I],
         "ib3d_set_coarse_tiles": [I],
+        "ib3d_selfcheck_resid_guard": [I64, I64, I64, I, PI64],
         "ib3d_set_momentum_tiles": [I],
         "ib3d_set_graphs": [I],
         "ib3d_poisson_stats": [PI64, PI64, PD, PD, PD, PD],
@@ -130,6 +131,15 @@ def _fp(a):
 INT_ARRAYS = {"cell", "minElemcell", "fluidIndexPtr", "solidIndexPtr", "interceptedIndexPtr",
               "redCellIndexPtr", "blackCellIndexPtr", "nelp", "nelu1", "nelu2", "nelv1", "nelv2",
               "nelw1", "nelw2", "Elemcell", "ucell", "vcell", "wcell", "pcell"}
+
+
+def resid_guard_selfcheck(nx, ny, nz, drop_interior=False):
+    """Reads outside the arrays that the residual kernel's guard admits on an
+    nx x ny x nz grid when every slot holds a finite b/dt (host check, no GPU
+    needed): 0 with the interior test, more without it."""
+    out = C.c_int64()
+    _ok(lib().ib3d_selfcheck_resid_guard(int(nx), int(ny), int(nz), int(bool(drop_interior)), C.byref(out)))
+    return out.value
 
 
 class Solver:

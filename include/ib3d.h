@@ -194,6 +194,13 @@ int ib3d_set_fused_sweeps(int levels, int planes, int blocks, int coarse_blocks)
  * the second (n = 2..4), instead of 2n half-sweep launches.  Both give
  * identical results. */
 int ib3d_set_coarse_tiles(int from_level);
+/* No counterpart in the reference; needs no GPU.  Host self-check of the
+ * residual kernel's guard on an nx x ny x nz grid: with a finite b/dt in
+ * every slot (an overwritten NaN sentinel), counts the neighbour and
+ * coefficient reads of admitted slots that fall outside the arrays (0 is
+ * correct).  drop_interior != 0 checks the guard without its interior
+ * test instead, which reads outside. */
+int ib3d_selfcheck_resid_guard(int64_t nx, int64_t ny, int64_t nz, int drop_interior, int64_t *outside);
 /* nsMomentum: the interior columns from 64 x 8 LDS tiles marching in z
  * (on != 0) or with every stencil value read from HBM (on == 0, the default).
  * Both give identical results. */

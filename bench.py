@@ -284,6 +284,59 @@ def host_cpu():
     return {"model": model, "logical_cpus_available": len(os.sched_getaffinity(0))}
 
 
+def port_sample(case, nsweeps):
+    """Stage times of the C restatement's OpenMP build (the oracle, as the
+    all-cores CPU baseline) on the case in `case`: one predictor / forcing /
+    projection pass and `nsweeps` reference-rule SOR iterations.  Runs in
+    the process that calls it; the thread count is OMP_NUM_THREADS's."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.use_openmp()
+    o = pyoracle.Oracle.from_case(case)
+    t = {}
+
+    def timed(name, fn):
+        t0 = time.perf_counter()
+        fn()
+        t[name] = t.get(name, 0.0) + time.perf_counter() - t0
+    timed("nsMomentum", lambda: o.call("ns_momentum"))
+    timed("velocityBC", lambda: o.call("velocity_bc"))
+    timed("solidCellBC", lambda: o.call("solid_cell_bc"))
+    timed("velocityForcing1+BC", lambda: (o.call("velocity_forcing"), o.call("velocity_bc")))
+    timed("computeDiv", lambda: o.compute_div())
+    t0 = time.perf_counter()
+    for _ in range(nsweeps):
+        o.redblack_sor(1e300)   # one iteration each
+    t["sor_per_sweep"] = (time.perf_counter() - t0) / nsweeps
+    timed("correct+BC", lambda: (o.call("correct_pressure"), o.call("correct_velocity"), o.call("velocity_bc")))
+    timed("pressureForcing1", lambda: o.call("pressure_forcing"))
+    timed("stressCal1", lambda: o.stress(True))
+    return t
+
+
+def cpu_baseline_all_cores(case, gpu_sweeps_per_step, nsweeps):
+    """The all-cores leg: port_sample in a child process with OMP_NUM_THREADS
+    set to the logical CPUs this process may use."""
+    threads = len(os.sched_getaffinity(0))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    t0 = time.perf_counter()
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--port-sample", case, str(nsweeps)],
+                         capture_output=True, text=True, env=env, timeout=900, check=True).stdout
+    wall = time.perf_counter() - t0
+    vals = json.loads(out.strip().splitlines()[-1])
+    per_step = (vals["nsMomentum"] + 2 * vals["velocityBC"] + vals["solidCellBC"] + vals["velocityForcing1+BC"] +
+                vals["computeDiv"] + gpu_sweeps_per_step * vals["sor_per_sweep"] + vals["correct+BC"] +
+                vals["pressureForcing1"] + vals["stressCal1"])
+    return {"value": round(1.0 / per_step, 6), "unit": "steps/s", "cores": threads, "kind": "port",
+            "extrapolated": True,
+            "sample": (f"the C restatement (gcc -O2, OpenMP on its independent loops: predictor, divergence, "
+                       f"red-black SOR colours, corrections; {threads} threads) on the same case: one "
+                       f"predictor/forcing/projection pass and {nsweeps} SOR iterations "
+                       f"({vals['sor_per_sweep'] * 1e3:.1f} ms each), {wall:.1f} s wall with setup; the value is the "
+                       f"stage times with the SOR part scaled to {gpu_sweeps_per_step:.0f} sweeps/step = "
+                       f"{per_step:.2f} s per step")}
+
+
 def cpu_baseline(case, gpu_sweeps_per_step, nsweeps, cfg1_steps):
     """The reference's own Fortran (oracle/_ref), one thread: a measured
     whole-step time on the 128x64x64 sphere, and an extrapolated per-step
@@ -327,6 +380,10 @@ def cpu_baseline(case, gpu_sweeps_per_step, nsweeps, cfg1_steps):
     finally:
         shutil.rmtree(c1, ignore_errors=True)
     cpu = host_cpu()
+    try:
+        all_cores = cpu_baseline_all_cores(case, gpu_sweeps_per_step, 4 * nsweeps)
+    except (subprocess.SubprocessError, OSError, ValueError, KeyError) as e:
+        all_cores = {"error": str(e)[:200]}
     return {"value": round(1.0 / per_step, 6), "unit": "steps/s", "cores": 1, "kind": "reference",
             "extrapolated": True, "cpu_model": cpu["model"], "logical_cpus_available": cpu["logical_cpus_available"],
             "sample": (f"reference Fortran (amdflang -O3, 1 thread) on the same 256x192x192 case: setup, one "
@@ -334,7 +391,7 @@ def cpu_baseline(case, gpu_sweeps_per_step, nsweeps, cfg1_steps):
                        f"({vals['sor_per_sweep'] * 1e3:.1f} ms each), {wall:.1f} s wall; the value is an "
                        f"extrapolation: the stage times with the SOR part scaled to {gpu_sweeps_per_step:.0f} "
                        f"sweeps/step = {per_step:.2f} s per step"),
-            "measured_whole_steps": measured}
+            "measured_whole_steps": measured, "all_cores": all_cores}
 
 
 def main():
@@ -361,6 +418,7 @@ def main():
     ap.add_argument("--gmres-m", type=int, default=GMRES["m"])
     ap.add_argument("--gmres-k", type=int, default=GMRES["k"])
     ap.add_argument("--gmres-omega", type=float, default=GMRES["omega"])
+    ap.add_argument("--port-sample", nargs=2, metavar=("CASE_DIR", "SWEEPS"), help=argparse.SUPPRESS)
     ap.add_argument("--profile-momentum", action="store_true",
                     help="set the case up, run one step, then three nsMomentum calls with the direct kernel and three "
                          "with the LDS-tiled one (a short program for rocprofv3 counter passes)")
@@ -372,6 +430,9 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
               f"(the hot path's multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         raise SystemExit(2)
+    if a.port_sample:
+        print(json.dumps(port_sample(a.port_sample[0], int(a.port_sample[1]))), flush=True)
+        return
     from ib3d_amd import cases
     from ib3d_amd.native import Solver
 

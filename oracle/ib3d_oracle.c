@@ -670,6 +670,7 @@ int orc_ns_momentum(orc_state *s) {
     const double *u = s->u, *v = s->v, *w = s->w, *p = s->p;
     const double *dxa = s->deltax, *dya = s->deltay, *dza = s->deltaz;
     const int64_t *cell = s->cell;
+#pragma omp parallel for schedule(static)
     for (int64_t n = 1; n <= nf; n++) {
         int64_t i = LST(s->fluidList, nf, n, 1), j = LST(s->fluidList, nf, n, 2), k = LST(s->fluidList, nf, n, 3);
         double dxr = dxa[i + 1], dx = dxa[i], dxl = dxa[i - 1];
@@ -1012,6 +1013,7 @@ int orc_pressure_forcing(orc_state *s) {
 int orc_compute_div(orc_state *s, double *divmax) {
     int64_t nf = s->fluidCount;
     double dm = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : dm)
     for (int64_t n = 1; n <= nf; n++) {
         int64_t i = LST(s->fluidList, nf, n, 1), j = LST(s->fluidList, nf, n, 2), k = LST(s->fluidList, nf, n, 3);
         double bb = ((F(s->ut, i, j, k) - F(s->ut, i - 1, j, k)) / s->deltax[i] +
@@ -1028,6 +1030,7 @@ int orc_compute_div(orc_state *s, double *divmax) {
 /* one colour sweep of subPcorVcor.f90:164-190 */
 static void sor_colour(orc_state *s, const int64_t *lst, int64_t cnt, const double *src, double omega) {
     const double deltat = s->deltat;
+#pragma omp parallel for schedule(static)
     for (int64_t n = 1; n <= cnt; n++) {
         int64_t i = LST(lst, cnt, n, 1), j = LST(lst, cnt, n, 2), k = LST(lst, cnt, n, 3);
         double g = ((((((F(s->b, i, j, k) / deltat - ACX(i - 1, 1) * F(src, i - 1, j, k)) -
@@ -1053,6 +1056,7 @@ int orc_redblack_sor(orc_state *s, double epsi, int64_t *isum, double *derr2_out
         derr2 = 0.0;
         sor_colour(s, s->redList, s->redCount, s->pco, omega);
         sor_colour(s, s->blackList, s->blackCount, s->pc, omega);
+#pragma omp parallel for schedule(static) reduction(max : derr2)
         for (int64_t n = 1; n <= nf; n++) {
             int64_t i = LST(s->fluidList, nf, n, 1), j = LST(s->fluidList, nf, n, 2), k = LST(s->fluidList, nf, n, 3);
             double d = fabs(F(s->pc, i, j, k) - F(s->pco, i, j, k));
@@ -1069,6 +1073,7 @@ int orc_redblack_sor(orc_state *s, double epsi, int64_t *isum, double *derr2_out
 /* subPcorVcor.f90:113-118 correctPressure */
 int orc_correct_pressure(orc_state *s) {
     int64_t nf = s->fluidCount;
+#pragma omp parallel for schedule(static)
     for (int64_t n = 1; n <= nf; n++) {
         int64_t i = LST(s->fluidList, nf, n, 1), j = LST(s->fluidList, nf, n, 2), k = LST(s->fluidList, nf, n, 3);
         F(s->p, i, j, k) = F(s->p, i, j, k) + F(s->pc, i, j, k);
@@ -1080,6 +1085,7 @@ int orc_correct_pressure(orc_state *s) {
 int orc_correct_velocity(orc_state *s) {
     int64_t nf = s->fluidCount;
     const double dt = s->deltat;
+#pragma omp parallel for schedule(static)
     for (int64_t n = 1; n <= nf; n++) {
         int64_t i = LST(s->fluidList, nf, n, 1), j = LST(s->fluidList, nf, n, 2), k = LST(s->fluidList, nf, n, 3);
         F(s->ut, i, j, k) = F(s->ut, i, j, k) -

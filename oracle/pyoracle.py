@@ -22,6 +22,16 @@ from ib3d_amd import cases  # noqa: E402  (host-side readers of the case files)
 _lib = None
 
 
+def use_openmp():
+    """Load the OpenMP build of the restatement (the all-cores CPU baseline)
+    instead of the serial one; call before the first use.  Both give the
+    same results bit for bit; the thread count is OMP_NUM_THREADS's."""
+    global _LIB
+    if _lib is not None:
+        raise RuntimeError("use_openmp: the oracle library is already loaded")
+    _LIB = os.path.join(_HERE, "_build", "libib3d_oracle_omp.so")
+
+
 def lib():
     global _lib
     if _lib is None:

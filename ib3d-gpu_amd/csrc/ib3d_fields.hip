@@ -12,6 +12,8 @@
 // through correctly rounded reciprocals (dv below), which is also exact.
 #include "ib3d_internal.h"
 
+#include <type_traits>
+
 namespace ib3d {
 
 struct Tabs {
@@ -101,14 +103,39 @@ __device__ __forceinline__ double up1(PT g, int n, double vel, double den, doubl
     return dv<FAST>(vel, den, rden) * t1 - (fabs(vel) * g[6 * n]) * t2;
 }
 
-// Where the predictor reads u, v, w at offset (a, b, d) from its cell.
-// GlobalFields: straight from the padded fields in HBM.
+// Where the predictor reads u, v, w at offset (a, b, d) from its cell, and
+// how it addresses the other arrays of the padded grid (at<a, b, d>(base):
+// the element at that offset).
+// GlobalFields: straight from the padded fields in HBM, 64-bit indices.
 struct GlobalFields {
     const double *__restrict__ u, *__restrict__ v, *__restrict__ w;
     int64_t c, sx, sxy;
     template <int a, int b, int d> __device__ __forceinline__ double U() const { return u[c + a + b * sx + d * sxy]; }
     template <int a, int b, int d> __device__ __forceinline__ double V() const { return v[c + a + b * sx + d * sxy]; }
     template <int a, int b, int d> __device__ __forceinline__ double W() const { return w[c + a + b * sx + d * sxy]; }
+    template <int a, int b, int d, class T> __device__ __forceinline__ T &at(T *base) const {
+        return base[c + a + b * sx + d * sxy];
+    }
+    __device__ __forceinline__ void split() const {}
+};
+
+// GlobalFields32: the same values through 32-bit byte offsets from the
+// uniform array bases, one offset register per address instead of a 64-bit
+// pair (fewer registers, one more wave per SIMD).  For grids whose FP64
+// fields span less than 4 GiB.
+struct GlobalFields32 {
+    const double *__restrict__ u, *__restrict__ v, *__restrict__ w;
+    uint32_t c, sx, sxy;  // element offsets; c * 8 < 2^32
+    template <int a, int b, int d, int S> __device__ __forceinline__ uint32_t off() const {
+        return (c + (uint32_t)a + (uint32_t)b * sx + (uint32_t)d * sxy) * (uint32_t)S;
+    }
+    template <int a, int b, int d> __device__ __forceinline__ double U() const { return at<a, b, d>(u); }
+    template <int a, int b, int d> __device__ __forceinline__ double V() const { return at<a, b, d>(v); }
+    template <int a, int b, int d> __device__ __forceinline__ double W() const { return at<a, b, d>(w); }
+    template <int a, int b, int d, class T> __device__ __forceinline__ T &at(T *base) const {
+        typedef std::conditional_t<std::is_const_v<T>, const char, char> B;
+        return *reinterpret_cast<T *>(reinterpret_cast<B *>(base) + off<a, b, d, (int)sizeof(T)>());
+    }
     __device__ __forceinline__ void split() const {}
 };
 
@@ -137,6 +164,9 @@ struct TiledFields {
         else if constexpr (a == 0 && b == 0) return qw[d + 2];
         else return w[c + a + b * sx + d * sxy];
     }
+    template <int a, int b, int d, class T> __device__ __forceinline__ T &at(T *base) const {
+        return base[c + a + b * sx + d * sxy];
+    }
     // the three components are scheduled one after the other: their values
     // come from LDS, so loading them late costs little, and keeping the
     // components apart keeps the register count down
@@ -153,14 +183,13 @@ __device__ __forceinline__ void momentum_cell(
     double *__restrict__ ut, double *__restrict__ vt, double *__restrict__ wt, double *__restrict__ ru,
     double *__restrict__ rv, double *__restrict__ rw, const Tabs &T, const MomTabs &M, int nx, int ny, int nz,
     double rev, double deltat) {
-    const int64_t sx = nx + 2, sxy = sx * (ny + 2);
-    const int64_t c = (i - 1) + sx * ((j - 1) + (int64_t)(ny + 2) * (k - 1));
-    if (cell[c] != 0) return;
+    if (A.template at<0, 0, 0>(cell) != 0) return;
 #define U(a, b, d) A.template U<a, b, d>()
 #define V(a, b, d) A.template V<a, b, d>()
 #define W(a, b, d) A.template W<a, b, d>()
-#define P(a, b, d) p[c + (a) + (b) * sx + (d) * sxy]
-#define CL(a, b, d) cell[c + (a) + (b) * sx + (d) * sxy]
+#define P(a, b, d) A.template at<a, b, d>(p)
+#define CL(a, b, d) A.template at<a, b, d>(cell)
+#define AT(f) A.template at<0, 0, 0>(f)
     // the y and z records go through the constant address space: the kernel
     // never writes them, so uniform reads of them stay scalar loads even after
     // the stores of an earlier cell in the same thread
@@ -171,24 +200,19 @@ __device__ __forceinline__ void momentum_cell(
 #define MX(q) M.m[0][(q) * M.mn[0] + i]
 #define MY(q) ry[q]
 #define MZ(q) rz[q]
-    const double u22 = U(-1, 0, 0) - U(0, 0, 0), u4 = U(0, 0, 0) - U(1, 0, 0);
-    const double u6 = U(0, -1, 0) - U(0, 0, 0), u8 = U(0, 0, 0) - U(0, 1, 0);
-    const double u10 = U(0, 0, -1) - U(0, 0, 0), u12 = U(0, 0, 0) - U(0, 0, 1);
-    const double v4 = V(-1, 0, 0) - V(0, 0, 0), v6 = V(0, 0, 0) - V(1, 0, 0);
-    const double v8 = V(0, -1, 0) - V(0, 0, 0), v10 = V(0, 0, 0) - V(0, 1, 0);
-    const double v12 = V(0, 0, -1) - V(0, 0, 0), v14 = V(0, 0, 0) - V(0, 0, 1);
-    const double w6 = W(-1, 0, 0) - W(0, 0, 0), w8 = W(0, 0, 0) - W(1, 0, 0);
-    const double w10 = W(0, -1, 0) - W(0, 0, 0), w12 = W(0, 0, 0) - W(0, 1, 0);
-    const double w14 = W(0, 0, -1) - W(0, 0, 0), w16 = W(0, 0, 0) - W(0, 0, 1);
-    const double dpdx = dv<FAST>(P(0, 0, 0) - P(1, 0, 0), MX(M_H1), MX(M_RH1));
-    const double dpdy = dv<FAST>(P(0, 0, 0) - P(0, 1, 0), MY(M_H1), MY(M_RH1));
-    const double dpdz = dv<FAST>(P(0, 0, 0) - P(0, 0, 1), MZ(M_H1), MZ(M_RH1));
+    // each component's differences and pressure gradient are formed inside
+    // its own block, so that none of them stays live across another
+    // component
     const bool no_ib = CL(1, 0, 0) != 2 && CL(-1, 0, 0) != 2 && CL(0, 1, 0) != 2 && CL(0, -1, 0) != 2 &&
                        CL(0, 0, 1) != 2 && CL(0, 0, -1) != 2;
     const int mx = M.mn[0], my = 1, mz = 1;
 
     // U momentum (subNavierStokes.f90:792-897)
     {
+        const double u22 = U(-1, 0, 0) - U(0, 0, 0), u4 = U(0, 0, 0) - U(1, 0, 0);
+        const double u6 = U(0, -1, 0) - U(0, 0, 0), u8 = U(0, 0, 0) - U(0, 1, 0);
+        const double u10 = U(0, 0, -1) - U(0, 0, 0), u12 = U(0, 0, 0) - U(0, 0, 1);
+        const double dpdx = dv<FAST>(P(0, 0, 0) - P(1, 0, 0), MX(M_H1), MX(M_RH1));
         const double ax = MX(M_AS);
         const double vu_e = V(1, 0, 0) + ax * (V(0, 0, 0) - V(1, 0, 0));
         const double vu_w = V(1, -1, 0) + ax * (V(0, -1, 0) - V(1, -1, 0));
@@ -215,12 +239,16 @@ __device__ __forceinline__ void momentum_cell(
         const double d2z = MZ(M_CC) * (dv<FAST>(-u12, MZ(M_H1), MZ(M_RH1)) + dv<FAST>(u10, MZ(M_H0), MZ(M_RH0)));
         const double xtt2 = rev * ((d2x + d2y) + d2z);
         const double res = ((-a - b) - d) + xtt2;
-        ut[c] = U(0, 0, 0) + deltat * (res + dpdx);
-        ru[c] = res;
+        AT(ut) = U(0, 0, 0) + deltat * (res + dpdx);
+        AT(ru) = res;
     }
     A.split();
     // V momentum (subNavierStokes.f90:901-1006)
     {
+        const double v4 = V(-1, 0, 0) - V(0, 0, 0), v6 = V(0, 0, 0) - V(1, 0, 0);
+        const double v8 = V(0, -1, 0) - V(0, 0, 0), v10 = V(0, 0, 0) - V(0, 1, 0);
+        const double v12 = V(0, 0, -1) - V(0, 0, 0), v14 = V(0, 0, 0) - V(0, 0, 1);
+        const double dpdy = dv<FAST>(P(0, 0, 0) - P(0, 1, 0), MY(M_H1), MY(M_RH1));
         const double ay = MY(M_AS);
         const double uv_e = U(0, 1, 0) + ay * (U(0, 0, 0) - U(0, 1, 0));
         const double uv_w = U(-1, 1, 0) + ay * (U(-1, 0, 0) - U(-1, 1, 0));
@@ -247,12 +275,16 @@ __device__ __forceinline__ void momentum_cell(
         const double d2z = MZ(M_CC) * (dv<FAST>(-v14, MZ(M_H1), MZ(M_RH1)) + dv<FAST>(v12, MZ(M_H0), MZ(M_RH0)));
         const double ytt2 = rev * ((d2x + d2y) + d2z);
         const double res = ((-a - b) - d) + ytt2;
-        vt[c] = V(0, 0, 0) + deltat * (res + dpdy);
-        rv[c] = res;
+        AT(vt) = V(0, 0, 0) + deltat * (res + dpdy);
+        AT(rv) = res;
     }
     A.split();
     // W momentum (subNavierStokes.f90:1010-1116)
     {
+        const double w6 = W(-1, 0, 0) - W(0, 0, 0), w8 = W(0, 0, 0) - W(1, 0, 0);
+        const double w10 = W(0, -1, 0) - W(0, 0, 0), w12 = W(0, 0, 0) - W(0, 1, 0);
+        const double w14 = W(0, 0, -1) - W(0, 0, 0), w16 = W(0, 0, 0) - W(0, 0, 1);
+        const double dpdz = dv<FAST>(P(0, 0, 0) - P(0, 0, 1), MZ(M_H1), MZ(M_RH1));
         const double az = MZ(M_AS);
         const double uw_e = U(0, 0, 1) + az * (U(0, 0, 0) - U(0, 0, 1));
         const double uw_w = U(-1, 0, 1) + az * (U(-1, 0, 0) - U(-1, 0, 1));
@@ -279,14 +311,15 @@ __device__ __forceinline__ void momentum_cell(
         const double d2z = MZ(M_CS) * (dv<FAST>(-w16, MZ(M_D1), MZ(M_RD1)) + dv<FAST>(w14, MZ(M_D0), MZ(M_RD0)));
         const double ztt2 = rev * ((d2x + d2y) + d2z);
         const double res = ((-a - b) - d) + ztt2;
-        wt[c] = W(0, 0, 0) + deltat * (res + dpdz);
-        rw[c] = res;
+        AT(wt) = W(0, 0, 0) + deltat * (res + dpdz);
+        AT(rw) = res;
     }
 #undef U
 #undef V
 #undef W
 #undef P
 #undef CL
+#undef AT
 #undef MX
 #undef MY
 #undef MZ
@@ -296,8 +329,11 @@ __device__ __forceinline__ void momentum_cell(
 // and the tags only.  A block is 64 x 4 with one wave per (j, k) row, so j
 // and k are wave-uniform: their table entries come through scalar loads and
 // the wall tests are scalar branches.
-template <bool FAST>
-__global__ void __launch_bounds__(256)
+// With 32-bit offsets and the fast divisions the kernel fits 96 registers:
+// five waves per SIMD instead of four (the other forms need more, and
+// keep the compiler's choice)
+template <bool FAST, bool OFF32>
+__global__ void __launch_bounds__(256, (FAST && OFF32) ? 5 : 1)
 k_momentum(const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
            const double *__restrict__ p, const int8_t *__restrict__ cell, double *__restrict__ ut,
            double *__restrict__ vt, double *__restrict__ wt, double *__restrict__ ru, double *__restrict__ rv,
@@ -307,8 +343,14 @@ k_momentum(const double *__restrict__ u, const double *__restrict__ v, const dou
     const int k = (int)blockIdx.z + 2;
     if (j > ny + 1 || i > nx - 1) return;
     const int64_t sx = nx + 2, sxy = sx * (ny + 2);
-    const GlobalFields A{u, v, w, (i - 1) + sx * ((j - 1) + (int64_t)(ny + 2) * (k - 1)), sx, sxy};
-    momentum_cell<FAST>(A, i, j, k, p, cell, ut, vt, wt, ru, rv, rw, T, M, nx, ny, nz, rev, deltat);
+    const int64_t c = (i - 1) + sx * ((j - 1) + (int64_t)(ny + 2) * (k - 1));
+    if constexpr (OFF32) {
+        const GlobalFields32 A{u, v, w, (uint32_t)c, (uint32_t)sx, (uint32_t)sxy};
+        momentum_cell<FAST>(A, i, j, k, p, cell, ut, vt, wt, ru, rv, rw, T, M, nx, ny, nz, rev, deltat);
+    } else {
+        const GlobalFields A{u, v, w, c, sx, sxy};
+        momentum_cell<FAST>(A, i, j, k, p, cell, ut, vt, wt, ru, rv, rw, T, M, nx, ny, nz, rev, deltat);
+    }
 }
 
 // The same interior columns, a 64 x 8 tile of (i, j) marching through a chunk
@@ -647,7 +689,7 @@ static int build_mom_tables(Ctx &c) {
 // the interior predictor from LDS tiles (default) or straight from HBM
 static int g_mom_lds = 0;
 int set_momentum_tiles(int on) {
-    IB3D_REQUIRE(on >= 0 && on <= 3, "set_momentum_tiles: variant must be 0..3");
+    IB3D_REQUIRE(on >= 0 && on <= 4, "set_momentum_tiles: variant must be 0..4");
     g_mom_lds = on;
     return 0;
 }
@@ -667,6 +709,8 @@ int ns_momentum(Ctx &c, double rev, double deltat) {
     const dim3 blk(64, 4, 1);
     const int nxi = std::max(0, nx - 3);   // interior columns 3..nx-1
     const int nedge = 3 * ny * nz;
+    // 32-bit byte offsets need the FP64 fields to span less than 4 GiB
+    const bool off32 = c.P * (int64_t)sizeof(double) < ((int64_t)1 << 32);
     auto launch = [&](auto fast) {
         constexpr bool F = decltype(fast)::value;
         if (nxi > 0 && g_mom_lds == 1)
@@ -681,8 +725,12 @@ int ns_momentum(Ctx &c, double rev, double deltat) {
             hipLaunchKernelGGL((k_momentum_lds<F, 8, 3>), dim3(nblocks(nxi, kMomTX), nblocks(ny, 8), nblocks(nz, kMomTZ)),
                                dim3(kMomTX, 8, 1), 0, c.stream, c.u, c.v, c.w, c.p, c.tag, c.ut, c.vt, c.wt,
                                c.resi_u, c.resi_v, c.resi_w, T, M, nx, ny, nz, rev, deltat);
+        else if (nxi > 0 && g_mom_lds == 0 && off32)
+            hipLaunchKernelGGL((k_momentum<F, true>), dim3(nblocks(nxi, 64), nblocks(ny, 4), (unsigned)nz), blk, 0,
+                               c.stream, c.u, c.v, c.w, c.p, c.tag, c.ut, c.vt, c.wt, c.resi_u, c.resi_v, c.resi_w, T,
+                               M, nx, ny, nz, rev, deltat);
         else if (nxi > 0)
-            hipLaunchKernelGGL(k_momentum<F>, dim3(nblocks(nxi, 64), nblocks(ny, 4), (unsigned)nz), blk, 0,
+            hipLaunchKernelGGL((k_momentum<F, false>), dim3(nblocks(nxi, 64), nblocks(ny, 4), (unsigned)nz), blk, 0,
                                c.stream, c.u, c.v, c.w, c.p, c.tag, c.ut, c.vt, c.wt, c.resi_u, c.resi_v, c.resi_w, T,
                                M, nx, ny, nz, rev, deltat);
         hipLaunchKernelGGL(k_momentum_edge<F>, dim3(nblocks(nedge, 256)), dim3(256), 0, c.stream, c.u, c.v, c.w,

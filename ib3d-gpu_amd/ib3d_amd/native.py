@@ -312,7 +312,8 @@ class Solver:
         _ok(self.L.ib3d_set_coarse_tiles(int(from_level)))
 
     def set_momentum_tiles(self, on=True):
-        """nsMomentum's interior from LDS tiles (on) or straight from HBM (off);
+        """nsMomentum's interior kernel: 0 straight from HBM (the default), 1-3
+        from LDS tiles marching in z, 4 straight from HBM with 64-bit indices;
         identical results."""
         _ok(self.L.ib3d_set_momentum_tiles(int(on)))
 

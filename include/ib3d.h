@@ -201,9 +201,11 @@ int ib3d_set_coarse_tiles(int from_level);
  * correct).  drop_interior != 0 checks the guard without its interior
  * test instead, which reads outside. */
 int ib3d_selfcheck_resid_guard(int64_t nx, int64_t ny, int64_t nz, int drop_interior, int64_t *outside);
-/* nsMomentum: the interior columns from 64 x 8 LDS tiles marching in z
- * (on != 0) or with every stencil value read from HBM (on == 0, the default).
- * Both give identical results. */
+/* nsMomentum's interior columns: on == 0 (the default) reads every stencil
+ * value from HBM through 32-bit byte offsets (64-bit indices when the FP64
+ * fields span 4 GiB or more); 1, 2, 3 march LDS tiles of 64 x 8, 64 x 4 and
+ * 64 x 8 columns (3 and 4 waves per SIMD) in z; 4 reads from HBM with
+ * 64-bit indices.  All give identical results. */
 int ib3d_set_momentum_tiles(int on);
 /* modes 3 and 4: replay each refinement iteration (Krylov cycle, FP64
  * residual and its norm) as one captured HIP graph (on != 0) or launch its

@@ -1,9 +1,10 @@
 """nsMomentum with its interior columns from LDS tiles marching in z (64 x 8
 or 64 x 4 columns; each plane of u, v, w staged once with a two-cell ring,
-each column's planes k-2..k+2 in registers) against the kernel that reads
-every stencil value from HBM: the same arithmetic in the same order, so ut,
-vt, wt and the residual arrays must be bit-identical, also against the
-oracle, and on fields with extreme values."""
+each column's planes k-2..k+2 in registers) and the direct kernel with
+64-bit indices (variant 4) against the default direct kernel (32-bit byte
+offsets): the same arithmetic in the same order, so ut, vt, wt and the
+residual arrays must be bit-identical, also against the oracle, and on
+fields with extreme values."""
 import numpy as np
 import pytest
 
@@ -26,7 +27,7 @@ def test_tiled_predictor_is_bitwise_the_direct_one(case_dir, name):
     try:
         for step in range(3):
             a = _predict(s, 0)
-            for variant in (1, 2, 3):
+            for variant in (1, 2, 3, 4):
                 b = _predict(s, variant)
                 for f in OUT:
                     assert_bits(f"{name} step {step} variant {variant} {f}", b[f], a[f])
@@ -54,7 +55,7 @@ def test_tiled_predictor_matches_oracle_on_extreme_values(case_dir):
             s.upload(f, x)
             o.array(f)[:] = x
         o.call("ns_momentum")
-        for variant in (1, 2, 3):
+        for variant in (0, 1, 2, 3, 4):
             s.set_momentum_tiles(variant)
             s.nsMomentum()
             for f in OUT:

@@ -989,6 +989,7 @@ int ib3d_set_fused_sweeps(int levels, int planes, int blocks, int coarse_blocks)
     return set_fused_sweeps(levels, planes, blocks, coarse_blocks);
 }
 int ib3d_set_coarse_tiles(int from_level) { return set_coarse_tiles(from_level); }
+int ib3d_set_fused_restrict(int levels) { return set_fused_restrict(levels); }
 
 int ib3d_selfcheck_resid_guard(int64_t nx, int64_t ny, int64_t nz, int drop_interior, int64_t *outside) {
     IB3D_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && outside, "selfcheck_resid_guard: bad grid or output");

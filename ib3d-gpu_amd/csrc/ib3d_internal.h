@@ -282,6 +282,7 @@ int compute_div_noalloc(Ctx &c, double deltat, double *divmax);
 int noalloc_setup(int device);
 int set_fused_sweeps(int on, int tz, int blocks, int coarse_blocks);
 int set_coarse_tiles(int from_level);
+int set_fused_restrict(int levels);
 int64_t resid_guard_selfcheck(int64_t nx, int64_t ny, int64_t nz, bool drop_interior);
 int set_momentum_tiles(int on);
 int redblack_sor_reference(Ctx &c, double epsi, int64_t *isum, double *derr2);

@@ -107,10 +107,22 @@ __device__ __forceinline__ void block_sum_to(double v, double *partial, int64_t 
 // skip this launch when the previous iteration already converged
 __device__ __forceinline__ bool gated(const double *gate, double thr) { return gate && *gate < thr; }
 
-__global__ void k_max_partials(const double *partial, int64_t n, double *out) {
+// One 1024-thread workgroup reduces n partials (tens of thousands after a
+// full-grid kernel): each thread takes q = t, t + 1024, ... in order, with
+// eight loads in flight ahead of its running value, then a tree.
+__global__ void __launch_bounds__(1024) k_max_partials(const double *partial, int64_t n, double *out) {
     __shared__ double sm[1024];
     double m = 0.0;
-    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) m = nan_max(m, partial[q]);
+    const int64_t st = blockDim.x;
+    int64_t q = threadIdx.x;
+    for (; q + 7 * st < n; q += 8 * st) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = partial[q + u * st];
+#pragma unroll
+        for (int u = 0; u < 8; u++) m = nan_max(m, v[u]);
+    }
+    for (; q < n; q += st) m = nan_max(m, partial[q]);
     sm[threadIdx.x] = m;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -124,10 +136,20 @@ void launch_max_partials(const double *partial, int64_t n, double *out) {
     hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(1024), 0, ctx().stream, partial, n, out);
 }
 
-__global__ void k_sum_partials(const double *partial, int64_t n, double *out) {
+// the same for a sum; the additions keep the plain loop's order
+__global__ void __launch_bounds__(1024) k_sum_partials(const double *partial, int64_t n, double *out) {
     __shared__ double sm[1024];
     double s = 0.0;
-    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) s += partial[q];
+    const int64_t st = blockDim.x;
+    int64_t q = threadIdx.x;
+    for (; q + 7 * st < n; q += 8 * st) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = partial[q + u * st];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; q < n; q += st) s += partial[q];
     sm[threadIdx.x] = s;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -1724,6 +1746,116 @@ k_mg_resid_restrict(MgView F, MgView C, SorGeom gc, const double *gate, double t
     C.b[q] = restrict_at<STORED>(F, C, cc, fc, I, J, K);
 }
 
+// The same restriction in one pass over the fine level, with no stored
+// residual: a thread per coarse column (I, J), marching a chunk of coarse
+// planes K.  Per fine plane it reads the rows of its four children
+// (ii = 2I-1, 2I; jj = 2J-1, 2J), their x and y neighbours and b: across a
+// wave every such read is one colour array at consecutive slots (the colour
+// of ii = 2I-1 + a does not depend on I), so the rows coalesce.  The x values
+// of the two fine planes below come along in registers.  The residuals use
+// cell_resid's operations and the children are summed in restrict_at's order,
+// so the coarse right-hand side is bit for bit the two-pass one.  Every
+// coarse slot is written (NaN outside the interior), as k_mg_resid_restrict
+// does.  kch: coarse planes per chunk; dW divides by 2*C.H, the coarse cells
+// of a padded row.
+__global__ void __launch_bounds__(kSorBlock)
+k_resid_restrict_pz(MgView F, MgView C, FastDiv dW, int kch, const double *gate, double thr) {
+    if (gated(gate, thr)) return;
+    int bx, by;
+    xcd_block2(bx, by);
+    const int W = 2 * C.H;
+    const int t = bx * kSorBlock + (int)threadIdx.x;
+    if (t >= W * C.sy) return;
+    const int J = (int)fdiv((uint32_t)t, dW);
+    const int I = t - J * W;
+    const int K0 = 1 + by * kch, K1 = min(K0 + kch, C.sz - 1);
+    auto cslot = [&](int K) { return ((I + J + K) & 1) * C.NC + (J + C.sy * K) * C.H + (I >> 1); };
+    const float nan = __builtin_nanf("");
+    if (K0 == 1) C.b[cslot(0)] = nan;
+    if (K1 == C.sz - 1) C.b[cslot(C.sz - 1)] = nan;
+    if (!(I >= 1 && I <= C.nx && J >= 1 && J <= C.ny)) {
+        for (int K = K0; K < K1; K++) C.b[cslot(K)] = nan;
+        return;
+    }
+    typedef __attribute__((address_space(4))) float CFloat;
+    const CFloat *az = (const CFloat *)F.az;
+    // x of fine cell (2I-1+a, jj, kk), a = -1..2: colour e = (jj+kk)&1 holds
+    // a = -1 and 1 at slots I-1 and I, colour 1-e holds a = 0 and 2
+    auto rowbase = [&](int jj, int kk) { return (jj + F.sy * kk) * F.H + I; };  // slot of ii = 2I
+    struct Pair {
+        float c0, c1;  // ii = 2I-1, 2I (the children), or 2I-2, 2I+1 (their x neighbours)
+    };
+    auto load_we = [&](int jj, int kk) {
+        const int f = rowbase(jj, kk), e = (jj + kk) & 1;
+        return Pair{F.x[(int64_t)e * F.NC + f - 1], F.x[(int64_t)(1 - e) * F.NC + f]};
+    };
+    auto load_pair = [&](const float *a, int jj, int kk) {
+        const int f = rowbase(jj, kk), e = (jj + kk) & 1;
+        return Pair{a[(int64_t)(1 - e) * F.NC + f - 1], a[(int64_t)e * F.NC + f]};
+    };
+    const int i0 = 2 * I - 1, j0 = 2 * J - 1;
+    const float a1x0 = F.ax[i0], a2x0 = F.ax[F.sxp + i0], a3x0 = F.ax[2 * F.sxp + i0];
+    const float a1x1 = F.ax[i0 + 1], a2x1 = F.ax[F.sxp + i0 + 1], a3x1 = F.ax[2 * F.sxp + i0 + 1];
+    const float a1y0 = F.ay[j0], a2y0 = F.ay[F.sy + j0], a3y0 = F.ay[2 * F.sy + j0];
+    const float a1y1 = F.ay[j0 + 1], a2y1 = F.ay[F.sy + j0 + 1], a3y1 = F.ay[2 * F.sy + j0 + 1];
+    // cell_resid, operand for operand
+    auto resid = [](float rb, float a1x, float a2x, float a3x, float a1y, float a2y, float a3y, float a1z, float a2z,
+                    float a3z, float w, float e, float s, float n, float b, float a, float own) {
+        const float ax = (((((a1x * w + a3x * e) + a1y * s) + a3y * n) + a1z * b) + a3z * a) + ((a2x + a2y) + a2z) * own;
+        return rb - ax;
+    };
+    // the children's x on the two fine planes below the current pair of
+    // planes (ka = 2K-1, kb = 2K), rows j0 and j0 + 1
+    Pair m0 = load_pair(F.x, j0, 2 * K0 - 2), m1 = load_pair(F.x, j0 + 1, 2 * K0 - 2);
+    Pair a0 = load_pair(F.x, j0, 2 * K0 - 1), a1 = load_pair(F.x, j0 + 1, 2 * K0 - 1);
+    for (int K = K0; K < K1; K++) {
+        const int ka = 2 * K - 1, kb = 2 * K;
+        const Pair wa0 = load_we(j0, ka), wa1 = load_we(j0 + 1, ka);
+        const Pair wb0 = load_we(j0, kb), wb1 = load_we(j0 + 1, kb);
+        const Pair b0 = load_pair(F.x, j0, kb), b1 = load_pair(F.x, j0 + 1, kb);
+        const Pair sa = load_pair(F.x, j0 - 1, ka), na = load_pair(F.x, j0 + 2, ka);
+        const Pair sb = load_pair(F.x, j0 - 1, kb), nb = load_pair(F.x, j0 + 2, kb);
+        const Pair c0 = load_pair(F.x, j0, kb + 1), c1 = load_pair(F.x, j0 + 1, kb + 1);
+        const Pair ba0 = load_pair(F.b, j0, ka), ba1 = load_pair(F.b, j0 + 1, ka);
+        const Pair bb0 = load_pair(F.b, j0, kb), bb1 = load_pair(F.b, j0 + 1, kb);
+        const float a1za = az[ka], a2za = az[F.sz + ka], a3za = az[2 * F.sz + ka];
+        const float a1zb = az[kb], a2zb = az[F.sz + kb], a3zb = az[2 * F.sz + kb];
+        float s = 0.0f;
+        int n = 0;
+        auto add = [&](float rb, float r) {
+            if (rb == rb) {
+                s += r;
+                n++;
+            }
+        };
+        // plane ka (children a0, a1): below m, above b; west/east: the other
+        // child of the pair or the x neighbour; south/north: the other row or
+        // the y neighbour
+        add(ba0.c0, resid(ba0.c0, a1x0, a2x0, a3x0, a1y0, a2y0, a3y0, a1za, a2za, a3za, wa0.c0, a0.c1, sa.c0, a1.c0,
+                          m0.c0, b0.c0, a0.c0));
+        add(ba0.c1, resid(ba0.c1, a1x1, a2x1, a3x1, a1y0, a2y0, a3y0, a1za, a2za, a3za, a0.c0, wa0.c1, sa.c1, a1.c1,
+                          m0.c1, b0.c1, a0.c1));
+        add(ba1.c0, resid(ba1.c0, a1x0, a2x0, a3x0, a1y1, a2y1, a3y1, a1za, a2za, a3za, wa1.c0, a1.c1, a0.c0, na.c0,
+                          m1.c0, b1.c0, a1.c0));
+        add(ba1.c1, resid(ba1.c1, a1x1, a2x1, a3x1, a1y1, a2y1, a3y1, a1za, a2za, a3za, a1.c0, wa1.c1, a0.c1, na.c1,
+                          m1.c1, b1.c1, a1.c1));
+        // plane kb (children b0, b1): below a, above c
+        add(bb0.c0, resid(bb0.c0, a1x0, a2x0, a3x0, a1y0, a2y0, a3y0, a1zb, a2zb, a3zb, wb0.c0, b0.c1, sb.c0, b1.c0,
+                          a0.c0, c0.c0, b0.c0));
+        add(bb0.c1, resid(bb0.c1, a1x1, a2x1, a3x1, a1y0, a2y0, a3y0, a1zb, a2zb, a3zb, b0.c0, wb0.c1, sb.c1, b1.c1,
+                          a0.c1, c0.c1, b0.c1));
+        add(bb1.c0, resid(bb1.c0, a1x0, a2x0, a3x0, a1y1, a2y1, a3y1, a1zb, a2zb, a3zb, wb1.c0, b1.c1, b0.c0, nb.c0,
+                          a1.c0, c1.c0, b1.c0));
+        add(bb1.c1, resid(bb1.c1, a1x1, a2x1, a3x1, a1y1, a2y1, a3y1, a1zb, a2zb, a3zb, b1.c0, wb1.c1, b0.c1, nb.c1,
+                          a1.c1, c1.c1, b1.c1));
+        C.b[cslot(K)] = n ? s / (float)n : nan;
+        m0 = b0;
+        m1 = b1;
+        a0 = c0;
+        a1 = c1;
+    }
+}
+
 // x_fine += trilinear interpolant of x_coarse, on active fine cells
 __global__ void __launch_bounds__(kSorBlock)
 k_mg_prolong(MgView F, MgView C, SorGeom gf, const double *gate, double thr) {
@@ -2237,6 +2369,7 @@ static dim3 level_grid(const MgLevel &L) {
 // chunk so that the grid holds about g_sweep_blocks workgroups.
 static int g_fused = 2;          // levels (from the finest) smoothed by whole sweeps in one launch
 static int g_coarse_tiles = 2;   // first level smoothed by two temporally blocked launches per phase (0: none)
+static int g_fused_restrict = 64;  // levels (from the finest) whose residual is restricted in one pass
 static int g_sweep_tz = 0;        // planes per chunk on the finest level (0: from g_sweep_blocks)
 static int g_sweep_blocks = 480;
 static int g_sweep_blocks_coarse = 192;  // workgroup target below the finest level
@@ -2346,7 +2479,13 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
     for (int l = 0; l < top; l++) {
         const MgLevel &F = g_mg[l], &C = g_mg[l + 1];
         mg_smooth(c, F, l == 0, l == 0 ? c.mg_nu1 : c.mg_nu1c, true, c.mg_omega, gate);
-        if (l == 0) {
+        if (l < g_fused_restrict) {
+            const int W = (int)(2 * C.H), nbx = nblocks(W * C.sy, kSorBlock), nk = (int)C.sz - 2;
+            // coarse planes per chunk: about 1536 workgroups in all
+            const int kch = std::max(1, std::min(nk, (nk * nbx + 1535) / 1536));
+            hipLaunchKernelGGL(k_resid_restrict_pz, dim3(nbx, (nk + kch - 1) / kch), tb, 0, c.stream, level_view(F),
+                               level_view(C), make_fastdiv(W), kch, gate, 0.5);
+        } else if (l == 0) {
             hipLaunchKernelGGL(k_matvec32_pz<true>, level_grid(F), tb, 0, c.stream, F.x, F.x + F.NC, F.b,
                                F.b + F.NC, F.r, F.r + F.NC, F.ax, F.ay, F.az, level_geom(F), gate, 0.5);
             hipLaunchKernelGGL(k_mg_resid_restrict<true>, dim3(nblocks(2 * C.NC, kSorBlock)), tb, 0, c.stream,
@@ -2558,6 +2697,7 @@ static int cycle_graph(Ctx &c, int m, int k, double *relv, double *conv) {
     key.push_back((uint64_t)c.mg_nu2c);
     key.push_back((uint64_t)g_fused);
     key.push_back((uint64_t)g_coarse_tiles);
+    key.push_back((uint64_t)g_fused_restrict);
     key.push_back((uint64_t)g_sweep_tz);
     key.push_back((uint64_t)g_sweep_blocks);
     key.push_back((uint64_t)g_sweep_blocks_coarse);
@@ -2725,6 +2865,12 @@ int set_fused_sweeps(int on, int tz, int blocks, int coarse_blocks) {
 int set_coarse_tiles(int from_level) {
     IB3D_REQUIRE(from_level >= 0 && from_level <= 64, "set_coarse_tiles: bad level");
     g_coarse_tiles = from_level;
+    return 0;
+}
+
+int set_fused_restrict(int levels) {
+    IB3D_REQUIRE(levels >= 0 && levels <= 64, "set_fused_restrict: bad level count");
+    g_fused_restrict = levels;
     return 0;
 }
 

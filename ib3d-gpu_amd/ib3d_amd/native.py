@@ -70,6 +70,7 @@ def lib():
         "ib3d_set_krylov_target": [D],
         "ib3d_set_fused_sweeps": [I, I, I, I],
         "ib3d_set_coarse_tiles": [I],
+        "ib3d_set_fused_restrict": [I],
         "ib3d_selfcheck_resid_guard": [I64, I64, I64, I, PI64],
         "ib3d_set_momentum_tiles": [I],
         "ib3d_set_graphs": [I],
@@ -310,6 +311,11 @@ class Solver:
         if from_level is True:
             from_level = 2
         _ok(self.L.ib3d_set_coarse_tiles(int(from_level)))
+
+    def set_fused_restrict(self, levels=64):
+        """Residual and restriction in one z-marching launch on the first
+        `levels` multigrid levels (0: none); identical results."""
+        _ok(self.L.ib3d_set_fused_restrict(int(levels)))
 
     def set_momentum_tiles(self, on=True):
         """nsMomentum's interior kernel: 0 straight from HBM (the default), 1-3

@@ -194,6 +194,12 @@ int ib3d_set_fused_sweeps(int levels, int planes, int blocks, int coarse_blocks)
  * the second (n = 2..4), instead of 2n half-sweep launches.  Both give
  * identical results. */
 int ib3d_set_coarse_tiles(int from_level);
+/* mode 4: on the first `levels` multigrid levels (64, the default: every
+ * level cycled by launches; 0 for none) form the residual and restrict it to
+ * the next level's right-hand side in one z-marching launch, instead of
+ * storing the residual (finest level) or gathering each child's stencil per
+ * coarse slot (below).  Both give identical results. */
+int ib3d_set_fused_restrict(int levels);
 /* No counterpart in the reference; needs no GPU.  Host self-check of the
  * residual kernel's guard on an nx x ny x nz grid: with a finite b/dt in
  * every slot (an overwritten NaN sentinel), counts the neighbour and

@@ -1143,11 +1143,32 @@ __global__ void k_act_init(double *act, int m, const double *gate, double thr) {
     if (t <= m) act[t] = (t == 0 && !gated(gate, thr)) ? 1.0 : 0.0;
 }
 
-__global__ void k_givens(double *__restrict__ H, int m, int j, double *cs, double *sn, double *gv, const double *beta,
-                         double *act, double target, const double *targetp) {
-    if (act[j] < 0.5 || threadIdx.x != 0) return;
-    if (targetp) target = *targetp;
+// The step's new norm h[j+1] = ||w|| comes first, from the orthogonalisation's
+// nblk block partials of w^2: 256 threads, each summing partial[t],
+// partial[t+256], ... in order, then a pairwise tree, then the square root.
+// It is also kept in nrm[j] for the scaling of the new basis vector, which
+// runs after the rotation has cleared h[j+1].
+__global__ void __launch_bounds__(256)
+k_givens(double *__restrict__ H, int m, int j, double *cs, double *sn, double *gv, const double *beta,
+         double *act, double target, const double *targetp, const double *__restrict__ partial, int nblk,
+         double *nrm) {
+    if (act[j] < 0.5) return;
     double *h = H + (int64_t)(m + 1) * j;
+    {
+        __shared__ double sm[256];
+        double s = 0.0;
+        for (int q = threadIdx.x; q < nblk; q += 256) s += partial[q];
+        sm[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x != 0) return;
+        h[j + 1] = sqrt(sm[0]);
+        nrm[j] = h[j + 1];
+    }
+    if (targetp) target = *targetp;
     if (j == 0) gv[0] = *beta;
     for (int i = 0; i < j; i++) {
         const double t = cs[i] * h[i] + sn[i] * h[i + 1];
@@ -2527,8 +2548,8 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
 }
 
 
-// g_kh: H ((m+1) x m, column-major), beta, y[m], cs[m], sn[m], g[m+1], act[m+1]
-static int64_t kh_size(int m) { return (int64_t)(m + 1) * m + 1 + 3 * m + 2 * (m + 1); }
+// g_kh: H ((m+1) x m, column-major), beta, y[m], cs[m], sn[m], g[m+1], act[m+1], nrm[m]
+static int64_t kh_size(int m) { return (int64_t)(m + 1) * m + 1 + 3 * m + 2 * (m + 1) + m; }
 
 static int gmres_buffers(Ctx &c, int m) {
     if (g_kv && g_k_nc == c.NC && g_k_m == m) return 0;
@@ -2569,7 +2590,7 @@ static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target,
     const SorGeom g = geom(c);
     const int64_t NC = c.NC, stride = 2 * NC, n2 = NC;  // n2: float pairs per vector
     double *H = g_kh, *beta = H + (int64_t)(m + 1) * m, *y = beta + 1, *cs = y + m, *sn = cs + m, *gv = sn + m,
-           *act = gv + (m + 1);
+           *act = gv + (m + 1), *nrm = act + (m + 1);
     const dim3 vb(kVecBlocks), tb(kSorBlock);
     const float om = (float)c.omega, omm = (float)(1.0 - c.omega);
     float *V0 = g_kv;
@@ -2602,11 +2623,10 @@ static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target,
         launch_dots(j + 1, vb, tb, c.stream, Vn, g_kv, stride, n2 / 2, g_kpart, aj);
         hipLaunchKernelGGL(k_rows_sum, dim3(j + 1), dim3(256), 0, c.stream, g_kpart, kVecBlocks, hj, 0, aj, 0.5);
         launch_orth(j + 1, vb, tb, c.stream, Vn, g_kv, stride, hj, n2 / 2, g_kpart2, aj);
-        hipLaunchKernelGGL(k_sqrt_sum, dim3(1), dim3(256), 0, c.stream, g_kpart2, kVecBlocks, hj + j + 1, aj, 0.5);
-        if (j + 1 < m)  // (before k_givens, which rotates h[j+1] to 0)
-            hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, Vn, Vn, hj + j + 1, n2, aj, 0.5);
-        hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, c.stream, H, m, j, cs, sn, gv, beta, act, target,
-                           targetp);
+        // the new norm (from orth's partials) and the rotation in one launch
+        hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c.stream, H, m, j, cs, sn, gv, beta, act, target,
+                           targetp, (const double *)g_kpart2, kVecBlocks, nrm);
+        if (j + 1 < m) hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, Vn, Vn, nrm + j, n2, aj, 0.5);
     }
     hipLaunchKernelGGL(k_lsq, dim3(1), dim3(64), 0, c.stream, H, m, gv, act, y, c.scalars + 24);
     hipLaunchKernelGGL(k_update_pc, vb, tb, 0, c.stream, c.pc[0], c.pc[1], g_kz, stride, m, y, V0, NC, act, 0.5);

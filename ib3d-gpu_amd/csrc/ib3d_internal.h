@@ -177,6 +177,7 @@ struct Ctx {
     // statistics of the last Poisson solve
     int64_t last_sweeps = 0;
     int64_t last_outer = 0;
+    int64_t last_cyc1 = 0;  // Arnoldi steps of the previous solve's first cycle (0: unknown)
     int64_t geom_version = 0;      // bumped whenever cellCount rebuilds the lists (the tags changed)
     double last_relres = 0.0, last_r0 = 0.0;
     double last_sor_ms = 0.0, last_poisson_ms = 0.0;

@@ -832,8 +832,10 @@ k_resid_pz(const double *__restrict__ p0, const double *__restrict__ p1, const f
 // start keeps the previous solution, so cells that stopped being fluid
 // (a moving body) are cleared before every solve.
 __global__ void __launch_bounds__(kSorBlock)
-k_clear_dead(double *p0, double *p1, float *e0, float *e1, const double *b0, const double *b1, int64_t NC) {
+k_clear_dead(double *p0, double *p1, float *e0, float *e1, const double *b0, const double *b1, int64_t NC,
+             double *zero_slot = nullptr) {
     const int64_t f2 = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
+    if (zero_slot && f2 == 0) *zero_slot = 0.0;
     if (f2 >= 2 * NC) return;
     const int colour = f2 >= NC;
     const int64_t f = colour ? f2 - NC : f2;
@@ -2586,7 +2588,16 @@ void gmres_release() {
 // sweeps (mode 3) or one multigrid V-cycle (mode 4), stopping early once the
 // least-squares residual is at most `target`, then pc += Z y in FP64.  All
 // gated on the previous step's convergence flag.
-static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target, const double *targetp = nullptr) {
+//
+// check_from: after each Arnoldi step j >= check_from but the last, the host
+// reads act[j+1] and stops queueing the cycle once it is clear.  The steps
+// after convergence would exit at once, but each still costs the host the
+// launches of a V-cycle (about 40, several microseconds each); one round trip
+// costs far less.  The rest of the cycle (k_lsq, the update) is queued as
+// always and reads act[] itself.
+static constexpr int kActMirror = 62;  // c.hscal slot of the checked flag
+static int gmres_cycle(Ctx &c, int m, int k, const double *gate, double target, const double *targetp = nullptr,
+                       int check_from = 1 << 30) {
     const SorGeom g = geom(c);
     const int64_t NC = c.NC, stride = 2 * NC, n2 = NC;  // n2: float pairs per vector
     double *H = g_kh, *beta = H + (int64_t)(m + 1) * m, *y = beta + 1, *cs = y + m, *sn = cs + m, *gv = sn + m,
@@ -2627,9 +2638,16 @@ static void gmres_cycle(Ctx &c, int m, int k, const double *gate, double target,
         hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c.stream, H, m, j, cs, sn, gv, beta, act, target,
                            targetp, (const double *)g_kpart2, kVecBlocks, nrm);
         if (j + 1 < m) hipLaunchKernelGGL(k_scale_to, vb, tb, 0, c.stream, Vn, Vn, nrm + j, n2, aj, 0.5);
+        if (j >= check_from && j + 1 < m) {
+            IB3D_CHECK(hipMemcpyAsync(c.hscal + kActMirror, act + j + 1, sizeof(double), hipMemcpyDeviceToHost,
+                                      c.stream));
+            IB3D_CHECK(hipStreamSynchronize(c.stream));
+            if (!(c.hscal[kActMirror] >= 0.5)) break;
+        }
     }
     hipLaunchKernelGGL(k_lsq, dim3(1), dim3(64), 0, c.stream, H, m, gv, act, y, c.scalars + 24);
     hipLaunchKernelGGL(k_update_pc, vb, tb, 0, c.stream, c.pc[0], c.pc[1], g_kz, stride, m, y, V0, NC, act, 0.5);
+    return 0;
 }
 
 // Cells outside the solve must read 0 in the preconditioned vectors Z and
@@ -2732,7 +2750,7 @@ static int cycle_graph(Ctx &c, int m, int k, double *relv, double *conv) {
     cycle_graph_release();
     double *S = c.scalars + kCycSlot;
     IB3D_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    gmres_cycle(c, m, k, S + 3, 0.0, S + 6);
+    if (gmres_cycle(c, m, k, S + 3, 0.0, S + 6)) return -1;
     residual_async(c, false, true, 0.0, S + 1, S + 2, S + 3, S + 5);
     hipLaunchKernelGGL(k_cycle_end, dim3(1), dim3(64), 0, c.stream, S, relv, conv);
     hipGraph_t graph = nullptr;
@@ -2762,9 +2780,9 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
     const int64_t max_outer = std::max<int64_t>(1, c.max_sweeps / ((int64_t)m * k));
     if (ensure_slots(2 * (max_outer + 2))) return -1;
     double *relv = c.slots, *conv = c.slots + c.n_slots / 2;
-    IB3D_CHECK(dzero(c.scalars + 24, sizeof(double)));  // Arnoldi steps run in this solve
+    // (c.scalars[24]: the Arnoldi steps run in this solve, cleared here)
     hipLaunchKernelGGL(k_clear_dead, dim3(nblocks(2 * c.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream, c.pc[0],
-                       c.pc[1], c.e32[0], c.e32[1], c.bdt[0], c.bdt[1], c.NC);
+                       c.pc[1], c.e32[0], c.e32[1], c.bdt[0], c.bdt[1], c.NC, c.scalars + 24);
     const bool cnn = cnn_active();
     if (cnn) {
         g_bn = bn;
@@ -2793,7 +2811,12 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
                 continue;
             }
             const double *gate = conv + (o - 1);
-            gmres_cycle(c, m, k, gate, c.krylov_target * c.tol * bn, dnorm ? dnorm + 1 : nullptr);
+            // the first cycle is checked from the step before the one at
+            // which the previous solve's first cycle ended (if it ended
+            // early), later cycles (usually a step or two) after every step
+            const int from = o > 1 ? 0 : c.last_cyc1 >= 1 && c.last_cyc1 < m ? (int)c.last_cyc1 - 1 : 1 << 30;
+            if (gmres_cycle(c, m, k, gate, c.krylov_target * c.tol * bn, dnorm ? dnorm + 1 : nullptr, from))
+                return -1;
             residual_async(c, false, true, bn, relv + o, conv + o, gate, dnorm);
         }
         IB3D_CHECK(hipGetLastError());
@@ -2830,6 +2853,7 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
                 const double steps = *ps;
                 *sweeps = (int64_t)steps * k;
                 *outer = oc;
+                c.last_cyc1 = oc == 1 ? (int64_t)steps : oc > 1 ? m : 0;
                 *rel = pr[q];
                 if (cnn && cnn_after_solve(c, c.scalars + 16, c.scalars + 18)) return -1;
                 return 0;

@@ -864,14 +864,14 @@ __global__ void __launch_bounds__(kSorBlock)
 k_div(const double *__restrict__ ut, const double *__restrict__ vt, const double *__restrict__ wt,
       const int8_t *__restrict__ tag, const double *__restrict__ dx, const double *__restrict__ dy,
       const double *__restrict__ dz, double *__restrict__ b0, double *__restrict__ b1, int sx, int sy,
-      int sz, int H, double deltat, double *pmax, double *psum) {
+      int sz, int H, FastDiv dsx, FastDiv dsy, double deltat, double *pmax, double *psum) {
     const int P = sx * sy * sz;
     const int idx = (int)blockIdx.x * kSorBlock + (int)threadIdx.x;
     double am = 0.0, bb = 0.0;
     if (idx < P) {
-        const int row = idx / sx;
+        const int row = (int)fdiv((uint32_t)idx, dsx);
         const int ii = idx - row * sx;
-        const int kk = row / sy;
+        const int kk = (int)fdiv((uint32_t)row, dsy);
         const int jj = row - kk * sy;
         const int colour = (ii + jj + kk) & 1;
         const int64_t f = (int64_t)row * H + (ii >> 1);
@@ -1397,8 +1397,8 @@ int compute_div_noalloc(Ctx &c, double deltat, double *divmax) {
                  "computeDiv_noalloc: the scratch address of this device has not been looked up (ib3d_init)");
     double *sc = g_scratch[c.device];
     hipLaunchKernelGGL(k_div, dim3((unsigned)nb), dim3(kSorBlock), 0, c.stream, c.ut, c.vt, c.wt, c.tag, c.dx,
-                       c.dy, c.dz, c.bdt[0], c.bdt[1], (int)c.sx, (int)c.sy, (int)c.sz, (int)c.H, deltat, sc,
-                       sc + nb);
+                       c.dy, c.dz, c.bdt[0], c.bdt[1], (int)c.sx, (int)c.sy, (int)c.sz, (int)c.H,
+                       make_fastdiv((int)c.sx), make_fastdiv((int)c.sy), deltat, sc, sc + nb);
     launch_max_partials(sc, nb, sc + 2 * nb);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, c.stream, sc + nb, nb, sc + 2 * nb + 1);
     IB3D_CHECK(hipGetLastError());
@@ -1417,8 +1417,8 @@ int compute_div(Ctx &c, double deltat, double *divmax, double *bnorm2, bool wait
     if (ensure_partial(std::max<int64_t>(2 * (int64_t)nb, nblocks(2 * c.NC, kSorBlock)))) return -1;
     IB3D_CHECK(hipMemsetAsync(c.scalars, 0, 8 * sizeof(double), c.stream));
     hipLaunchKernelGGL(k_div, dim3(nb), dim3(kSorBlock), 0, c.stream, c.ut, c.vt, c.wt, c.tag, c.dx, c.dy,
-                       c.dz, c.bdt[0], c.bdt[1], (int)c.sx, (int)c.sy, (int)c.sz, (int)c.H, deltat, c.partial + nb,
-                       c.partial);
+                       c.dz, c.bdt[0], c.bdt[1], (int)c.sx, (int)c.sy, (int)c.sz, (int)c.H, make_fastdiv((int)c.sx),
+                       make_fastdiv((int)c.sy), deltat, c.partial + nb, c.partial);
     launch_max_partials(c.partial + nb, (int64_t)nb, c.scalars);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, c.stream, c.partial, (int64_t)nb, c.scalars + 1);
     IB3D_CHECK(hipGetLastError());

@@ -907,15 +907,48 @@ __global__ void __launch_bounds__(kSorBlock) k_copy_probe(const double4 *__restr
 // back (2*NC floats, NaN on cells the solve does not update); H, beta and y
 // live on the device, so a whole cycle runs without the host.
 
+// The cells the solve updates, one bit per slot: word w of colour c holds
+// slots 32w..32w+31 (bit s & 31 of word s >> 5), bits[c * nw + w], nw even
+// words per colour.  Built from the NaN sentinel of b/dt once per geometry;
+// every vector of the solve is NaN on exactly the other slots, so a kernel
+// that only tests for NaN can read the bits instead of a whole vector.
+__global__ void __launch_bounds__(kSorBlock)
+k_active_bits(const double *__restrict__ b0, const double *__restrict__ b1, int64_t NC, uint32_t *__restrict__ bits,
+              int64_t nw) {
+    const int64_t per = nw * 32;  // slots covered per colour (a multiple of 64)
+    const int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
+    if (q >= 2 * per) return;      // whole waves: 2*per is a multiple of 64
+    const int c = q >= per ? 1 : 0;
+    const int64_t s = q - c * per;
+    bool on = false;
+    if (s < NC) {
+        const double v = (c ? b1 : b0)[s];
+        on = v == v;
+    }
+    const uint64_t m = __ballot(on);
+    if ((threadIdx.x & 63) == 0) {
+        bits[c * nw + (s >> 5)] = (uint32_t)m;
+        bits[c * nw + (s >> 5) + 1] = (uint32_t)(m >> 32);
+    }
+}
+
+// the two flags of the aligned slot pair (f, f+1) of colour c
+__device__ __forceinline__ void active_pair(const uint32_t *bits, int64_t nw, int c, int f, bool &ok0, bool &ok1) {
+    const uint32_t w = bits[c * nw + (f >> 5)];
+    ok0 = (w >> (f & 31)) & 1u;
+    ok1 = (w >> ((f & 31) + 1)) & 1u;
+}
+
 // y = A z (FP32) on cell pairs marching in z, both colours per thread;
 // NaN where the mask array is NaN.  RESID: y = m - A z instead (the mask
-// array is then the right-hand side).
+// array is then the right-hand side).  bits (not RESID): the slot flags of
+// the mask array, read instead of it.
 template <bool RESID>
 __global__ void __launch_bounds__(kSorBlock)
 k_matvec32_pz(const float *__restrict__ z0, const float *__restrict__ z1, const float *__restrict__ m0,
               const float *__restrict__ m1, float *__restrict__ y0, float *__restrict__ y1,
               const float *__restrict__ axc, const float *__restrict__ ayc, const float *__restrict__ azc, SorGeom g,
-              const double *gate, double thr) {
+              const double *gate, double thr, const uint32_t *__restrict__ bits = nullptr, int64_t nw = 0) {
     if (gated(gate, thr)) return;
     int bx, by;
     xcd_block2(bx, by);
@@ -941,8 +974,15 @@ k_matvec32_pz(const float *__restrict__ z0, const float *__restrict__ z1, const 
 #pragma unroll
         for (int c = 0; c < 2; c++) {
             const int o = 1 - c;
-            const f2v mk = *reinterpret_cast<const f2v *>(mc_[c] + f);
-            const bool ok0 = mk.x == mk.x, ok1 = mk.y == mk.y;
+            f2v mk;
+            bool ok0, ok1;
+            if constexpr (RESID) {
+                mk = *reinterpret_cast<const f2v *>(mc_[c] + f);
+                ok0 = mk.x == mk.x;
+                ok1 = mk.y == mk.y;
+            } else {
+                active_pair(bits, nw, c, f, ok0, ok1);
+            }
             f2v out = {__builtin_nanf(""), __builtin_nanf("")};
             if (ok0 || ok1) {
                 const int q = (c + jj + kk) & 1;
@@ -1641,6 +1681,8 @@ static int sor_l2(Ctx &c, bool mixed, double bn, int64_t *sweeps, int64_t *outer
 static float *g_kv = nullptr, *g_kz = nullptr;
 static int64_t g_z_geom = -1;
 static double *g_kh = nullptr, *g_kpart = nullptr, *g_kpart2 = nullptr;
+static uint32_t *g_actbits = nullptr;  // the finest level's slot flags (k_active_bits)
+static int64_t g_actbits_nw = 0;       // words per colour
 static int64_t g_k_nc = 0;
 static int g_k_m = 0;
 
@@ -1897,7 +1939,8 @@ k_mg_prolong(MgView F, MgView C, SorGeom gf, const double *gate, double thr) {
 // the coarse x-columns h-1..h+3 (clamped to the padded range), so the y-z
 // interpolant is formed once per column (5 x 4 gathers instead of 4 x 8).
 __global__ void __launch_bounds__(kSorBlock)
-k_mg_prolong_pz(MgView F, MgView C, SorGeom gf, const double *gate, double thr) {
+k_mg_prolong_pz(MgView F, MgView C, SorGeom gf, const uint32_t *__restrict__ bits, int64_t nw, const double *gate,
+                double thr) {
     if (gated(gate, thr)) return;
     int bx, by;
     xcd_block2(bx, by);
@@ -1907,8 +1950,10 @@ k_mg_prolong_pz(MgView F, MgView C, SorGeom gf, const double *gate, double thr) 
     if (!(p < gf.plane && jj >= 1 && jj <= gf.sy - 2)) return;
     const int kk = 1 + by;
     const int f = p + kk * gf.plane;
-    const f2v b0 = *reinterpret_cast<const f2v *>(F.b + f), b1 = *reinterpret_cast<const f2v *>(F.b + F.NC + f);
-    const bool a0x = b0.x == b0.x, a0y = b0.y == b0.y, a1x = b1.x == b1.x, a1y = b1.y == b1.y;
+    // the active cells: the slot flags of F.b
+    bool a0x, a0y, a1x, a1y;
+    active_pair(bits, nw, 0, f, a0x, a0y);
+    active_pair(bits, nw, 1, f, a1x, a1y);
     if (!(a0x || a0y || a1x || a1y)) return;
     const MgAxis ay = F.ty[jj], az = F.tz[kk];
     const int J[2] = {ay.i0, ay.i1}, K[2] = {az.i0, az.i1};
@@ -2541,7 +2586,8 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
         const MgLevel &F = g_mg[l], &C = g_mg[l + 1];
         if (l == 0)
             hipLaunchKernelGGL(k_mg_prolong_pz, dim3(nblocks((F.H * F.sy) / 2, kSorBlock), (unsigned)(F.sz - 2)), tb,
-                               0, c.stream, level_view(F), level_view(C), level_geom(F), gate, 0.5);
+                               0, c.stream, level_view(F), level_view(C), level_geom(F), (const uint32_t *)g_actbits,
+                               g_actbits_nw, gate, 0.5);
         else
             hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(2 * F.NC, kSorBlock)), tb, 0, c.stream, level_view(F),
                                level_view(C), level_geom(F), gate, 0.5);
@@ -2577,6 +2623,9 @@ static int gmres_buffers(Ctx &c, int m) {
 
 void gmres_release() {
     if (g_kv) { (void)hipFree(g_kv); (void)hipFree(g_kz); (void)hipFree(g_kh); (void)hipFree(g_kpart); (void)hipFree(g_kpart2); }
+    if (g_actbits) (void)hipFree(g_actbits);
+    g_actbits = nullptr;
+    g_actbits_nw = 0;
     g_kv = g_kz = nullptr;
     g_kh = g_kpart = g_kpart2 = nullptr;
     g_k_nc = 0;
@@ -2629,7 +2678,7 @@ static int gmres_cycle(Ctx &c, int m, int k, const double *gate, double target, 
             }
         }
         hipLaunchKernelGGL(k_matvec32_pz<false>, pz_grid(c), tb, 0, c.stream, Z, Z + NC, V0, V0 + NC, Vn, Vn + NC,
-                           g_axf, g_ayf, g_azf, g, aj, 0.5);
+                           g_axf, g_ayf, g_azf, g, aj, 0.5, (const uint32_t *)g_actbits, g_actbits_nw);
         double *hj = H + (int64_t)(m + 1) * j;
         launch_dots(j + 1, vb, tb, c.stream, Vn, g_kv, stride, n2 / 2, g_kpart, aj);
         hipLaunchKernelGGL(k_rows_sum, dim3(j + 1), dim3(256), 0, c.stream, g_kpart, kVecBlocks, hj, 0, aj, 0.5);
@@ -2660,6 +2709,19 @@ static int clear_for_geometry(Ctx &c, int m) {
     IB3D_CHECK(dzero(g_kz, (int64_t)m * 2 * c.NC * sizeof(float)));
     for (size_t l = 1; l < g_mg.size(); l++)
         if (g_mg[l].x) IB3D_CHECK(dzero(g_mg[l].x, 2 * g_mg[l].NC * sizeof(float)));
+    // the slot flags of this geometry, from the NaN sentinel of b/dt (set
+    // by computeDiv from the tags)
+    const int64_t nw = (c.NC + 63) / 64 * 2;
+    if (nw != g_actbits_nw) {
+        if (g_actbits) (void)hipFree(g_actbits);
+        g_actbits = nullptr;
+        g_actbits_nw = 0;
+        IB3D_CHECK(hipMalloc(&g_actbits, 2 * nw * sizeof(uint32_t)));
+        g_actbits_nw = nw;
+    }
+    hipLaunchKernelGGL(k_active_bits, dim3(nblocks(2 * nw * 32, kSorBlock)), dim3(kSorBlock), 0, c.stream, c.bdt[0],
+                       c.bdt[1], c.NC, g_actbits, nw);
+    IB3D_CHECK(hipGetLastError());
     g_z_geom = c.geom_version;
     return 0;
 }
@@ -2718,7 +2780,7 @@ static int cycle_graph(Ctx &c, int m, int k, double *relv, double *conv) {
                           (const void *)c.az, (const void *)c.partial, (const void *)c.scalars, (const void *)relv,
                           (const void *)conv, (const void *)g_axf, (const void *)g_ayf, (const void *)g_azf,
                           (const void *)g_kv, (const void *)g_kz, (const void *)g_kh, (const void *)g_kpart,
-                          (const void *)g_kpart2})
+                          (const void *)g_kpart2, (const void *)g_actbits})
         P(q);
     for (const MgLevel &L : g_mg)
         for (const void *q : {(const void *)L.x, (const void *)L.b, (const void *)L.r, (const void *)L.ax,

@@ -827,20 +827,72 @@ k_resid_pz(const double *__restrict__ p0, const double *__restrict__ p1, const f
     block_sum_to(rr, partial, (int64_t)by * gridDim.x + bx);
 }
 
+// The cells the solve updates, one bit per slot: word w of colour c holds
+// slots 32w..32w+31 (bit s & 31 of word s >> 5), bits[c * nw + w], nw even
+// words per colour.  Built from the NaN sentinel of b/dt once per geometry;
+// every vector of the solve is NaN on exactly the other slots, so a kernel
+// that only tests for NaN can read the bits instead of a whole vector.
+__global__ void __launch_bounds__(kSorBlock)
+k_active_bits(const double *__restrict__ b0, const double *__restrict__ b1, int64_t NC, uint32_t *__restrict__ bits,
+              int64_t nw) {
+    const int64_t per = nw * 32;  // slots covered per colour (a multiple of 64)
+    const int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
+    if (q >= 2 * per) return;      // whole waves: 2*per is a multiple of 64
+    const int c = q >= per ? 1 : 0;
+    const int64_t s = q - c * per;
+    bool on = false;
+    if (s < NC) {
+        const double v = (c ? b1 : b0)[s];
+        on = v == v;
+    }
+    const uint64_t m = __ballot(on);
+    if ((threadIdx.x & 63) == 0) {
+        bits[c * nw + (s >> 5)] = (uint32_t)m;
+        bits[c * nw + (s >> 5) + 1] = (uint32_t)(m >> 32);
+    }
+}
+
+// the two flags of the aligned slot pair (f, f+1) of colour c
+__device__ __forceinline__ void active_pair(const uint32_t *bits, int64_t nw, int c, int f, bool &ok0, bool &ok1) {
+    const uint32_t w = bits[c * nw + (f >> 5)];
+    ok0 = (w >> (f & 31)) & 1u;
+    ok1 = (w >> ((f & 31) + 1)) & 1u;
+}
+
 // Cells the solve does not update (b/dt is NaN there) must hold pc = 0 and
 // e = 0: they are the Dirichlet values of their fluid neighbours.  A warm
 // start keeps the previous solution, so cells that stopped being fluid
 // (a moving body) are cleared before every solve.
 __global__ void __launch_bounds__(kSorBlock)
-k_clear_dead(double *p0, double *p1, float *e0, float *e1, const double *b0, const double *b1, int64_t NC,
-             double *zero_slot = nullptr) {
+k_clear_dead(double *p0, double *p1, float *e0, float *e1, const double *b0, const double *b1, int64_t NC) {
     const int64_t f2 = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
-    if (zero_slot && f2 == 0) *zero_slot = 0.0;
     if (f2 >= 2 * NC) return;
     const int colour = f2 >= NC;
     const int64_t f = colour ? f2 - NC : f2;
     const double bd = (colour ? b1 : b0)[f];
     if (bd != bd) {
+        (colour ? p1 : p0)[f] = 0.0;
+        (colour ? e1 : e0)[f] = 0.0f;
+    }
+}
+
+// The same from the slot flags: one thread per 32-slot word, visiting only
+// its clear bits (a few percent of the slots).  zero_slot (if given) is set
+// to 0 as well.
+__global__ void __launch_bounds__(kSorBlock)
+k_clear_dead_bits(double *p0, double *p1, float *e0, float *e1, const uint32_t *__restrict__ bits, int64_t nw,
+                  int64_t NC, double *zero_slot) {
+    const int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
+    if (zero_slot && q == 0) *zero_slot = 0.0;
+    if (q >= 2 * nw) return;
+    const int colour = q >= nw ? 1 : 0;
+    const int64_t base = (q - colour * nw) * 32;
+    uint32_t dead = ~bits[q];
+    while (dead) {
+        const int b = __builtin_ctz(dead);
+        dead &= dead - 1u;
+        const int64_t f = base + b;
+        if (f >= NC) break;  // bits run upward: the rest lie past the array too
         (colour ? p1 : p0)[f] = 0.0;
         (colour ? e1 : e0)[f] = 0.0f;
     }
@@ -906,38 +958,6 @@ __global__ void __launch_bounds__(kSorBlock) k_copy_probe(const double4 *__restr
 // zero, then adds d to the FP64 pc.  Vectors are two colour arrays back to
 // back (2*NC floats, NaN on cells the solve does not update); H, beta and y
 // live on the device, so a whole cycle runs without the host.
-
-// The cells the solve updates, one bit per slot: word w of colour c holds
-// slots 32w..32w+31 (bit s & 31 of word s >> 5), bits[c * nw + w], nw even
-// words per colour.  Built from the NaN sentinel of b/dt once per geometry;
-// every vector of the solve is NaN on exactly the other slots, so a kernel
-// that only tests for NaN can read the bits instead of a whole vector.
-__global__ void __launch_bounds__(kSorBlock)
-k_active_bits(const double *__restrict__ b0, const double *__restrict__ b1, int64_t NC, uint32_t *__restrict__ bits,
-              int64_t nw) {
-    const int64_t per = nw * 32;  // slots covered per colour (a multiple of 64)
-    const int64_t q = (int64_t)blockIdx.x * kSorBlock + threadIdx.x;
-    if (q >= 2 * per) return;      // whole waves: 2*per is a multiple of 64
-    const int c = q >= per ? 1 : 0;
-    const int64_t s = q - c * per;
-    bool on = false;
-    if (s < NC) {
-        const double v = (c ? b1 : b0)[s];
-        on = v == v;
-    }
-    const uint64_t m = __ballot(on);
-    if ((threadIdx.x & 63) == 0) {
-        bits[c * nw + (s >> 5)] = (uint32_t)m;
-        bits[c * nw + (s >> 5) + 1] = (uint32_t)(m >> 32);
-    }
-}
-
-// the two flags of the aligned slot pair (f, f+1) of colour c
-__device__ __forceinline__ void active_pair(const uint32_t *bits, int64_t nw, int c, int f, bool &ok0, bool &ok1) {
-    const uint32_t w = bits[c * nw + (f >> 5)];
-    ok0 = (w >> (f & 31)) & 1u;
-    ok1 = (w >> ((f & 31) + 1)) & 1u;
-}
 
 // y = A z (FP32) on cell pairs marching in z, both colours per thread;
 // NaN where the mask array is NaN.  RESID: y = m - A z instead (the mask
@@ -1246,13 +1266,13 @@ __global__ void k_lsq(const double *__restrict__ H, int m, const double *gv, con
     }
 }
 
-// pc += sum_j y_j z_j on cells the solve updates (mask not NaN); one pair of
+// pc += sum_j y_j z_j on cells the solve updates (slot flag set); one pair of
 // cells of one colour per thread (NC is even, so pairs never straddle the
 // two colour arrays); y_j = 0 (steps that did not run) skips z_j
 __global__ void __launch_bounds__(kSorBlock)
 k_update_pc(double *__restrict__ p0, double *__restrict__ p1, const float *__restrict__ Z, int64_t stride, int m,
-            const double *__restrict__ y, const float *__restrict__ mask, int64_t NC, const double *gate,
-            double thr) {
+            const double *__restrict__ y, const uint32_t *__restrict__ bits, int64_t nw, int64_t NC,
+            const double *gate, double thr) {
     if (gated(gate, thr)) return;
     int nz = 0;  // steps after the last non-zero y add nothing
     for (int j = 0; j < m; j++)
@@ -1261,8 +1281,9 @@ k_update_pc(double *__restrict__ p0, double *__restrict__ p1, const float *__res
     for (int64_t q2 = (int64_t)blockIdx.x * kSorBlock + threadIdx.x; q2 < npair;
          q2 += (int64_t)gridDim.x * kSorBlock) {
         const int64_t q = 2 * q2;
-        const f2v mk = *reinterpret_cast<const f2v *>(mask + q);
-        const bool ok0 = mk.x == mk.x, ok1 = mk.y == mk.y;
+        const int c = q >= NC ? 1 : 0;
+        bool ok0, ok1;
+        active_pair(bits, nw, c, (int)(q - c * NC), ok0, ok1);
         if (!(ok0 || ok1)) continue;
         double d0 = 0.0, d1 = 0.0;
         for (int j = 0; j < nz; j++) {
@@ -2695,7 +2716,8 @@ static int gmres_cycle(Ctx &c, int m, int k, const double *gate, double target, 
         }
     }
     hipLaunchKernelGGL(k_lsq, dim3(1), dim3(64), 0, c.stream, H, m, gv, act, y, c.scalars + 24);
-    hipLaunchKernelGGL(k_update_pc, vb, tb, 0, c.stream, c.pc[0], c.pc[1], g_kz, stride, m, y, V0, NC, act, 0.5);
+    hipLaunchKernelGGL(k_update_pc, vb, tb, 0, c.stream, c.pc[0], c.pc[1], g_kz, stride, m, y,
+                       (const uint32_t *)g_actbits, g_actbits_nw, NC, act, 0.5);
     return 0;
 }
 
@@ -2843,8 +2865,9 @@ static int sor_gmres(Ctx &c, double bn, int64_t *sweeps, int64_t *outer, double 
     if (ensure_slots(2 * (max_outer + 2))) return -1;
     double *relv = c.slots, *conv = c.slots + c.n_slots / 2;
     // (c.scalars[24]: the Arnoldi steps run in this solve, cleared here)
-    hipLaunchKernelGGL(k_clear_dead, dim3(nblocks(2 * c.NC, kSorBlock)), dim3(kSorBlock), 0, c.stream, c.pc[0],
-                       c.pc[1], c.e32[0], c.e32[1], c.bdt[0], c.bdt[1], c.NC, c.scalars + 24);
+    hipLaunchKernelGGL(k_clear_dead_bits, dim3(nblocks(2 * g_actbits_nw, kSorBlock)), dim3(kSorBlock), 0, c.stream,
+                       c.pc[0], c.pc[1], c.e32[0], c.e32[1], (const uint32_t *)g_actbits, g_actbits_nw, c.NC,
+                       c.scalars + 24);
     const bool cnn = cnn_active();
     if (cnn) {
         g_bn = bn;

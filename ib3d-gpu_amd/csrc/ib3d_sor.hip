@@ -2094,6 +2094,71 @@ __device__ void blk_sweep(const MgView &L, int c, bool first, float om) {
     __syncthreads();
 }
 
+// The same sweeps with each thread's share of the level held in registers:
+// a thread owns slots f = tid + 1024 t (t < kMgSlots) of each colour, and
+// what does not change while the level is smoothed (the slot, its
+// right-hand side, the neighbour offset, the six off-diagonal coefficients
+// and the reciprocal of the diagonal) is loaded once per visit of the level
+// instead of once per half-sweep.  The arithmetic is blk_sweep's, operand
+// for operand.
+static constexpr int kMgSlots = 2;   // slots per colour and thread: NC <= 2048 (larger levels: blk_sweep)
+struct MgSlot {
+    int f, s;                        // slot (-1: none or not a cell), ii & 1
+    float rr, a1x, a3x, a1y, a3y, a1z, a3z, rd;  // rd = rcp((a2x + a2y) + a2z)
+};
+__device__ __forceinline__ void blk_cache(const MgView &L, MgSlot (&sl)[2][kMgSlots]) {
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int t = 0; t < kMgSlots; t++) {
+            MgSlot &q = sl[c][t];
+            const int f = (int)threadIdx.x + t * kMgSmallThreads;
+            q.f = -1;
+            if (f >= L.NC) continue;
+            const float rr = L.b[(int64_t)c * L.NC + f];
+            if (!(rr == rr)) continue;
+            int ii, jj, kk;
+            view_decode(L, f, c, ii, jj, kk);
+            q.f = f;
+            q.s = ii & 1;
+            q.rr = rr;
+            q.a1x = L.ax[ii];
+            q.a3x = L.ax[2 * L.sxp + ii];
+            q.a1y = L.ay[jj];
+            q.a3y = L.ay[2 * L.sy + jj];
+            q.a1z = L.az[kk];
+            q.a3z = L.az[2 * L.sz + kk];
+            q.rd = __builtin_amdgcn_rcpf((L.ax[L.sxp + ii] + L.ay[L.sy + jj]) + L.az[L.sz + kk]);
+        }
+}
+__device__ __forceinline__ void blk_sweep_cached(const MgView &L, const MgSlot (&sl)[kMgSlots], int c, bool first,
+                                                 float om) {
+    float *mine = L.x + (int64_t)c * L.NC;
+    const float *o = L.x + (int64_t)(1 - c) * L.NC;
+    const int plane = L.H * L.sy;
+    const bool no_other = first && c == 0;
+#pragma unroll
+    for (int t = 0; t < kMgSlots; t++) {
+        const MgSlot &q = sl[t];
+        if (q.f < 0) continue;
+        const int f = q.f;
+        float pw = 0.0f, pe = 0.0f, ps = 0.0f, pn = 0.0f, pb = 0.0f, pt = 0.0f;
+        if (!no_other) {
+            pw = o[f - 1 + q.s];
+            pe = o[f + q.s];
+            ps = o[f - L.H];
+            pn = o[f + L.H];
+            pb = o[f - plane];
+            pt = o[f + plane];
+        }
+        const float pold = first ? 0.0f : mine[f];
+        const float gs =
+            ((((((q.rr - q.a1x * pw) - q.a3x * pe) - q.a1y * ps) - q.a3y * pn) - q.a1z * pb) - q.a3z * pt) * q.rd;
+        mine[f] = (1.0f - om) * pold + om * gs;
+    }
+    __syncthreads();
+}
+
 __device__ void blk_resid_restrict(const MgView &F, const MgView &C) {
     for (int q = threadIdx.x; q < 2 * C.NC; q += blockDim.x) {
         const int cc = q >= C.NC ? 1 : 0;
@@ -2204,28 +2269,33 @@ __global__ void __launch_bounds__(kMgSmallThreads) k_mg_small(MgSmall S, const d
             if (w0 + tid + t * nt < words) *d[t] = v[t];
     }
     __syncthreads();
+    // n sweeps on level L (the first from zero when `first`), from registers
+    // when the level's slots fit kMgSlots per thread
+    MgSlot sl[2][kMgSlots];
+    auto smooth = [&](const MgView &L, int n, bool first, float om) {
+        if (L.NC <= kMgSlots * kMgSmallThreads) {
+            blk_cache(L, sl);
+            for (int s = 0; s < n; s++) {
+                blk_sweep_cached(L, sl[0], 0, first && s == 0, om);
+                blk_sweep_cached(L, sl[1], 1, first && s == 0, om);
+            }
+        } else {
+            for (int s = 0; s < n; s++) {
+                blk_sweep(L, 0, first && s == 0, om);
+                blk_sweep(L, 1, first && s == 0, om);
+            }
+        }
+    };
     for (int l = 0; l + 1 < S.n; l++) {
         const MgView F = lds_view(S, l, lds);
-        for (int s = 0; s < S.nu1; s++) {
-            blk_sweep(F, 0, s == 0, S.om);
-            blk_sweep(F, 1, s == 0, S.om);
-        }
+        smooth(F, S.nu1, true, S.om);
         blk_resid_restrict(F, lds_view(S, l + 1, lds));
     }
-    {
-        const MgView C = lds_view(S, S.n - 1, lds);
-        for (int s = 0; s < S.ncoarse; s++) {
-            blk_sweep(C, 0, s == 0, S.omc);
-            blk_sweep(C, 1, s == 0, S.omc);
-        }
-    }
+    smooth(lds_view(S, S.n - 1, lds), S.ncoarse, true, S.omc);
     for (int l = S.n - 2; l >= 0; l--) {
         const MgView F = lds_view(S, l, lds);
         blk_prolong(F, lds_view(S, l + 1, lds));
-        for (int s = 0; s < S.nu2; s++) {
-            blk_sweep(F, 0, false, S.om);
-            blk_sweep(F, 1, false, S.om);
-        }
+        smooth(F, S.nu2, false, S.om);
     }
     const MgView V0 = lds_view(S, 0, lds);
     for (int q = threadIdx.x; q < 2 * V0.NC; q += blockDim.x) S.L[0].x[q] = V0.x[q];

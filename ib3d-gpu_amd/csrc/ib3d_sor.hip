@@ -1980,10 +1980,18 @@ k_mg_prolong_pz(MgView F, MgView C, SorGeom gf, const uint32_t *__restrict__ bit
     if (!(p < gf.plane && jj >= 1 && jj <= gf.sy - 2)) return;
     const int kk = 1 + by;
     const int f = p + kk * gf.plane;
-    // the active cells: the slot flags of F.b
+    // the active cells: the slot flags of F.b (bits), or F.b itself
     bool a0x, a0y, a1x, a1y;
-    active_pair(bits, nw, 0, f, a0x, a0y);
-    active_pair(bits, nw, 1, f, a1x, a1y);
+    if (bits) {
+        active_pair(bits, nw, 0, f, a0x, a0y);
+        active_pair(bits, nw, 1, f, a1x, a1y);
+    } else {
+        const f2v b0 = *reinterpret_cast<const f2v *>(F.b + f), b1 = *reinterpret_cast<const f2v *>(F.b + F.NC + f);
+        a0x = b0.x == b0.x;
+        a0y = b0.y == b0.y;
+        a1x = b1.x == b1.x;
+        a1y = b1.y == b1.y;
+    }
     if (!(a0x || a0y || a1x || a1y)) return;
     const MgAxis ay = F.ty[jj], az = F.tz[kk];
     const int J[2] = {ay.i0, ay.i1}, K[2] = {az.i0, az.i1};
@@ -2538,6 +2546,7 @@ static dim3 level_grid(const MgLevel &L) {
 static int g_fused = 2;          // levels (from the finest) smoothed by whole sweeps in one launch
 static int g_coarse_tiles = 2;   // first level smoothed by two temporally blocked launches per phase (0: none)
 static int g_fused_restrict = 64;  // levels (from the finest) whose residual is restricted in one pass
+static int g_prolong_pz = 2;       // levels (from the finest) prolonged four cells of a row per thread
 static int g_sweep_tz = 0;        // planes per chunk on the finest level (0: from g_sweep_blocks)
 static int g_sweep_blocks = 480;
 static int g_sweep_blocks_coarse = 192;  // workgroup target below the finest level
@@ -2684,10 +2693,10 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
     }
     for (int l = top - 1; l >= 0; l--) {
         const MgLevel &F = g_mg[l], &C = g_mg[l + 1];
-        if (l == 0)
+        if (l < g_prolong_pz)
             hipLaunchKernelGGL(k_mg_prolong_pz, dim3(nblocks((F.H * F.sy) / 2, kSorBlock), (unsigned)(F.sz - 2)), tb,
-                               0, c.stream, level_view(F), level_view(C), level_geom(F), (const uint32_t *)g_actbits,
-                               g_actbits_nw, gate, 0.5);
+                               0, c.stream, level_view(F), level_view(C), level_geom(F),
+                               l == 0 ? (const uint32_t *)g_actbits : nullptr, g_actbits_nw, gate, 0.5);
         else
             hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(2 * F.NC, kSorBlock)), tb, 0, c.stream, level_view(F),
                                level_view(C), level_geom(F), gate, 0.5);

@@ -1846,8 +1846,9 @@ k_mg_resid_restrict(MgView F, MgView C, SorGeom gc, const double *gate, double t
 // planes K.  Per fine plane it reads the rows of its four children
 // (ii = 2I-1, 2I; jj = 2J-1, 2J), their x and y neighbours and b: across a
 // wave every such read is one colour array at consecutive slots (the colour
-// of ii = 2I-1 + a does not depend on I), so the rows coalesce.  The x values
-// of the two fine planes below come along in registers.  The residuals use
+// of ii = 2I-1 + a does not depend on I), so the rows coalesce.  The
+// children's x values on the plane below the pair and on the pair's first
+// plane come from the previous step in registers.  The residuals use
 // cell_resid's operations and the children are summed in restrict_at's order,
 // so the coarse right-hand side is bit for bit the two-pass one.  Every
 // coarse slot is written (NaN outside the interior), as k_mg_resid_restrict
@@ -1899,8 +1900,8 @@ k_resid_restrict_pz(MgView F, MgView C, FastDiv dW, int kch, const double *gate,
         const float ax = (((((a1x * w + a3x * e) + a1y * s) + a3y * n) + a1z * b) + a3z * a) + ((a2x + a2y) + a2z) * own;
         return rb - ax;
     };
-    // the children's x on the two fine planes below the current pair of
-    // planes (ka = 2K-1, kb = 2K), rows j0 and j0 + 1
+    // the children's x (rows j0 and j0 + 1) on plane ka - 1 (m) and on plane
+    // ka (a) of the current pair of planes (ka = 2K-1, kb = 2K)
     Pair m0 = load_pair(F.x, j0, 2 * K0 - 2), m1 = load_pair(F.x, j0 + 1, 2 * K0 - 2);
     Pair a0 = load_pair(F.x, j0, 2 * K0 - 1), a1 = load_pair(F.x, j0 + 1, 2 * K0 - 1);
     for (int K = K0; K < K1; K++) {

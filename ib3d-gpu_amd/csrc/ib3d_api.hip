@@ -962,6 +962,7 @@ int ib3d_set_multigrid(int nu1, int nu2, double omega, int coarse_sweeps, double
     IB3D_REQUIRE(max_levels == 0 || max_levels >= 2, "set_multigrid: max_levels must be 0 (no cap) or >= 2");
     c.mg_nu1 = c.mg_nu1c = nu1;
     c.mg_nu2 = c.mg_nu2c = nu2;
+    for (auto &lv : c.mg_nu_lv) lv[0] = lv[1] = -1;
     c.mg_omega = omega;
     c.mg_coarse = coarse_sweeps;
     c.mg_coarse_omega = coarse_omega;
@@ -982,6 +983,16 @@ int ib3d_set_multigrid_coarse(int nu1, int nu2) {
     IB3D_REQUIRE(nu2 >= 0 && nu2 <= 64, "set_multigrid_coarse: post-smoothing sweeps must be 0..64");
     c.mg_nu1c = nu1;
     c.mg_nu2c = nu2;
+    return 0;
+}
+
+int ib3d_set_multigrid_level(int level, int nu1, int nu2) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE(level >= 1 && level <= 7, "set_multigrid_level: level must be 1..7");
+    IB3D_REQUIRE((nu1 == -1 && nu2 == -1) || (nu1 >= 1 && nu1 <= 64 && nu2 >= 0 && nu2 <= 64),
+                 "set_multigrid_level: sweeps must be 1..64 before and 0..64 after (-1, -1: no override)");
+    c.mg_nu_lv[level][0] = nu1;
+    c.mg_nu_lv[level][1] = nu2;
     return 0;
 }
 

@@ -162,6 +162,9 @@ struct Ctx {
     double krylov_target = 0.3;    // modes 3, 4: a cycle stops once its residual estimate is below this x tol x ||b/dt||
     int mg_nu1 = 4, mg_nu2 = 4;    // mode 4: pre- and post-smoothing sweeps per level
     int mg_nu1c = 4, mg_nu2c = 4;  // mode 4: the same on the levels below the finest
+    // mode 4: per-level overrides of (nu1c, nu2c) for levels 1..7 cycled by
+    // launches (-1: none)
+    int mg_nu_lv[8][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};
     double mg_omega = 1.55;        // mode 4: smoother relaxation factor
     int mg_coarse = 4;             // mode 4: sweeps on the coarsest level
     double mg_coarse_omega = 1.5;  // mode 4: relaxation factor there

@@ -2646,6 +2646,14 @@ static MgView level_view(const MgLevel &L) {
     return V;
 }
 
+// pre- (post = 0) or post-smoothing (post = 1) sweeps on level l cycled by
+// launches: the finest level's, a per-level override, or the coarse levels'
+static int level_sweeps(const Ctx &c, int l, int post) {
+    if (l == 0) return post ? c.mg_nu2 : c.mg_nu1;
+    if (l < 8 && c.mg_nu_lv[l][post] >= 0) return c.mg_nu_lv[l][post];
+    return post ? c.mg_nu2c : c.mg_nu1c;
+}
+
 // z = M^-1 v: one V-cycle from a zero guess (v and z are 2*NC fine vectors).
 // Levels from g_mg_small on are cycled by one single-workgroup launch.
 static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
@@ -2656,7 +2664,7 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
     const int top = std::min(ns, nl - 1);  // levels [0, top) restrict through launches
     for (int l = 0; l < top; l++) {
         const MgLevel &F = g_mg[l], &C = g_mg[l + 1];
-        mg_smooth(c, F, l == 0, l == 0 ? c.mg_nu1 : c.mg_nu1c, true, c.mg_omega, gate);
+        mg_smooth(c, F, l == 0, level_sweeps(c, l, 0), true, c.mg_omega, gate);
         if (l < g_fused_restrict) {
             const int W = (int)(2 * C.H), nbx = nblocks(W * C.sy, kSorBlock), nk = (int)C.sz - 2;
             // coarse planes per chunk: about 1536 workgroups in all
@@ -2701,7 +2709,7 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
         else
             hipLaunchKernelGGL(k_mg_prolong, dim3(nblocks(2 * F.NC, kSorBlock)), tb, 0, c.stream, level_view(F),
                                level_view(C), level_geom(F), gate, 0.5);
-        mg_smooth(c, F, l == 0, l == 0 ? c.mg_nu2 : c.mg_nu2c, false, c.mg_omega, gate);
+        mg_smooth(c, F, l == 0, level_sweeps(c, l, 1), false, c.mg_omega, gate);
     }
 }
 
@@ -2906,6 +2914,10 @@ static int cycle_graph(Ctx &c, int m, int k, double *relv, double *conv) {
     key.push_back((uint64_t)c.mg_nu2);
     key.push_back((uint64_t)c.mg_nu1c);
     key.push_back((uint64_t)c.mg_nu2c);
+    for (const auto &lv : c.mg_nu_lv) {
+        key.push_back((uint64_t)(int64_t)lv[0]);
+        key.push_back((uint64_t)(int64_t)lv[1]);
+    }
     key.push_back((uint64_t)g_fused);
     key.push_back((uint64_t)g_coarse_tiles);
     key.push_back((uint64_t)g_fused_restrict);

@@ -67,6 +67,7 @@ def lib():
         "ib3d_set_krylov": [I],
         "ib3d_set_multigrid": [I, I, D, I, D, I],
         "ib3d_set_multigrid_coarse": [I, I],
+        "ib3d_set_multigrid_level": [I, I, I],
         "ib3d_set_krylov_target": [D],
         "ib3d_set_fused_sweeps": [I, I, I, I],
         "ib3d_set_coarse_tiles": [I],
@@ -330,6 +331,11 @@ class Solver:
     def set_multigrid_coarse(self, nu1, nu2):
         """Smoothing sweeps on the levels below the finest (after set_multigrid)."""
         _ok(self.L.ib3d_set_multigrid_coarse(int(nu1), int(nu2)))
+
+    def set_multigrid_level(self, level, nu1=-1, nu2=-1):
+        """Smoothing sweeps on one multigrid level (1..7) cycled by launches,
+        overriding the coarse levels' counts; (-1, -1) removes the override."""
+        _ok(self.L.ib3d_set_multigrid_level(int(level), int(nu1), int(nu2)))
 
     def set_graphs(self, on=True):
         """Replay each GMRES-IR iteration as a captured HIP graph or launch its

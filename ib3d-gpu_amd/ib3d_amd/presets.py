@@ -23,6 +23,12 @@ GMRES_SOR = {"m": 12, "k": 20, "omega": 1.9}
 # post-smoothing sweeps per level and their omega, sweeps and omega on the
 # coarsest level
 MULTIGRID = {"m": 6, "nu1": 4, "nu2": 4, "omega": 1.55, "coarse": 4, "coarse_omega": 1.5}
+# sweeps before and after on the second level (one eighth of the finest),
+# where each sweep is still a launch of real work; None: the coarse levels'
+# counts.  At 256x192x192, (3, 2) gives the whole 500-step run 1.4% more
+# steps/s but the steps right after the impulsive start 1.4% fewer, and
+# (3, 3) costs 1% at 512x320x320, so the default keeps 4+4.
+LEVEL1 = None
 # how the smoothing is launched (identical results either way): whole sweeps
 # in one launch on the first `fused` levels, two temporally blocked launches
 # per smoothing phase from level `tiles` on
@@ -50,6 +56,8 @@ def apply(solver, mode, omega=1.99, tol=1e-6, inner=20, gmres=None, multigrid=No
         solver.set_poisson_mode(4, tol=tol, warm=True)
         solver.set_krylov(mg["m"])
         solver.set_multigrid(mg["nu1"], mg["nu2"], mg["omega"], mg["coarse"], mg["coarse_omega"], 0)
+        if LEVEL1 is not None:
+            solver.set_multigrid_level(1, *LEVEL1)
         solver.set_fused_sweeps(LAUNCH["fused"])
         solver.set_coarse_tiles(LAUNCH["tiles"])
     else:

@@ -179,6 +179,12 @@ int ib3d_set_krylov_target(double target);
  * Those levels are small enough that a sweep costs about a launch, whatever
  * the level's size. */
 int ib3d_set_multigrid_coarse(int nu1, int nu2);
+/* mode 4: pre- and post-smoothing sweeps on one level (1..7) cycled by
+ * launches, overriding the coarse levels' counts there; (-1, -1) removes the
+ * override.  ib3d_set_multigrid removes every override.  A sweep of level 1
+ * (one eighth of the finest level) is still one launch of real work, so it
+ * may pay to sweep it less than the smaller levels. */
+int ib3d_set_multigrid_level(int level, int nu1, int nu2);
 /* mode 4: smooth the first `levels` multigrid levels (2, the default: the
  * finest and the next) with whole red-black sweeps in one launch each, the
  * others with two half-sweep launches (0: none), unless they are tiled

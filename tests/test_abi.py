@@ -80,6 +80,13 @@ def test_solver_option_entry_points_validate_their_arguments():
     for args, word in cases:
         assert lib.ib3d_set_multigrid(*args) != 0
         assert word in _err(lib), (args, _err(lib))
+    lib.ib3d_set_multigrid_level.argtypes = [I, I, I]
+    assert lib.ib3d_set_multigrid_level(1, 3, 2) == 0
+    assert lib.ib3d_set_multigrid_level(1, -1, -1) == 0
+    for args, word in [((0, 3, 2), "level"), ((8, 3, 2), "level"), ((1, 0, 2), "sweeps"), ((1, 3, -1), "sweeps"),
+                       ((1, 65, 2), "sweeps")]:
+        assert lib.ib3d_set_multigrid_level(*args) != 0
+        assert word in _err(lib), (args, _err(lib))
     assert lib.ib3d_cnn_set_accept(0.5) == 0
     for bad in (0.0, 1.5, -1.0):
         assert lib.ib3d_cnn_set_accept(bad) != 0

@@ -24,7 +24,8 @@ Poisson modes (same stopping rule: ||b/dt - A p||_2 <= 1e-6 ||b/dt||_2):
   mixed         -- the same FP64 refinement and FP32 flexible GMRES(6),
                    every Arnoldi step preconditioned by one FP32 geometric
                    multigrid V-cycle (red-black SOR smoothing, 4+4 sweeps per
-                   level at omega 1.55, trilinear prolongation),
+                   level at omega 1.55, 5+4 on the levels cycled inside one
+                   workgroup, trilinear prolongation),
                    previous-step guess (headline)
   mixed+learned -- mixed, starting from the CNN's guess when its residual
                    is below half the previous-step guess's
@@ -73,6 +74,7 @@ from ib3d_amd import presets  # noqa: E402
 
 GMRES = dict(presets.GMRES_SOR)
 MG = presets.MULTIGRID
+SMALL = presets.SMALL
 
 
 def log(msg):
@@ -562,7 +564,10 @@ def main():
                                    f"{n_tris} triangles, dt={cfg.dt}",
                        "poisson": (f"{a.mode}: FP64 refinement, FP32 flexible GMRES({MG['m']}) preconditioned by "
                                    f"one FP32 multigrid V-cycle (RB-SOR {MG['nu1']}+{MG['nu2']} sweeps per level at "
-                                   f"omega={MG['omega']}, {MG['coarse']} on the coarsest, trilinear prolongation)"
+                                   f"omega={MG['omega']}"
+                                   + (f", {SMALL[0]}+{SMALL[1]} on the levels cycled inside one workgroup"
+                                      if SMALL else "")
+                                   + f", {MG['coarse']} on the coarsest, trilinear prolongation)"
                                    if a.mode in ("mixed", "mixed+learned") else
                                    f"{a.mode}: FP64 refinement, FP32 flexible GMRES({GMRES['m']}) preconditioned by "
                                    f"{GMRES['k']} FP32 RB-SOR sweeps at omega={GMRES['omega']}"

@@ -963,6 +963,7 @@ int ib3d_set_multigrid(int nu1, int nu2, double omega, int coarse_sweeps, double
     c.mg_nu1 = c.mg_nu1c = nu1;
     c.mg_nu2 = c.mg_nu2c = nu2;
     for (auto &lv : c.mg_nu_lv) lv[0] = lv[1] = -1;
+    c.mg_nu_small[0] = c.mg_nu_small[1] = -1;
     c.mg_omega = omega;
     c.mg_coarse = coarse_sweeps;
     c.mg_coarse_omega = coarse_omega;
@@ -993,6 +994,15 @@ int ib3d_set_multigrid_level(int level, int nu1, int nu2) {
                  "set_multigrid_level: sweeps must be 1..64 before and 0..64 after (-1, -1: no override)");
     c.mg_nu_lv[level][0] = nu1;
     c.mg_nu_lv[level][1] = nu2;
+    return 0;
+}
+
+int ib3d_set_multigrid_small(int nu1, int nu2) {
+    Ctx &c = ctx();
+    IB3D_REQUIRE((nu1 == -1 && nu2 == -1) || (nu1 >= 1 && nu1 <= 64 && nu2 >= 0 && nu2 <= 64),
+                 "set_multigrid_small: sweeps must be 1..64 before and 0..64 after (-1, -1: the coarse counts)");
+    c.mg_nu_small[0] = nu1;
+    c.mg_nu_small[1] = nu2;
     return 0;
 }
 

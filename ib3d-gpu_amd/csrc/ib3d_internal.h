@@ -165,6 +165,9 @@ struct Ctx {
     // mode 4: per-level overrides of (nu1c, nu2c) for levels 1..7 cycled by
     // launches (-1: none)
     int mg_nu_lv[8][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};
+    // mode 4: sweeps before and after on the levels smoothed inside the
+    // one-workgroup kernel (-1: nu1c, nu2c)
+    int mg_nu_small[2] = {-1, -1};
     double mg_omega = 1.55;        // mode 4: smoother relaxation factor
     int mg_coarse = 4;             // mode 4: sweeps on the coarsest level
     double mg_coarse_omega = 1.5;  // mode 4: relaxation factor there

@@ -2684,8 +2684,8 @@ static void mg_vcycle(Ctx &c, const float *v, float *z, const double *gate) {
     if (ns < nl) {
         MgSmall S;
         S.n = nl - ns;
-        S.nu1 = c.mg_nu1c;
-        S.nu2 = c.mg_nu2c;
+        S.nu1 = c.mg_nu_small[0] >= 0 ? c.mg_nu_small[0] : c.mg_nu1c;
+        S.nu2 = c.mg_nu_small[1] >= 0 ? c.mg_nu_small[1] : c.mg_nu2c;
         S.ncoarse = c.mg_coarse;
         S.om = (float)c.mg_omega;
         S.omc = (float)c.mg_coarse_omega;
@@ -2918,6 +2918,8 @@ static int cycle_graph(Ctx &c, int m, int k, double *relv, double *conv) {
         key.push_back((uint64_t)(int64_t)lv[0]);
         key.push_back((uint64_t)(int64_t)lv[1]);
     }
+    key.push_back((uint64_t)(int64_t)c.mg_nu_small[0]);
+    key.push_back((uint64_t)(int64_t)c.mg_nu_small[1]);
     key.push_back((uint64_t)g_fused);
     key.push_back((uint64_t)g_coarse_tiles);
     key.push_back((uint64_t)g_fused_restrict);

@@ -68,6 +68,7 @@ def lib():
         "ib3d_set_multigrid": [I, I, D, I, D, I],
         "ib3d_set_multigrid_coarse": [I, I],
         "ib3d_set_multigrid_level": [I, I, I],
+        "ib3d_set_multigrid_small": [I, I],
         "ib3d_set_krylov_target": [D],
         "ib3d_set_fused_sweeps": [I, I, I, I],
         "ib3d_set_coarse_tiles": [I],
@@ -336,6 +337,11 @@ class Solver:
         """Smoothing sweeps on one multigrid level (1..7) cycled by launches,
         overriding the coarse levels' counts; (-1, -1) removes the override."""
         _ok(self.L.ib3d_set_multigrid_level(int(level), int(nu1), int(nu2)))
+
+    def set_multigrid_small(self, nu1=-1, nu2=-1):
+        """Smoothing sweeps on the coarsest levels, those smoothed inside the
+        one-workgroup kernel; (-1, -1): the coarse levels' counts."""
+        _ok(self.L.ib3d_set_multigrid_small(int(nu1), int(nu2)))
 
     def set_graphs(self, on=True):
         """Replay each GMRES-IR iteration as a captured HIP graph or launch its

@@ -13,7 +13,8 @@ Every L2 preset stops at ||b/dt - A pc||_2 <= tol ||b/dt||_2, checked in FP64.
                 preconditioned by 20 FP32 red-black SOR sweeps at omega 1.9
   mixed      -- FP64 refinement, FP32 flexible GMRES(6), every Arnoldi step
                 preconditioned by one FP32 multigrid V-cycle (4+4 red-black
-                SOR sweeps per level at omega 1.55, 4 on the coarsest)
+                SOR sweeps per level at omega 1.55, 5+4 on the levels cycled
+                inside one workgroup, 4 on the coarsest)
 """
 
 # FP32 GMRES with SOR preconditioning: Arnoldi steps per refinement step,
@@ -29,6 +30,13 @@ MULTIGRID = {"m": 6, "nu1": 4, "nu2": 4, "omega": 1.55, "coarse": 4, "coarse_ome
 # steps/s but the steps right after the impulsive start 1.4% fewer, and
 # (3, 3) costs 1% at 512x320x320, so the default keeps 4+4.
 LEVEL1 = None
+# sweeps before and after on the coarsest levels, those cycled inside one
+# workgroup (16x12x12 and below at 256x192x192), where a sweep costs a
+# barrier rather than a launch.  Poisson ms per step against 4+4, one box:
+# 4.296 / 4.346 at 256x192x192 (steps 21-100), 4.620 / 4.669 over its
+# 500-step run, 10.381 / 10.406 at 384x256x256, 29.85 / 29.99 at
+# 512x320x320; 6+6 was better on the last two but not at 256x192x192.
+SMALL = (5, 4)
 # how the smoothing is launched (identical results either way): whole sweeps
 # in one launch on the first `fused` levels, two temporally blocked launches
 # per smoothing phase from level `tiles` on
@@ -58,6 +66,8 @@ def apply(solver, mode, omega=1.99, tol=1e-6, inner=20, gmres=None, multigrid=No
         solver.set_multigrid(mg["nu1"], mg["nu2"], mg["omega"], mg["coarse"], mg["coarse_omega"], 0)
         if LEVEL1 is not None:
             solver.set_multigrid_level(1, *LEVEL1)
+        if SMALL is not None:
+            solver.set_multigrid_small(*SMALL)
         solver.set_fused_sweeps(LAUNCH["fused"])
         solver.set_coarse_tiles(LAUNCH["tiles"])
     else:

@@ -185,6 +185,11 @@ int ib3d_set_multigrid_coarse(int nu1, int nu2);
  * (one eighth of the finest level) is still one launch of real work, so it
  * may pay to sweep it less than the smaller levels. */
 int ib3d_set_multigrid_level(int level, int nu1, int nu2);
+/* mode 4: pre- and post-smoothing sweeps on the coarsest levels, those whose
+ * arrays together fit in 64 KB of LDS and are cycled inside one workgroup
+ * (-1, -1: the coarse levels' counts; ib3d_set_multigrid resets it).  Their
+ * sweeps cost a barrier each, not a launch, so more of them can pay. */
+int ib3d_set_multigrid_small(int nu1, int nu2);
 /* mode 4: smooth the first `levels` multigrid levels (2, the default: the
  * finest and the next) with whole red-black sweeps in one launch each, the
  * others with two half-sweep launches (0: none), unless they are tiled

@@ -87,6 +87,12 @@ def test_solver_option_entry_points_validate_their_arguments():
                        ((1, 65, 2), "sweeps")]:
         assert lib.ib3d_set_multigrid_level(*args) != 0
         assert word in _err(lib), (args, _err(lib))
+    lib.ib3d_set_multigrid_small.argtypes = [I, I]
+    assert lib.ib3d_set_multigrid_small(5, 4) == 0
+    assert lib.ib3d_set_multigrid_small(-1, -1) == 0
+    for args in ((0, 4), (5, -1), (65, 4), (-1, 4)):
+        assert lib.ib3d_set_multigrid_small(*args) != 0
+        assert "sweeps" in _err(lib), (args, _err(lib))
     assert lib.ib3d_cnn_set_accept(0.5) == 0
     for bad in (0.0, 1.5, -1.0):
         assert lib.ib3d_cnn_set_accept(bad) != 0

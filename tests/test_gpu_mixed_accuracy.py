@@ -9,7 +9,8 @@ The fixed sphere at 128x64x64 runs 40 steps three times on the GPU:
 - FP32 flexible GMRES preconditioned by FP32 RB-SOR inside FP64 refinement
   (GMRES-IR) to the same FP64 residual;
 - the same with one FP32 geometric-multigrid V-cycle as the preconditioner.
-All L2 modes must keep Cd within 0.5% of the reference's Cd series, and the
+Each L2 mode runs its preset (ib3d_amd.presets: fp64, mixed-dc, gmres-sor,
+mixed), which sets every option the solver keeps between solves.  All L2 modes must keep Cd within 0.5% of the reference's Cd series, and the
 field differences are printed (they are the per-configuration errors stated
 in the documentation).  The differences come from the stopping rule, not
 from FP32: the mixed and the FP64 L2 solutions agree far more closely with
@@ -32,20 +33,14 @@ def _say(msg):
     sys.stdout.flush()
 
 
+PRESET = {"fp64": "fp64", "mixed": "mixed-dc", "gmres": "gmres-sor", "multigrid": "mixed"}
+
+
 def run(case, mode):
+    from ib3d_amd import presets
     from ib3d_amd.native import Solver
     s = Solver.from_case(case)
-    if mode == "fp64":
-        s.set_poisson_mode(1, omega=1.99, tol=1e-6, check=20, warm=True)
-    elif mode == "mixed":
-        s.set_poisson_mode(2, omega=1.99, tol=1e-6, check=20, warm=True)
-    elif mode == "gmres":
-        s.set_poisson_mode(3, omega=1.9, tol=1e-6, check=20, warm=True)
-        s.set_krylov(12)
-    elif mode == "multigrid":
-        s.set_poisson_mode(4, tol=1e-6, warm=True)
-        s.set_krylov(12)
-        s.set_multigrid(2, 2, 1.0, 16, 1.5, 0)
+    presets.apply(s, PRESET.get(mode, "reference"), omega=1.99, tol=1e-6, inner=20)
     cds, relres = [], []
     for _ in range(NSTEPS):
         r = s.step()

@@ -567,7 +567,8 @@ def main():
                                    f"omega={MG['omega']}"
                                    + (f", {SMALL[0]}+{SMALL[1]} on the levels cycled inside one workgroup"
                                       if SMALL else "")
-                                   + f", {MG['coarse']} on the coarsest, trilinear prolongation)"
+                                   + f", {MG['coarse']} on the coarsest, trilinear prolongation; a Krylov cycle "
+                                   f"hands back to the FP64 check at {MG['target']} x tol)"
                                    if a.mode in ("mixed", "mixed+learned") else
                                    f"{a.mode}: FP64 refinement, FP32 flexible GMRES({GMRES['m']}) preconditioned by "
                                    f"{GMRES['k']} FP32 RB-SOR sweeps at omega={GMRES['omega']}"
